@@ -1,0 +1,191 @@
+"""Benchmark: site-likelihood evaluations per second on one MI355X.
+
+Workload (DNA GTR+I+G4, 200 taxa x 100,000 sites, ~60k unique patterns, 2% gap /
+ambiguity cells, p_inv 0.2, Gamma alpha 0.5, Exp(0.1) branch lengths, seeded):
+one step = one full post-order traversal with the tree lnL, then one
+optimizeAllBranches sweep of Newton-Raphson over all 397 branches, starting
+from the same initial branch lengths every step.
+
+Unit: a site-likelihood evaluation is one unique pattern's likelihood under
+one whole tree; every tree-level evaluation (the traversal lnL, each Newton
+derivative evaluation, each lnL-from-buffer) counts the P patterns once.
+value = P * evaluations / second, FP64 form (the drop-in). The FP32-storage
+form's rate and its |delta lnL| against FP64 are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+FP64_PEAK = 78.6e12        # dense FP64, flop/s
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(ia, args):
+    return ia.Workload(seq_type=ia.DNA, ntaxa=args.taxa, nsite=args.sites, ncat=4, mean_len=0.1,
+                       alpha=0.5, p_inv=0.2, gap_frac=0.02, seed=20261202 + 1)
+
+
+def run_steps(ctx, tree, steps, profile=False):
+    evals = 0
+    lnls = []
+    if profile:
+        ctx.profile_start()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tree.reset()
+        lnl0 = tree.compute_likelihood()
+        lnl, nd = tree.optimize_all_branches(iterations=1, tolerance=1e-3, max_nr=100)
+        evals += 1 + 1 + nd + 1      # traversal lnL, sweep's initial lnL, derivatives, from-buffer
+        lnls.append((lnl0, lnl))
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_stop() if profile else None
+    return elapsed, evals, lnls, prof
+
+
+def time_traversal(ctx, tree, reps):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tree.reset()
+        lnl = tree.compute_likelihood()
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / reps, lnl
+
+
+def cpu_baseline(ia, w, budget_s, threads):
+    """Oracle (FP64 CPU restatement, AVX2/FMA, OpenMP) on a bounded sample of
+    the same step: the full traversal, then Newton on branches in the same
+    pre-order until the time budget is spent."""
+    from oracle.binding import OracleTree
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w, nthreads=threads)
+    branches = planner.branches()
+    evals = 0
+    t0 = time.perf_counter()
+    a0, b0 = branches[0]
+    ops, node, dad, length = planner.plan(b0, a0)
+    orc.likelihood_branch(ops, node, dad, length)
+    evals += 1
+    nbr = 0
+    for a, b in branches:
+        ops, node, dad, length = planner.plan(b, a)
+        x, _, nd, st = orc.optimize_branch(ops, node, dad, length)
+        evals += nd + 1
+        planner.set_length(a, b, x)
+        nbr += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    elapsed = time.perf_counter() - t0
+    return dict(value=w.npatterns * evals / elapsed, unit="site-lik evals/s", cores=threads,
+                kind="port",
+                sample=f"full traversal + Newton on the first {nbr} of {len(branches)} branches "
+                       f"({evals} tree evaluations x {w.npatterns} patterns, {elapsed:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--taxa", type=int, default=200)
+    ap.add_argument("--sites", type=int, default=100000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the likelihood path shards by partition/request; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import iqtree2_amd as ia
+    t0 = time.perf_counter()
+    w = make_workload(ia, args)
+    log(f"workload: {w.ntaxa} taxa, {w.nsite} sites, {w.npatterns} patterns "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    ctx = ia.Context(0)
+    tree = ia.PhyloTree(ctx, w, precision=ia.PREC_F64)
+    for i in range(args.warmup):
+        run_steps(ctx, tree, 1)
+        log(f"warmup step {i + 1} done")
+    elapsed, evals, lnls, prof = run_steps(ctx, tree, args.steps, profile=True)
+    P = w.npatterns
+    value = P * evals / elapsed
+    log(f"fp64: {args.steps} steps in {elapsed:.3f} s, {evals} tree evaluations, "
+        f"lnL {lnls[-1][0]:.6f} -> {lnls[-1][1]:.6f}")
+    trav_s, trav_lnl = time_traversal(ctx, tree, 5)
+
+    # dominant kernel and its roofline
+    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
+    achieved = dom["bytes"] / (dom["total_ms"] * 1e-3)
+    kernels = {k: dict(launches=v["launches"], ms=round(v["total_ms"], 3),
+                       avg_us=round(1e3 * v["total_ms"] / v["launches"], 2),
+                       gbps=round(v["bytes"] / max(v["total_ms"], 1e-9) * 1e-6, 1))
+               for k, v in prof.items()}
+    log(f"kernels: {json.dumps(kernels)}")
+
+    fp32 = None
+    if not args.no_fp32:
+        tree32 = ia.PhyloTree(ctx, w, precision=ia.PREC_F32)
+        run_steps(ctx, tree32, 1)
+        e32, ev32, l32, _ = run_steps(ctx, tree32, args.steps)
+        t32_s, t32_lnl = time_traversal(ctx, tree32, 5)
+        fp32 = dict(value=P * ev32 / e32, ms_per_step=1e3 * e32 / args.steps,
+                    traversal_ms=1e3 * t32_s,
+                    abs_dlnl_traversal=abs(t32_lnl - trav_lnl),
+                    abs_dlnl_after_newton=abs(l32[-1][1] - lnls[-1][1]),
+                    rel_dlnl_traversal=abs(t32_lnl - trav_lnl) / abs(trav_lnl))
+        log(f"fp32: {json.dumps(fp32)}")
+        tree32.close()
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(ia, w, args.cpu_seconds, threads)
+        log(f"cpu baseline: {json.dumps(cpu)}")
+
+    out = {
+        "metric": "site-lik evals/s FP64 vs batched FP32 at |dlnL|<=1e-6; p50/p99 req latency",
+        "value": value,
+        "unit": "site-lik evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded simulation down a random tree)",
+        "config": {"workload": "DNA GTR+I+G4, 200 taxa x 100,000 sites: full post-order "
+                               "traversal + Newton over all 397 branches",
+                   "taxa": w.ntaxa, "sites": w.nsite, "patterns": P,
+                   "tree_evals_per_step": evals / args.steps,
+                   "traversal_ms": 1e3 * trav_s,
+                   "traversal_site_lik_evals_per_s": P / trav_s},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": None, "avg_launch_us": 1e3 * dom["total_ms"] / dom["launches"],
+                     "bytes_per_launch": dom["bytes"] / dom["launches"]},
+        "kernels": kernels,
+        "fp32_storage": fp32,
+        "cpu_baseline": cpu,
+    }
+    tree.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,182 @@
+/*
+ * plk.h - C ABI of the MI355X Felsenstein-pruning likelihood engine.
+ *
+ * This is the drop-in boundary for IQ-TREE 2's likelihood kernel set: a
+ * PhyloTree kernel set registered as LK_GPU (tree/phylotreesse.cpp:91-199,
+ * tree/phylokernelfma.cpp:36-182) implements its four member-function
+ * pointers with these calls. Plain pointers and sizes only; every function
+ * returns a plk_status and never aborts the caller.
+ *
+ * Data contract (all host arrays FP64 unless stated):
+ *   n   = number of states, C = rate categories, block = C*n
+ *   P   = unique patterns (aln->size()), SU = STATE_UNKNOWN of the alignment
+ *   states      uint8 [ntaxa][P]      converted tip states (0..SU)
+ *   eval        [n]                   eigenvalues of Q
+ *   evec        [n][n]                U (row x, column i)
+ *   inv_evec    [n][n]                U^-1
+ *   tip_eigen   [SU+1][n]             U^-1 * indicator(state)
+ *   cat_rates   [C], cat_props [C]    site_rate->getRate / getProp
+ * Partial likelihoods live in device memory in "slots" chosen by the caller
+ * (a PhyloNeighbor's partial_lh maps to slot (ptr - base) / block), stored
+ * pattern-major [P][C][n] in the eigen basis, with one scale counter per
+ * pattern (or per pattern and category in safe mode).
+ */
+#ifndef PLK_H
+#define PLK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PLK_OK = 0,
+    PLK_ERR_ARG = 1,          /* invalid argument or shape */
+    PLK_ERR_NOMEM = 2,        /* HBM budget or allocation exhausted */
+    PLK_ERR_DEVICE = 3,       /* HIP runtime error */
+    PLK_ERR_NONFINITE = 4,    /* non-finite likelihood/derivative (see error text) */
+    PLK_ERR_STATE = 5,        /* call order violated (e.g. derivative without theta) */
+    PLK_ERR_UNSUPPORTED = 6,  /* configuration not built into this library */
+    PLK_ERR_NEWTON = 7        /* Newton-Raphson did not converge within max steps */
+} plk_status;
+
+typedef enum { PLK_PREC_F64 = 0, PLK_PREC_F32 = 1 } plk_precision;
+
+typedef struct plk_context plk_context;
+
+typedef struct {
+    int device;           /* HIP device ordinal */
+    void* stream;         /* hipStream_t to launch on; NULL = context-owned stream */
+    size_t hbm_budget;    /* bytes this context may allocate; 0 = 90% of free HBM */
+} plk_config;
+
+typedef struct {
+    int ntaxa;            /* leaves */
+    int nstates;          /* n */
+    int ncat;             /* C (1, 2, 4 or 8) */
+    int npatterns;        /* P */
+    int nslots;           /* partial-likelihood buffers */
+    int state_unknown;    /* SU */
+    int precision;        /* plk_precision */
+    int safe_numeric;     /* 1: per-category scale counters (tree/phylotreesse.cpp:95-96) */
+} plk_instance_desc;
+
+/* one child of a traversal step: a tip (taxon id) or a partial slot */
+typedef struct {
+    int is_tip;
+    int id;
+    double length;        /* branch length node -> child */
+} plk_child;
+
+#define PLK_MAX_CHILDREN 2
+
+/* TraversalInfo (tree/phylotree.h:320-331): compute slot dad_slot from its
+ * children. Steps are executed in list order (post-order). */
+typedef struct {
+    int dad_slot;
+    int nchildren;        /* 2 (binary nodes) */
+    plk_child child[PLK_MAX_CHILDREN];
+} plk_op;
+
+/* end of a branch: a tip (taxon) or a partial slot */
+typedef struct {
+    int is_tip;
+    int id;
+} plk_end;
+
+/* ---- context -------------------------------------------------------- */
+plk_status plk_create(const plk_config* cfg, plk_context** out);
+plk_status plk_destroy(plk_context* ctx);
+/* last error text of the calling thread (static storage, never NULL) */
+const char* plk_last_error(void);
+/* bytes currently allocated by the context and its budget */
+plk_status plk_memory(plk_context* ctx, size_t* used, size_t* budget);
+/* stream the context launches on (hipStream_t) */
+void* plk_stream(plk_context* ctx);
+plk_status plk_synchronize(plk_context* ctx);
+
+/* ---- instances (one PhyloTree's device state) ------------------------ */
+/* HBM bytes an instance of this shape needs (admission by footprint) */
+size_t plk_instance_footprint(const plk_instance_desc* desc);
+plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* desc, int* instance);
+plk_status plk_instance_destroy(plk_context* ctx, int instance);
+
+/* computeTipPartialLikelihood inputs: tip states, ptn_freq, ptn_invar
+ * (tree/phylotreesse.cpp:243-376, 537-617) */
+plk_status plk_set_patterns(plk_context* ctx, int instance, const uint8_t* states,
+                            const double* freq, const double* invar);
+/* model eigensystem and rate categories (model/modelmarkov.cpp:1389-1602,
+ * model/rategamma.cpp:98-171) */
+plk_status plk_set_model(plk_context* ctx, int instance, const double* eval,
+                         const double* evec, const double* inv_evec, const double* tip_eigen,
+                         const double* cat_rates, const double* cat_props);
+
+/* ---- kernels ---------------------------------------------------------- */
+/* computePartialLikelihood over a post-order traversal list
+ * (tree/phylokernelnew.h:1289-2039, with computePartialInfo 895-1110) */
+plk_status plk_update_partials(plk_context* ctx, int instance, const plk_op* ops, int nops);
+
+/* computeLikelihoodBranch (tree/phylokernelnew.h:2641-3223): run ops, then
+ * the branch lnL between node (a slot) and dad (slot or tip) at `length`.
+ * pattern_lh (nullable, host [P]) receives log-likelihood per pattern. */
+plk_status plk_likelihood_branch(plk_context* ctx, int instance, const plk_op* ops, int nops,
+                                 plk_end node, plk_end dad, double length, double* lnl,
+                                 double* pattern_lh);
+
+/* computeLikelihoodBufferSIMD (tree/phylokernelnew.h:2048-2231): run ops,
+ * then theta = partial(node) * partial(dad) and buffer_scale_all. */
+plk_status plk_likelihood_buffer(plk_context* ctx, int instance, const plk_op* ops, int nops,
+                                 plk_end node, plk_end dad);
+
+/* computeLikelihoodDerv on the theta buffer (tree/phylokernelnew.h:2233-2633):
+ * +dlnL/dt and +d2lnL/dt2 at `length`; lnl (nullable) gets lnL at `length`. */
+plk_status plk_likelihood_derv(plk_context* ctx, int instance, double length, double* df,
+                               double* ddf, double* lnl);
+
+/* computeLikelihoodFromBuffer (tree/phylokernelnew.h:3232-3428) */
+plk_status plk_likelihood_from_buffer(plk_context* ctx, int instance, double length,
+                                      double* lnl, double* pattern_lh);
+
+/* optimizeOneBranch with Newton-Raphson (tree/phylotree.cpp:2594-2644,
+ * utils/optimization.cpp:422-499): theta for (node, dad) after ops, then
+ * minimizeNewton(min_len, length, max_len, xacc=min_len, max_nr); reports the
+ * optimised length, lnL at it, and the number of derivative evaluations. */
+plk_status plk_optimize_branch(plk_context* ctx, int instance, const plk_op* ops, int nops,
+                               plk_end node, plk_end dad, double length, double min_len,
+                               double max_len, int max_nr, double* out_length, double* out_lnl,
+                               int* n_derv);
+
+/* RELL (tree/iqtree.cpp:3568-3625, tree/phylokernel.h:64-70): out[b] =
+ * sum_p weights[b][p] * pattern_lh[p] over the last evaluated pattern_lh. */
+plk_status plk_rell(plk_context* ctx, int instance, const float* weights, int nboot,
+                    double* out);
+
+/* ---- instrumentation (no counterpart in the reference) ------------------ */
+/* Per-kernel aggregates over the launches made between start and stop:
+ * launches, device time from HIP events recorded around each launch on the
+ * context's stream, and the algorithmic HBM bytes and flops those launches
+ * must move (per-pattern unit cost x patterns x steps). */
+typedef struct {
+    char name[32];
+    int launches;
+    double total_ms;
+    double bytes;
+    double flops;
+} plk_kernel_stat;
+
+plk_status plk_profile_start(plk_context* ctx);
+plk_status plk_profile_stop(plk_context* ctx, plk_kernel_stat* stats, int cap, int* nstats);
+
+/* ---- inspection --------------------------------------------------------- */
+/* copy one slot back as FP64 [P][C][n] and its scale counters
+ * (uint16 [P] or [P][C]; the FP32 form reports its exponent counters) */
+plk_status plk_read_partial(plk_context* ctx, int instance, int slot, double* partial,
+                            int32_t* scale);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLK_H */

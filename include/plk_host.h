@@ -1,0 +1,127 @@
+/*
+ * plk_host.h - host-side C ABI around the likelihood engine:
+ *   - the data formats either side of the kernels (site patterns, model
+ *     eigensystem, Gamma rates, eigen-basis tip vectors);
+ *   - a seeded synthetic workload generator;
+ *   - a PhyloTree-shaped driver (traversal planning with per-node buffer
+ *     reorientation, branch Newton optimisation, optimizeAllBranches) that
+ *     calls the engine in plk.h.
+ * Functions returning int use the plk_status codes of plk.h.
+ */
+#ifndef PLK_HOST_H
+#define PLK_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "plk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PLKH_DNA = 0, PLKH_PROTEIN = 1, PLKH_CODON = 2 };
+
+/* ---- model inputs ------------------------------------------------------ */
+int plkh_num_states(int seq_type);
+int plkh_state_unknown(int seq_type);
+/* Q from exchangeabilities (upper triangle) and pi, normalised to mean rate 1
+ * (model/modelmarkov.cpp:1499-1512) */
+int plkh_rate_matrix(int n, const double* exch_upper, const double* pi, double* Q);
+/* eigensystem of a reversible Q (model/modelmarkov.cpp:1514-1556) */
+int plkh_decompose(int n, const double* Q, const double* pi, double* eval, double* evec,
+                   double* inv_evec);
+/* discrete-Gamma mean rates and weights with +I rescaling
+ * (model/rategamma.cpp:98-171, model/rategammainvar.cpp:25-42) */
+int plkh_gamma_rates(int ncat, double alpha, double p_inv, double* rates, double* props);
+/* eigen-basis tip vectors [SU+1][n] (tree/phylotreesse.cpp:455-466) */
+int plkh_tip_vectors(int seq_type, const double* inv_evec, double* tip);
+/* GY94 codon exchangeabilities (model/modelcodon.cpp:845-870) */
+int plkh_codon_exchangeabilities(double kappa, double omega, double* exch_upper);
+
+/* ---- site patterns (alignment/alignment.cpp:1725-1929, 925-1022) ---------- */
+typedef struct plkh_patterns plkh_patterns;
+/* chars: nseq rows of nchar characters, row-major */
+plkh_patterns* plkh_patterns_build(int seq_type, int nseq, int nchar, const char* chars, int* status);
+void plkh_patterns_destroy(plkh_patterns* p);
+int plkh_patterns_dims(const plkh_patterns* p, int* nseq, int* nsite, int* npattern);
+/* states uint8 [nseq][npattern], freq int32 [npattern], const_char int32
+ * [npattern], site_pattern int32 [nsite]; any pointer may be NULL */
+int plkh_patterns_get(const plkh_patterns* p, uint8_t* states, int32_t* freq, int32_t* const_char,
+                      int32_t* site_pattern);
+/* ptn_invar (tree/phylotreesse.cpp:549-617) for the first npattern entries */
+int plkh_patterns_invar(const plkh_patterns* p, const double* pi, double p_inv, double* invar);
+
+/* ---- synthetic workloads ---------------------------------------------- */
+typedef struct {
+    int seq_type;      /* PLKH_DNA / PLKH_PROTEIN / PLKH_CODON */
+    int ntaxa;
+    int nsite;         /* alignment columns (codons for PLKH_CODON) */
+    int ncat;
+    double mean_len;   /* Exp(mean) branch lengths */
+    double alpha;      /* Gamma shape; <= 0 draws U(0.2, 2) */
+    double p_inv;
+    double gap_frac;   /* DNA: fraction of tip cells set to gap / ambiguity */
+    uint64_t seed;
+} plkh_workload_spec;
+
+typedef struct plkh_workload plkh_workload;
+plkh_workload* plkh_workload_create(const plkh_workload_spec* spec, int* status);
+/* the same generator on a caller-given tree (edge list over ntaxa leaves) */
+plkh_workload* plkh_workload_create_on_tree(const plkh_workload_spec* spec, int nedges,
+                                            const int* u, const int* v, const double* len,
+                                            int* status);
+void plkh_workload_destroy(plkh_workload* w);
+int plkh_workload_dims(const plkh_workload* w, int* ntaxa, int* nstates, int* ncat, int* npattern,
+                       int* state_unknown, int* nsite);
+/* arrays by name: "states" uint8 [ntaxa][P]; "freq", "invar" [P];
+ * "eval" [n]; "evec", "inv_evec", "Q" [n][n]; "pi" [n]; "tip" [SU+1][n];
+ * "rates", "props" [C]; "edges_u", "edges_v" int32 [2N-3]; "edges_len" [2N-3];
+ * "site_pattern" int32 [nsite]; "const_char" int32 [P]; "params" double
+ * [alpha, p_inv, kappa, omega] */
+int plkh_workload_get(const plkh_workload* w, const char* name, void* out);
+
+/* ---- PhyloTree-shaped driver --------------------------------------------- */
+typedef struct plkh_tree plkh_tree;
+/* tree over the workload's topology; ctx may be NULL (planning only) */
+plkh_tree* plkh_tree_create(plk_context* ctx, const plkh_workload* w, int precision,
+                            int safe_numeric, int* status);
+void plkh_tree_destroy(plkh_tree* t);
+int plkh_tree_instance(const plkh_tree* t);
+int plkh_tree_num_branches(const plkh_tree* t);
+/* branches in pre-order along the tree diameter (computeBestTraversal,
+ * tree/phylotree.cpp:2689-2696); arrays of length num_branches */
+int plkh_tree_branches(plkh_tree* t, int32_t* a, int32_t* b);
+int plkh_tree_get_length(const plkh_tree* t, int a, int b, double* len);
+int plkh_tree_set_length(plkh_tree* t, int a, int b, double len);  /* invalidates partials */
+int plkh_tree_clear_all_partial(plkh_tree* t);
+/* restore the branch lengths the tree was created with and clear all partials */
+int plkh_tree_reset(plkh_tree* t);
+/* computeTraversalInfo for branch (a, b) (tree/phylotree.cpp:5918-6028):
+ * writes the post-order op list of stale partials (marking them computed)
+ * and the branch ends as the engine expects them (node = partial slot,
+ * dad = slot or tip). */
+int plkh_tree_plan(plkh_tree* t, int a, int b, plk_op* ops, int cap, int* nops, plk_end* node,
+                   plk_end* dad, double* length);
+/* PhyloTree::computeLikelihood (tree/phylotree.cpp:1233-1270) */
+int plkh_tree_compute_likelihood(plkh_tree* t, double* lnl, double* pattern_lh);
+/* computeLikelihoodBranch on branch (a, b) */
+int plkh_tree_likelihood_branch(plkh_tree* t, int a, int b, double* lnl, double* pattern_lh);
+/* computeLikelihoodDerv at the current length of (a, b): theta is rebuilt
+ * when the branch changes (PhyloTree::theta_computed) */
+int plkh_tree_likelihood_derv(plkh_tree* t, int a, int b, double* df, double* ddf);
+/* optimizeOneBranch(a, b, clearLH = true, max_nr) */
+int plkh_tree_optimize_branch(plkh_tree* t, int a, int b, int max_nr, double* new_len,
+                              int* n_derv);
+/* optimizeAllBranches(iterations, tolerance, max_nr) (tree/phylotree.cpp:2698-2770);
+ * lnl = final tree lnL, n_derv = derivative evaluations */
+int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double tolerance, int max_nr,
+                                    double* lnl, int* n_derv);
+/* min/max branch length used by optimizeOneBranch (defaults 1e-6, 10) */
+int plkh_tree_set_branch_bounds(plkh_tree* t, double min_len, double max_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLK_HOST_H */

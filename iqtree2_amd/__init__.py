@@ -1,0 +1,13 @@
+"""MI355X-native Felsenstein-pruning likelihood engine for IQ-TREE 2.
+
+The compute path is libplk.so (hand-written HIP kernels for gfx950 behind a
+C ABI, include/plk.h); this package is its Python host mirror.
+"""
+from . import _capi
+from ._capi import PREC_F32, PREC_F64, PlkError, lib
+from .engine import Context, Instance, Patterns, PhyloTree, Workload, end, ops_array
+
+DNA, PROTEIN, CODON = _capi.DNA, _capi.PROTEIN, _capi.CODON
+
+__all__ = ["Context", "Instance", "Patterns", "PhyloTree", "Workload", "end", "ops_array",
+           "PREC_F32", "PREC_F64", "PlkError", "lib", "DNA", "PROTEIN", "CODON"]

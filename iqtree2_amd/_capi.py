@@ -1,0 +1,184 @@
+"""ctypes bindings of libplk.so (include/plk.h and include/plk_host.h).
+
+The library is the product: when it is missing or fails to load this module
+raises ImportError; there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libplk.so")
+
+c_int, c_double, c_size_t, c_void_p, c_char_p = (
+    ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p)
+P = ctypes.POINTER
+
+PLK_OK = 0
+STATUS_NAMES = {0: "PLK_OK", 1: "PLK_ERR_ARG", 2: "PLK_ERR_NOMEM", 3: "PLK_ERR_DEVICE",
+                4: "PLK_ERR_NONFINITE", 5: "PLK_ERR_STATE", 6: "PLK_ERR_UNSUPPORTED",
+                7: "PLK_ERR_NEWTON"}
+PREC_F64, PREC_F32 = 0, 1
+DNA, PROTEIN, CODON = 0, 1, 2
+
+
+class PlkChild(ctypes.Structure):
+    _fields_ = [("is_tip", c_int), ("id", c_int), ("length", c_double)]
+
+
+class PlkOp(ctypes.Structure):
+    _fields_ = [("dad_slot", c_int), ("nchildren", c_int), ("child", PlkChild * 2)]
+
+
+class PlkEnd(ctypes.Structure):
+    _fields_ = [("is_tip", c_int), ("id", c_int)]
+
+
+class PlkConfig(ctypes.Structure):
+    _fields_ = [("device", c_int), ("stream", c_void_p), ("hbm_budget", c_size_t)]
+
+
+class PlkInstanceDesc(ctypes.Structure):
+    _fields_ = [("ntaxa", c_int), ("nstates", c_int), ("ncat", c_int), ("npatterns", c_int),
+                ("nslots", c_int), ("state_unknown", c_int), ("precision", c_int),
+                ("safe_numeric", c_int)]
+
+
+class PlkKernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", c_int), ("total_ms", c_double),
+                ("bytes", c_double), ("flops", c_double)]
+
+
+class PlkhWorkloadSpec(ctypes.Structure):
+    _fields_ = [("seq_type", c_int), ("ntaxa", c_int), ("nsite", c_int), ("ncat", c_int),
+                ("mean_len", c_double), ("alpha", c_double), ("p_inv", c_double),
+                ("gap_frac", c_double), ("seed", ctypes.c_uint64)]
+
+
+# numpy view of an op list with the memory layout of plk_op
+OP_DTYPE = np.dtype([("dad_slot", "<i4"), ("nchildren", "<i4"),
+                     ("c0_tip", "<i4"), ("c0_id", "<i4"), ("c0_len", "<f8"),
+                     ("c1_tip", "<i4"), ("c1_id", "<i4"), ("c1_len", "<f8")])
+assert OP_DTYPE.itemsize == ctypes.sizeof(PlkOp) == 40
+
+_DP = P(c_double)
+_U8 = P(ctypes.c_uint8)
+_I32 = P(ctypes.c_int32)
+_F32 = P(ctypes.c_float)
+
+# name -> (restype, argtypes)
+_PROTOS = {
+    # plk.h
+    "plk_create": (c_int, [P(PlkConfig), P(c_void_p)]),
+    "plk_destroy": (c_int, [c_void_p]),
+    "plk_last_error": (c_char_p, []),
+    "plk_memory": (c_int, [c_void_p, P(c_size_t), P(c_size_t)]),
+    "plk_stream": (c_void_p, [c_void_p]),
+    "plk_synchronize": (c_int, [c_void_p]),
+    "plk_instance_footprint": (c_size_t, [P(PlkInstanceDesc)]),
+    "plk_instance_create": (c_int, [c_void_p, P(PlkInstanceDesc), P(c_int)]),
+    "plk_instance_destroy": (c_int, [c_void_p, c_int]),
+    "plk_set_patterns": (c_int, [c_void_p, c_int, _U8, _DP, _DP]),
+    "plk_set_model": (c_int, [c_void_p, c_int, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "plk_update_partials": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "plk_likelihood_branch": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd, c_double,
+                                      _DP, _DP]),
+    "plk_likelihood_buffer": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd]),
+    "plk_likelihood_derv": (c_int, [c_void_p, c_int, c_double, _DP, _DP, _DP]),
+    "plk_likelihood_from_buffer": (c_int, [c_void_p, c_int, c_double, _DP, _DP]),
+    "plk_optimize_branch": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd, c_double,
+                                    c_double, c_double, c_int, _DP, _DP, P(c_int)]),
+    "plk_rell": (c_int, [c_void_p, c_int, _F32, c_int, _DP]),
+    "plk_read_partial": (c_int, [c_void_p, c_int, c_int, _DP, _I32]),
+    "plk_profile_start": (c_int, [c_void_p]),
+    "plk_profile_stop": (c_int, [c_void_p, P(PlkKernelStat), c_int, P(c_int)]),
+    # plk_host.h
+    "plkh_num_states": (c_int, [c_int]),
+    "plkh_state_unknown": (c_int, [c_int]),
+    "plkh_rate_matrix": (c_int, [c_int, _DP, _DP, _DP]),
+    "plkh_decompose": (c_int, [c_int, _DP, _DP, _DP, _DP, _DP]),
+    "plkh_gamma_rates": (c_int, [c_int, c_double, c_double, _DP, _DP]),
+    "plkh_tip_vectors": (c_int, [c_int, _DP, _DP]),
+    "plkh_codon_exchangeabilities": (c_int, [c_double, c_double, _DP]),
+    "plkh_patterns_build": (c_void_p, [c_int, c_int, c_int, c_char_p, P(c_int)]),
+    "plkh_patterns_destroy": (None, [c_void_p]),
+    "plkh_patterns_dims": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
+    "plkh_patterns_get": (c_int, [c_void_p, _U8, _I32, _I32, _I32]),
+    "plkh_patterns_invar": (c_int, [c_void_p, _DP, c_double, _DP]),
+    "plkh_workload_create": (c_void_p, [P(PlkhWorkloadSpec), P(c_int)]),
+    "plkh_workload_create_on_tree": (c_void_p, [P(PlkhWorkloadSpec), c_int, _I32, _I32, _DP,
+                                                P(c_int)]),
+    "plkh_workload_destroy": (None, [c_void_p]),
+    "plkh_workload_dims": (c_int, [c_void_p] + [P(c_int)] * 6),
+    "plkh_workload_get": (c_int, [c_void_p, c_char_p, c_void_p]),
+    "plkh_tree_create": (c_void_p, [c_void_p, c_void_p, c_int, c_int, P(c_int)]),
+    "plkh_tree_destroy": (None, [c_void_p]),
+    "plkh_tree_instance": (c_int, [c_void_p]),
+    "plkh_tree_num_branches": (c_int, [c_void_p]),
+    "plkh_tree_branches": (c_int, [c_void_p, _I32, _I32]),
+    "plkh_tree_get_length": (c_int, [c_void_p, c_int, c_int, _DP]),
+    "plkh_tree_set_length": (c_int, [c_void_p, c_int, c_int, c_double]),
+    "plkh_tree_clear_all_partial": (c_int, [c_void_p]),
+    "plkh_tree_reset": (c_int, [c_void_p]),
+    "plkh_tree_plan": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, P(c_int), P(PlkEnd),
+                               P(PlkEnd), _DP]),
+    "plkh_tree_compute_likelihood": (c_int, [c_void_p, _DP, _DP]),
+    "plkh_tree_likelihood_branch": (c_int, [c_void_p, c_int, c_int, _DP, _DP]),
+    "plkh_tree_likelihood_derv": (c_int, [c_void_p, c_int, c_int, _DP, _DP]),
+    "plkh_tree_optimize_branch": (c_int, [c_void_p, c_int, c_int, c_int, _DP, P(c_int)]),
+    "plkh_tree_optimize_all_branches": (c_int, [c_void_p, c_int, c_double, c_int, _DP, P(c_int)]),
+    "plkh_tree_set_branch_bounds": (c_int, [c_void_p, c_double, c_double]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libplk.so with prototypes set (ImportError when absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libplk.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        try:
+            handle = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise ImportError(f"failed to load {LIB_PATH}: {e}") from e
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_names():
+    return list(_PROTOS)
+
+
+class PlkError(RuntimeError):
+    def __init__(self, status, where=""):
+        msg = lib().plk_last_error().decode(errors="replace")
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {msg}")
+
+
+def check(status, where=""):
+    if status != PLK_OK:
+        raise PlkError(status, where)
+    return status
+
+
+def dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def u8ptr(a):
+    return a.ctypes.data_as(_U8)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(_I32)
+
+
+def f32ptr(a):
+    return a.ctypes.data_as(_F32)

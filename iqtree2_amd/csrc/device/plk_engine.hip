@@ -1,0 +1,921 @@
+// C-ABI device engine: contexts, instances (one PhyloTree's device state),
+// launches of the kernels in plk_kernels.h. See include/plk.h for the
+// contract of every entry point.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "plk.h"
+#include "plk_kernels.h"
+
+using namespace plkdev;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+plk_status fail(plk_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return fail(PLK_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(_e));      \
+    } while (0)
+
+constexpr int kPadPatterns = 256;  // Ppad is a multiple of every tile size
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+struct Instance {
+    plk_instance_desc d{};
+    int Ppad = 0;
+    size_t block = 0;
+    size_t elem = 8;            // bytes of one stored value
+    size_t scale_elem = 2;      // bytes of one scale counter
+    int table_size = 0;         // elements per (op, child) table
+    size_t bytes = 0;           // HBM held
+    // device buffers
+    uint8_t* states = nullptr;
+    double* freq = nullptr;
+    double* invar = nullptr;
+    double* model = nullptr;    // eval | evec | inv_evec | tip | rates | props
+    void* partial = nullptr;
+    void* scale = nullptr;
+    void* theta = nullptr;
+    double* buffer_scale = nullptr;
+    double* ptn_lh = nullptr;
+    double* block_sums = nullptr;
+    double* red_out = nullptr;
+    double* root_tab = nullptr;
+    plk_op* ops = nullptr;
+    int ops_cap = 0;
+    void* tables = nullptr;
+    bool patterns_set = false, model_set = false, theta_ready = false, ptn_lh_ready = false;
+    double theta_length = 0.0;
+    double staged_bytes = 0.0, staged_flops = 0.0;   // per-pattern cost of the staged ops
+};
+
+size_t footprint_of(const plk_instance_desc* d, int Ppad) {
+    const size_t elem = d->precision == PLK_PREC_F32 ? 4 : 8;
+    const size_t scale_elem = d->precision == PLK_PREC_F32 ? 4 : 2;
+    const size_t block = (size_t)d->nstates * d->ncat;
+    const size_t ncnt = d->safe_numeric ? d->ncat : 1;
+    size_t b = 0;
+    b += round_up((size_t)d->ntaxa * Ppad, 256);                 // states
+    b += 2 * (size_t)Ppad * 8;                                   // freq, invar
+    b += (size_t)d->nslots * Ppad * block * elem;                // partials
+    b += round_up((size_t)d->nslots * Ppad * ncnt * scale_elem, 256);
+    b += (size_t)Ppad * block * elem;                            // theta
+    b += 2 * (size_t)Ppad * 8;                                   // buffer_scale, ptn_lh
+    b += (size_t)(Ppad / 8 + 64) * 3 * 8;                        // block sums
+    return b;
+}
+
+}  // namespace
+
+enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
+                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_NKIND };
+static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
+                                          "derv", "lnl_from_buffer", "prep_ops", "prep_root",
+                                          "reduce", "rell"};
+
+struct ProfRec {
+    int kind;
+    double bytes, flops;
+    int ev;
+};
+
+struct plk_context {
+    int device = 0;
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev;
+    size_t prof_next = 0;
+    std::vector<ProfRec> prof_recs;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t budget = 0, used = 0;
+    std::mutex mu;
+    std::vector<Instance*> inst;
+    double* h_out = nullptr;  // pinned host staging for reduced scalars
+};
+
+namespace {
+
+template <typename P>
+plk_status dmalloc(plk_context* ctx, Instance* in, P** ptr, size_t bytes) {
+    bytes = round_up(bytes ? bytes : 8, 256);
+    if (ctx->used + bytes > ctx->budget)
+        return fail(PLK_ERR_NOMEM, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use",
+                    bytes, ctx->used, ctx->budget);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(PLK_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    ctx->used += bytes;
+    if (in) in->bytes += bytes;
+    *ptr = reinterpret_cast<P*>(p);
+    return PLK_OK;
+}
+
+void dfree(plk_context* ctx, Instance* in, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    bytes = round_up(bytes ? bytes : 8, 256);
+    ctx->used = ctx->used >= bytes ? ctx->used - bytes : 0;
+    if (in) in->bytes = in->bytes >= bytes ? in->bytes - bytes : 0;
+}
+
+// event pair around a launch when profiling is on; returns the pair index or -1
+int prof_begin(plk_context* ctx) {
+    if (!ctx->prof_on) return -1;
+    if (ctx->prof_next + 2 > ctx->prof_ev.size()) {
+        const size_t old = ctx->prof_ev.size();
+        ctx->prof_ev.resize(old ? 2 * old : 8192);
+        for (size_t i = old; i < ctx->prof_ev.size(); i++)
+            if (hipEventCreate(&ctx->prof_ev[i]) != hipSuccess) return -1;
+    }
+    const int idx = (int)ctx->prof_next;
+    ctx->prof_next += 2;
+    (void)hipEventRecord(ctx->prof_ev[idx], ctx->stream);
+    return idx;
+}
+
+void prof_end(plk_context* ctx, int idx, int kind, double bytes, double flops) {
+    if (idx < 0) return;
+    (void)hipEventRecord(ctx->prof_ev[idx + 1], ctx->stream);
+    ctx->prof_recs.push_back(ProfRec{kind, bytes, flops, idx});
+}
+
+// algorithmic bytes / flops per pattern of one traversal step: child partials
+// read once, the parent written once, one state byte per tip child, scale
+// counters read and written
+void op_cost(const Instance* in, const plk_op& op, double& bytes, double& flops) {
+    const double s = (double)in->elem, n = in->d.nstates, C = in->d.ncat;
+    int k_int = 0, k_tip = 0;
+    for (int k = 0; k < op.nchildren; k++) (op.child[k].is_tip ? k_tip : k_int)++;
+    const double ncnt = in->d.safe_numeric ? C : 1;
+    bytes = C * n * s * (k_int + 1) + k_tip + in->scale_elem * ncnt * (k_int + 1);
+    flops = C * (2.0 * n * n * k_int + n * (k_int + k_tip - 1) + 2.0 * n * n);
+}
+
+Instance* get_instance(plk_context* ctx, int id) {
+    if (!ctx || id < 0 || id >= (int)ctx->inst.size()) return nullptr;
+    return ctx->inst[id];
+}
+
+InstDev inst_dev(const Instance* in) {
+    InstDev I;
+    I.P = in->d.npatterns;
+    I.Ppad = in->Ppad;
+    I.n = in->d.nstates;
+    I.C = in->d.ncat;
+    I.SU = in->d.state_unknown;
+    I.safe = in->d.safe_numeric;
+    I.states = in->states;
+    I.freq = in->freq;
+    I.invar = in->invar;
+    I.partial = in->partial;
+    I.scale = in->scale;
+    I.theta = in->theta;
+    I.buffer_scale = in->buffer_scale;
+    I.ptn_lh = in->ptn_lh;
+    I.slot_stride = (size_t)in->Ppad * in->block;
+    I.scale_stride = (size_t)in->Ppad * (in->d.safe_numeric ? in->d.ncat : 1);
+    return I;
+}
+
+ModelDev model_dev(const Instance* in) {
+    const int n = in->d.nstates;
+    ModelDev m;
+    m.eval = in->model;
+    m.evec = m.eval + n;
+    m.inv_evec = m.evec + n * n;
+    m.tip = m.inv_evec + n * n;
+    m.rates = m.tip + (size_t)(in->d.state_unknown + 1) * n;
+    m.props = m.rates + in->d.ncat;
+    return m;
+}
+
+plk_status check_ops(const Instance* in, const plk_op* ops, int nops) {
+    if (nops < 0 || (nops > 0 && !ops)) return fail(PLK_ERR_ARG, "invalid op list");
+    for (int o = 0; o < nops; o++) {
+        const plk_op& op = ops[o];
+        if (op.dad_slot < 0 || op.dad_slot >= in->d.nslots)
+            return fail(PLK_ERR_ARG, "op %d: dad slot %d out of range", o, op.dad_slot);
+        if (op.nchildren != 2)
+            return fail(PLK_ERR_UNSUPPORTED, "op %d: %d children (binary nodes only)", o, op.nchildren);
+        for (int k = 0; k < 2; k++) {
+            const plk_child& ch = op.child[k];
+            const int lim = ch.is_tip ? in->d.ntaxa : in->d.nslots;
+            if (ch.id < 0 || ch.id >= lim)
+                return fail(PLK_ERR_ARG, "op %d child %d: id %d out of range", o, k, ch.id);
+            if (!ch.is_tip && ch.id == op.dad_slot)
+                return fail(PLK_ERR_ARG, "op %d: child slot equals dad slot", o);
+            if (!(ch.length >= 0.0) || !std::isfinite(ch.length))
+                return fail(PLK_ERR_ARG, "op %d child %d: invalid length", o, k);
+        }
+    }
+    return PLK_OK;
+}
+
+plk_status check_end(const Instance* in, plk_end e, bool allow_tip) {
+    if (e.is_tip) {
+        if (!allow_tip) return fail(PLK_ERR_ARG, "branch end must be a partial slot");
+        if (e.id < 0 || e.id >= in->d.ntaxa) return fail(PLK_ERR_ARG, "tip id %d out of range", e.id);
+    } else if (e.id < 0 || e.id >= in->d.nslots) {
+        return fail(PLK_ERR_ARG, "slot %d out of range", e.id);
+    }
+    return PLK_OK;
+}
+
+// ---- kernel dispatch over (precision, states, categories) -------------------
+template <typename T, int N, int C>
+plk_status launch_traverse_t(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                             plk_end dad) {
+    const int PB = kThreads / C;
+    const int grid = in->Ppad / PB;
+    const size_t tab_bytes = 2 * (size_t)in->table_size * sizeof(T);
+    const size_t root_bytes = (size_t)(in->d.state_unknown + 2) * in->block * 8;
+    const size_t smem = round_up((size_t)N * N * sizeof(T), 16) + std::max(tab_bytes, root_bytes);
+    if (smem > 160 * 1024) return fail(PLK_ERR_UNSUPPORTED, "LDS need %zu bytes", smem);
+    const int pe = prof_begin(ctx);
+    hipLaunchKernelGGL((k_traverse<T, N, C>), dim3(grid), dim3(kThreads), smem, ctx->stream,
+                       inst_dev(in), model_dev(in), in->ops, nops,
+                       reinterpret_cast<const T*>(in->tables), in->table_size, mode, node, dad,
+                       in->root_tab, in->block_sums);
+    HIP_TRY(hipGetLastError());
+    if (pe >= 0) {
+        const double P = in->d.npatterns, s = (double)in->elem, blk = (double)in->block;
+        const double ncnt = in->d.safe_numeric ? in->d.ncat : 1;
+        double b = in->staged_bytes, f = in->staged_flops;
+        int kind = K_TRAVERSE;
+        if (mode != MODE_PARTIALS) {
+            const int nint = 1 + (dad.is_tip ? 0 : 1);
+            b += blk * s * nint + (dad.is_tip ? 1 : 0) + in->scale_elem * ncnt * nint;
+            if (mode == MODE_BRANCH) {
+                b += 24.0;   // invar, freq, pattern lnL
+                f += 3.0 * blk;
+                kind = K_TRAVERSE_LNL;
+            } else {
+                b += blk * s + 8.0;   // theta, buffer_scale
+                f += blk;
+                kind = K_TRAVERSE_THETA;
+            }
+        }
+        prof_end(ctx, pe, kind, b * P, f * P);
+    }
+    return PLK_OK;
+}
+
+template <typename T, int N>
+plk_status launch_traverse_n(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                             plk_end dad) {
+    switch (in->d.ncat) {
+    case 1: return launch_traverse_t<T, N, 1>(ctx, in, nops, mode, node, dad);
+    case 2: return launch_traverse_t<T, N, 2>(ctx, in, nops, mode, node, dad);
+    case 4: return launch_traverse_t<T, N, 4>(ctx, in, nops, mode, node, dad);
+    case 8: return launch_traverse_t<T, N, 8>(ctx, in, nops, mode, node, dad);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+}
+
+template <typename T>
+plk_status launch_traverse_p(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                             plk_end dad) {
+    switch (in->d.nstates) {
+    case 4: return launch_traverse_n<T, 4>(ctx, in, nops, mode, node, dad);
+    case 20: return launch_traverse_n<T, 20>(ctx, in, nops, mode, node, dad);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
+}
+
+plk_status launch_traverse(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                           plk_end dad) {
+    if (in->d.precision == PLK_PREC_F32) return launch_traverse_p<float>(ctx, in, nops, mode, node, dad);
+    return launch_traverse_p<double>(ctx, in, nops, mode, node, dad);
+}
+
+template <typename T, int N, int C>
+plk_status launch_buffer_t(plk_context* ctx, Instance* in, double length, int mode) {
+    const int PB = kThreads / C;
+    const int grid = in->Ppad / PB;
+    int pe = prof_begin(ctx);
+    hipLaunchKernelGGL((k_buffer<T, N, C>), dim3(grid), dim3(kThreads), 0, ctx->stream,
+                       inst_dev(in), model_dev(in), length, mode, in->block_sums);
+    HIP_TRY(hipGetLastError());
+    {
+        const double P = in->d.npatterns, blk = (double)in->block, s = (double)in->elem;
+        // theta + invar + freq + buffer_scale (+ pattern lnL written by lnL-from-buffer)
+        const double b = blk * s + 24.0 + (mode == BUF_LNL ? 8.0 : 0.0);
+        const double f = (mode == BUF_DERV ? 6.0 : 2.0) * blk + 10.0;
+        prof_end(ctx, pe, mode == BUF_DERV ? K_DERV : K_FROM_BUFFER, b * P, f * P);
+    }
+    pe = prof_begin(ctx);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->block_sums, grid, 3,
+                       in->red_out);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_REDUCE, 3.0 * 8 * grid, 3.0 * grid);
+    return PLK_OK;
+}
+
+template <typename T, int N>
+plk_status launch_buffer_n(plk_context* ctx, Instance* in, double length, int mode) {
+    switch (in->d.ncat) {
+    case 1: return launch_buffer_t<T, N, 1>(ctx, in, length, mode);
+    case 2: return launch_buffer_t<T, N, 2>(ctx, in, length, mode);
+    case 4: return launch_buffer_t<T, N, 4>(ctx, in, length, mode);
+    case 8: return launch_buffer_t<T, N, 8>(ctx, in, length, mode);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+}
+
+plk_status launch_buffer(plk_context* ctx, Instance* in, double length, int mode) {
+    const bool f32 = in->d.precision == PLK_PREC_F32;
+    switch (in->d.nstates) {
+    case 4: return f32 ? launch_buffer_n<float, 4>(ctx, in, length, mode)
+                       : launch_buffer_n<double, 4>(ctx, in, length, mode);
+    case 20: return f32 ? launch_buffer_n<float, 20>(ctx, in, length, mode)
+                        : launch_buffer_n<double, 20>(ctx, in, length, mode);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
+}
+
+// upload ops and build their transition tables
+plk_status stage_ops(plk_context* ctx, Instance* in, const plk_op* ops, int nops) {
+    plk_status st = check_ops(in, ops, nops);
+    if (st != PLK_OK) return st;
+    in->staged_bytes = in->staged_flops = 0.0;
+    if (nops == 0) return PLK_OK;
+    if (nops > in->ops_cap) {
+        const size_t elem = in->elem;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        dfree(ctx, in, in->ops, sizeof(plk_op) * in->ops_cap);
+        dfree(ctx, in, in->tables, (size_t)in->ops_cap * 2 * in->table_size * elem);
+        in->ops = nullptr;
+        in->tables = nullptr;
+        const int cap = std::max(nops, 2 * in->ops_cap);
+        st = dmalloc(ctx, in, &in->ops, sizeof(plk_op) * cap);
+        if (st != PLK_OK) return st;
+        st = dmalloc(ctx, in, &in->tables, (size_t)cap * 2 * in->table_size * elem);
+        if (st != PLK_OK) return st;
+        in->ops_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(in->ops, ops, sizeof(plk_op) * nops, hipMemcpyHostToDevice, ctx->stream));
+    in->staged_bytes = in->staged_flops = 0.0;
+    for (int o = 0; o < nops; o++) {
+        double b, f;
+        op_cost(in, ops[o], b, f);
+        in->staged_bytes += b;
+        in->staged_flops += f;
+    }
+    const size_t smem = (size_t)in->d.ncat * in->d.nstates * sizeof(double);
+    const int pe = prof_begin(ctx);
+    if (in->d.precision == PLK_PREC_F32) {
+        hipLaunchKernelGGL(k_prep_ops<float>, dim3(2 * nops), dim3(kThreads), smem, ctx->stream,
+                           in->ops, nops, model_dev(in), in->d.nstates, in->d.ncat,
+                           in->d.state_unknown, in->table_size, reinterpret_cast<float*>(in->tables));
+    } else {
+        hipLaunchKernelGGL(k_prep_ops<double>, dim3(2 * nops), dim3(kThreads), smem, ctx->stream,
+                           in->ops, nops, model_dev(in), in->d.nstates, in->d.ncat,
+                           in->d.state_unknown, in->table_size, reinterpret_cast<double*>(in->tables));
+    }
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_PREP_OPS, 2.0 * nops * in->table_size * in->elem, 0.0);
+    return PLK_OK;
+}
+
+plk_status ready(Instance* in) {
+    if (!in) return fail(PLK_ERR_ARG, "unknown instance");
+    if (!in->patterns_set) return fail(PLK_ERR_STATE, "patterns not set");
+    if (!in->model_set) return fail(PLK_ERR_STATE, "model not set");
+    return PLK_OK;
+}
+
+plk_status fetch_scalars(plk_context* ctx, Instance* in, int count) {
+    HIP_TRY(hipMemcpyAsync(ctx->h_out, in->red_out, sizeof(double) * count, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PLK_OK;
+}
+
+// per-pattern fix-up of a non-finite tree lnL (tree/phylokernelnew.h:3115-3132)
+plk_status fixup_nonfinite(plk_context* ctx, Instance* in, double* lnl, double* pattern_lh) {
+    const int P = in->d.npatterns;
+    std::vector<double> plh(P), freq(P);
+    HIP_TRY(hipMemcpyAsync(plh.data(), in->ptn_lh, sizeof(double) * P, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(freq.data(), in->freq, sizeof(double) * P, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (int p = 0; p < P; p++) {
+        if (!std::isfinite(plh[p])) plh[p] = kLogScaleThreshold * 4;
+        s += plh[p] * freq[p];
+    }
+    HIP_TRY(hipMemcpyAsync(in->ptn_lh, plh.data(), sizeof(double) * P, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (pattern_lh) std::memcpy(pattern_lh, plh.data(), sizeof(double) * P);
+    *lnl = s;
+    return PLK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* plk_last_error(void) { return g_last_error.c_str(); }
+
+plk_status plk_create(const plk_config* cfg, plk_context** out) {
+    if (!out) return fail(PLK_ERR_ARG, "null output");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    const int dev = cfg ? cfg->device : 0;
+    if (dev < 0 || dev >= ndev) return fail(PLK_ERR_ARG, "device %d of %d", dev, ndev);
+    HIP_TRY(hipSetDevice(dev));
+    plk_context* ctx = new plk_context();
+    ctx->device = dev;
+    if (cfg && cfg->stream) {
+        ctx->stream = reinterpret_cast<hipStream_t>(cfg->stream);
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ctx;
+            return fail(PLK_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+        }
+        ctx->own_stream = true;
+    }
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    ctx->budget = (cfg && cfg->hbm_budget) ? cfg->hbm_budget : (size_t)(0.9 * (double)free_b);
+    hipError_t e = hipHostMalloc((void**)&ctx->h_out, 64 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(PLK_ERR_DEVICE, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return PLK_OK;
+}
+
+plk_status plk_destroy(plk_context* ctx) {
+    if (!ctx) return PLK_OK;
+    for (int i = 0; i < (int)ctx->inst.size(); i++)
+        if (ctx->inst[i]) plk_instance_destroy(ctx, i);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    (void)hipHostFree(ctx->h_out);
+    delete ctx;
+    return PLK_OK;
+}
+
+plk_status plk_memory(plk_context* ctx, size_t* used, size_t* budget) {
+    if (!ctx) return fail(PLK_ERR_ARG, "null context");
+    if (used) *used = ctx->used;
+    if (budget) *budget = ctx->budget;
+    return PLK_OK;
+}
+
+void* plk_stream(plk_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+plk_status plk_synchronize(plk_context* ctx) {
+    if (!ctx) return fail(PLK_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PLK_OK;
+}
+
+size_t plk_instance_footprint(const plk_instance_desc* d) {
+    if (!d || d->npatterns <= 0) return 0;
+    return footprint_of(d, (int)round_up(d->npatterns, kPadPatterns));
+}
+
+plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* d, int* id) {
+    if (!ctx || !d || !id) return fail(PLK_ERR_ARG, "null argument");
+    if (d->ntaxa < 3 || d->npatterns <= 0 || d->nslots <= 0 || d->nstates < 2 || d->ncat < 1 ||
+        d->state_unknown < d->nstates || d->state_unknown > 255)
+        return fail(PLK_ERR_ARG, "invalid instance shape");
+    if (d->nstates != 4 && d->nstates != 20)
+        return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built into this library", d->nstates);
+    if (d->ncat != 1 && d->ncat != 2 && d->ncat != 4 && d->ncat != 8)
+        return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built into this library", d->ncat);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int Ppad = (int)round_up(d->npatterns, kPadPatterns);
+    const size_t need = footprint_of(d, Ppad);
+    if (ctx->used + need > ctx->budget)
+        return fail(PLK_ERR_NOMEM, "instance needs %zu bytes, %zu of %zu in use", need, ctx->used,
+                    ctx->budget);
+    HIP_TRY(hipSetDevice(ctx->device));
+    Instance* in = new Instance();
+    in->d = *d;
+    in->Ppad = Ppad;
+    in->block = (size_t)d->nstates * d->ncat;
+    in->elem = d->precision == PLK_PREC_F32 ? 4 : 8;
+    in->scale_elem = d->precision == PLK_PREC_F32 ? 4 : 2;
+    in->table_size = (int)std::max(in->block * d->nstates, (size_t)(d->state_unknown + 1) * in->block);
+    const int n = d->nstates;
+    const size_t ncnt = d->safe_numeric ? d->ncat : 1;
+    const size_t model_doubles = n + 2 * (size_t)n * n + (size_t)(d->state_unknown + 1) * n + 2 * d->ncat;
+    const int max_grid = Ppad / (kThreads / 8) + 1;
+    plk_status st = PLK_OK;
+    auto A = [&](auto** p, size_t bytes) {
+        if (st == PLK_OK) st = dmalloc(ctx, in, p, bytes);
+    };
+    A(&in->states, (size_t)d->ntaxa * Ppad);
+    A(&in->freq, (size_t)Ppad * 8);
+    A(&in->invar, (size_t)Ppad * 8);
+    A(&in->model, model_doubles * 8);
+    A(&in->partial, (size_t)d->nslots * Ppad * in->block * in->elem);
+    A(&in->scale, (size_t)d->nslots * Ppad * ncnt * in->scale_elem);
+    A(&in->theta, (size_t)Ppad * in->block * in->elem);
+    A(&in->buffer_scale, (size_t)Ppad * 8);
+    A(&in->ptn_lh, (size_t)Ppad * 8);
+    A(&in->block_sums, (size_t)max_grid * 3 * 8);
+    A(&in->red_out, 8 * 8);
+    A(&in->root_tab, (size_t)(d->state_unknown + 2) * in->block * 8);
+    if (st == PLK_OK) {
+        // padding patterns: unknown tips, zero weight
+        if (hipMemsetAsync(in->states, d->state_unknown, (size_t)d->ntaxa * Ppad, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(in->freq, 0, (size_t)Ppad * 8, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(in->invar, 0, (size_t)Ppad * 8, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(in->scale, 0, (size_t)d->nslots * Ppad * ncnt * in->scale_elem, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            st = fail(PLK_ERR_DEVICE, "initialising instance buffers failed");
+    }
+    ctx->inst.push_back(in);
+    *id = (int)ctx->inst.size() - 1;
+    if (st != PLK_OK) {
+        std::string msg = g_last_error;
+        plk_instance_destroy(ctx, *id);
+        *id = -1;
+        g_last_error = msg;
+        return st;
+    }
+    return PLK_OK;
+}
+
+plk_status plk_instance_destroy(plk_context* ctx, int id) {
+    Instance* in = get_instance(ctx, id);
+    if (!in) return fail(PLK_ERR_ARG, "unknown instance %d", id);
+    (void)hipStreamSynchronize(ctx->stream);
+    void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
+                    in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
+                    in->root_tab, in->ops, in->tables};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    ctx->used = ctx->used >= in->bytes ? ctx->used - in->bytes : 0;
+    delete in;
+    ctx->inst[id] = nullptr;
+    return PLK_OK;
+}
+
+plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, const double* freq,
+                            const double* invar) {
+    Instance* in = get_instance(ctx, id);
+    if (!in || !states || !freq || !invar) return fail(PLK_ERR_ARG, "null argument");
+    const int P = in->d.npatterns;
+    for (size_t i = 0; i < (size_t)in->d.ntaxa * P; i++)
+        if (states[i] > in->d.state_unknown) return fail(PLK_ERR_ARG, "tip state %d > STATE_UNKNOWN", states[i]);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipMemcpy2DAsync(in->states, in->Ppad, states, P, P, in->d.ntaxa, hipMemcpyHostToDevice,
+                             ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->freq, freq, sizeof(double) * P, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->invar, invar, sizeof(double) * P, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    in->patterns_set = true;
+    in->theta_ready = false;
+    return PLK_OK;
+}
+
+plk_status plk_set_model(plk_context* ctx, int id, const double* eval, const double* evec,
+                         const double* inv_evec, const double* tip_eigen, const double* cat_rates,
+                         const double* cat_props) {
+    Instance* in = get_instance(ctx, id);
+    if (!in || !eval || !evec || !inv_evec || !tip_eigen || !cat_rates || !cat_props)
+        return fail(PLK_ERR_ARG, "null argument");
+    const int n = in->d.nstates, C = in->d.ncat, SU = in->d.state_unknown;
+    std::vector<double> h;
+    h.insert(h.end(), eval, eval + n);
+    h.insert(h.end(), evec, evec + n * n);
+    h.insert(h.end(), inv_evec, inv_evec + n * n);
+    h.insert(h.end(), tip_eigen, tip_eigen + (size_t)(SU + 1) * n);
+    h.insert(h.end(), cat_rates, cat_rates + C);
+    h.insert(h.end(), cat_props, cat_props + C);
+    for (double v : h)
+        if (!std::isfinite(v)) return fail(PLK_ERR_ARG, "non-finite model parameter");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipMemcpyAsync(in->model, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    in->model_set = true;
+    in->theta_ready = false;
+    return PLK_OK;
+}
+
+plk_status plk_update_partials(plk_context* ctx, int id, const plk_op* ops, int nops) {
+    Instance* in = get_instance(ctx, id);
+    plk_status st = ready(in);
+    if (st != PLK_OK) return st;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = stage_ops(ctx, in, ops, nops);
+    if (st != PLK_OK || nops == 0) return st;
+    plk_end none{0, 0};
+    st = launch_traverse(ctx, in, nops, MODE_PARTIALS, none, none);
+    if (st != PLK_OK) return st;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PLK_OK;
+}
+
+plk_status plk_likelihood_branch(plk_context* ctx, int id, const plk_op* ops, int nops,
+                                 plk_end node, plk_end dad, double length, double* lnl,
+                                 double* pattern_lh) {
+    Instance* in = get_instance(ctx, id);
+    plk_status st = ready(in);
+    if (st != PLK_OK) return st;
+    if (!lnl) return fail(PLK_ERR_ARG, "null lnl");
+    if ((st = check_end(in, node, false)) != PLK_OK) return st;
+    if ((st = check_end(in, dad, true)) != PLK_OK) return st;
+    if (!(length >= 0.0) || !std::isfinite(length)) return fail(PLK_ERR_ARG, "invalid length");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = stage_ops(ctx, in, ops, nops);
+    if (st != PLK_OK) return st;
+    int pe = prof_begin(ctx);
+    hipLaunchKernelGGL(k_prep_root, dim3(1), dim3(kThreads), 0, ctx->stream, model_dev(in),
+                       in->d.nstates, in->d.ncat, in->d.state_unknown, length, dad.is_tip,
+                       in->root_tab);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_PREP_ROOT, (in->d.state_unknown + 2.0) * in->block * 8, 0.0);
+    st = launch_traverse(ctx, in, nops, MODE_BRANCH, node, dad);
+    if (st != PLK_OK) return st;
+    const int grid = in->Ppad / (kThreads / in->d.ncat);
+    pe = prof_begin(ctx);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->block_sums, grid, 1,
+                       in->red_out);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_REDUCE, 8.0 * grid, (double)grid);
+    if ((st = fetch_scalars(ctx, in, 1)) != PLK_OK) return st;
+    *lnl = ctx->h_out[0];
+    in->ptn_lh_ready = true;
+    if (!std::isfinite(*lnl)) {
+        st = fixup_nonfinite(ctx, in, lnl, pattern_lh);
+        if (st != PLK_OK) return st;
+        return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch; per-pattern fix-up applied");
+    }
+    if (pattern_lh) {
+        HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * in->d.npatterns,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return PLK_OK;
+}
+
+plk_status plk_likelihood_buffer(plk_context* ctx, int id, const plk_op* ops, int nops,
+                                 plk_end node, plk_end dad) {
+    Instance* in = get_instance(ctx, id);
+    plk_status st = ready(in);
+    if (st != PLK_OK) return st;
+    if ((st = check_end(in, node, false)) != PLK_OK) return st;
+    if ((st = check_end(in, dad, true)) != PLK_OK) return st;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = stage_ops(ctx, in, ops, nops);
+    if (st != PLK_OK) return st;
+    st = launch_traverse(ctx, in, nops, MODE_THETA, node, dad);
+    if (st != PLK_OK) return st;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    in->theta_ready = true;
+    return PLK_OK;
+}
+
+plk_status plk_likelihood_derv(plk_context* ctx, int id, double length, double* df, double* ddf,
+                               double* lnl) {
+    Instance* in = get_instance(ctx, id);
+    plk_status st = ready(in);
+    if (st != PLK_OK) return st;
+    if (!df || !ddf) return fail(PLK_ERR_ARG, "null output");
+    if (!in->theta_ready) return fail(PLK_ERR_STATE, "theta buffer not computed");
+    if (!std::isfinite(length)) return fail(PLK_ERR_ARG, "invalid length");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = launch_buffer(ctx, in, length, BUF_DERV);
+    if (st != PLK_OK) return st;
+    if ((st = fetch_scalars(ctx, in, 3)) != PLK_OK) return st;
+    *df = ctx->h_out[0];
+    *ddf = ctx->h_out[1];
+    if (lnl) *lnl = ctx->h_out[2];
+    if (!std::isfinite(*df)) {
+        *df = *ddf = 0.0;
+        return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-derivative");
+    }
+    return PLK_OK;
+}
+
+plk_status plk_likelihood_from_buffer(plk_context* ctx, int id, double length, double* lnl,
+                                      double* pattern_lh) {
+    Instance* in = get_instance(ctx, id);
+    plk_status st = ready(in);
+    if (st != PLK_OK) return st;
+    if (!lnl) return fail(PLK_ERR_ARG, "null output");
+    if (!in->theta_ready) return fail(PLK_ERR_STATE, "theta buffer not computed");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    st = launch_buffer(ctx, in, length, BUF_LNL);
+    if (st != PLK_OK) return st;
+    if ((st = fetch_scalars(ctx, in, 3)) != PLK_OK) return st;
+    *lnl = ctx->h_out[2];
+    in->ptn_lh_ready = true;
+    if (!std::isfinite(*lnl)) {
+        st = fixup_nonfinite(ctx, in, lnl, pattern_lh);
+        if (st != PLK_OK) return st;
+        if (!in->d.safe_numeric)
+            return fail(PLK_ERR_NONFINITE, "numerical underflow (lh-from-buffer); use the safe kernel");
+        return PLK_OK;
+    }
+    if (pattern_lh) {
+        HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * in->d.npatterns,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return PLK_OK;
+}
+
+plk_status plk_optimize_branch(plk_context* ctx, int id, const plk_op* ops, int nops,
+                               plk_end node, plk_end dad, double length, double min_len,
+                               double max_len, int max_nr, double* out_length, double* out_lnl,
+                               int* n_derv) {
+    if (!out_length) return fail(PLK_ERR_ARG, "null output");
+    if (!(min_len > 0.0) || !(max_len > min_len)) return fail(PLK_ERR_ARG, "invalid bracket");
+    plk_status st = plk_likelihood_buffer(ctx, id, ops, nops, node, dad);
+    if (st != PLK_OK) return st;
+    // Optimization::minimizeNewton (utils/optimization.cpp:422-499) on
+    // f = -dlnL/dt (PhyloTree::computeFuncDerv, tree/phylotree.cpp:2529-2536)
+    int evals = 0;
+    auto fderv = [&](double x, double& f, double& dfx) -> plk_status {
+        double d1, d2;
+        plk_status s = plk_likelihood_derv(ctx, id, x, &d1, &d2, nullptr);
+        evals++;
+        f = -d1;
+        dfx = -d2;
+        return s;
+    };
+    const double x1 = min_len, x2 = max_len, xacc = min_len;
+    double f, df, dx, dxold, temp, xh, xl, rts, rts_old;
+    double result = 0.0;
+    bool done = false;
+    rts = length;
+    if (rts < x1) rts = x1;
+    if (rts > x2) rts = x2;
+    if ((st = fderv(rts, f, df)) != PLK_OK) return st;
+    if (df >= 0.0 && std::fabs(f) < xacc) {
+        result = rts;
+        done = true;
+    }
+    if (!done) {
+        if (f < 0.0) { xl = rts; xh = x2; } else { xh = rts; xl = x1; }
+        dx = dxold = std::fabs(xh - xl);
+        for (int j = 1; j <= max_nr && !done; j++) {
+            rts_old = rts;
+            if ((df <= 0.0) || (((rts - xh) * df - f) * ((rts - xl) * df - f) >= 0.0)) {
+                dxold = dx;
+                dx = 0.5 * (xh - xl);
+                rts = xl + dx;
+                if (xl == rts) { result = rts; done = true; break; }
+            } else {
+                dxold = dx;
+                dx = f / df;
+                temp = rts;
+                rts -= dx;
+                if (temp == rts) { result = rts; done = true; break; }
+            }
+            if (std::fabs(dx) < xacc || j == max_nr) { result = rts_old; done = true; break; }
+            if ((st = fderv(rts, f, df)) != PLK_OK) return st;
+            if (df > 0.0 && std::fabs(f) < xacc) { result = rts; done = true; break; }
+            if (f < 0.0) xl = rts;
+            else if (f > 0.0) xh = rts;
+        }
+        (void)dxold;
+        if (!done) return fail(PLK_ERR_NEWTON, "maximum number of iterations exceeded in minimizeNewton");
+    }
+    double optx = result;
+    // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625)
+    if (optx > max_len * 0.95) {
+        double opt_lh, orig_lh;
+        if ((st = plk_likelihood_from_buffer(ctx, id, optx, &opt_lh, nullptr)) != PLK_OK) return st;
+        if ((st = plk_likelihood_from_buffer(ctx, id, length, &orig_lh, nullptr)) != PLK_OK) return st;
+        if (orig_lh > opt_lh) optx = length;
+    }
+    *out_length = optx;
+    if (n_derv) *n_derv = evals;
+    if (out_lnl) {
+        if ((st = plk_likelihood_from_buffer(ctx, id, optx, out_lnl, nullptr)) != PLK_OK) return st;
+    }
+    return PLK_OK;
+}
+
+plk_status plk_rell(plk_context* ctx, int id, const float* weights, int nboot, double* out) {
+    Instance* in = get_instance(ctx, id);
+    if (!in || !weights || !out || nboot <= 0) return fail(PLK_ERR_ARG, "invalid argument");
+    if (!in->ptn_lh_ready) return fail(PLK_ERR_STATE, "no pattern likelihoods evaluated yet");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int P = in->d.npatterns;
+    float* dw = nullptr;
+    double* dout = nullptr;
+    plk_status st = dmalloc(ctx, nullptr, &dw, (size_t)nboot * P * sizeof(float));
+    if (st != PLK_OK) return st;
+    st = dmalloc(ctx, nullptr, &dout, (size_t)nboot * sizeof(double));
+    if (st != PLK_OK) {
+        dfree(ctx, nullptr, dw, (size_t)nboot * P * sizeof(float));
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(dw, weights, (size_t)nboot * P * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_rell, dim3(nboot), dim3(kThreads), 0, ctx->stream, dw, in->ptn_lh, P, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, nboot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(ctx, nullptr, dw, (size_t)nboot * P * sizeof(float));
+    dfree(ctx, nullptr, dout, (size_t)nboot * sizeof(double));
+    if (e != hipSuccess) return fail(PLK_ERR_DEVICE, "rell: %s", hipGetErrorString(e));
+    return PLK_OK;
+}
+
+plk_status plk_profile_start(plk_context* ctx) {
+    if (!ctx) return fail(PLK_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->prof_on = true;
+    ctx->prof_next = 0;
+    ctx->prof_recs.clear();
+    return PLK_OK;
+}
+
+plk_status plk_profile_stop(plk_context* ctx, plk_kernel_stat* stats, int cap, int* nstats) {
+    if (!ctx) return fail(PLK_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->prof_on = false;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    plk_kernel_stat agg[K_NKIND];
+    std::memset(agg, 0, sizeof(agg));
+    for (int k = 0; k < K_NKIND; k++) std::snprintf(agg[k].name, sizeof(agg[k].name), "%s", kProfNames[k]);
+    for (const ProfRec& r : ctx->prof_recs) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_ev[r.ev], ctx->prof_ev[r.ev + 1]));
+        agg[r.kind].launches++;
+        agg[r.kind].total_ms += ms;
+        agg[r.kind].bytes += r.bytes;
+        agg[r.kind].flops += r.flops;
+    }
+    int n = 0;
+    for (int k = 0; k < K_NKIND; k++)
+        if (agg[k].launches > 0) {
+            if (stats && n < cap) stats[n] = agg[k];
+            n++;
+        }
+    if (nstats) *nstats = n;
+    ctx->prof_recs.clear();
+    ctx->prof_next = 0;
+    return PLK_OK;
+}
+
+plk_status plk_read_partial(plk_context* ctx, int id, int slot, double* partial, int32_t* scale) {
+    Instance* in = get_instance(ctx, id);
+    if (!in) return fail(PLK_ERR_ARG, "unknown instance");
+    if (slot < 0 || slot >= in->d.nslots) return fail(PLK_ERR_ARG, "slot out of range");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int P = in->d.npatterns;
+    const size_t ncnt = in->d.safe_numeric ? in->d.ncat : 1;
+    if (partial) {
+        const size_t cnt = (size_t)P * in->block;
+        std::vector<unsigned char> h(cnt * in->elem);
+        const unsigned char* src = reinterpret_cast<const unsigned char*>(in->partial) +
+                                   (size_t)slot * in->Ppad * in->block * in->elem;
+        HIP_TRY(hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < cnt; i++)
+            partial[i] = in->elem == 8 ? reinterpret_cast<const double*>(h.data())[i]
+                                       : (double)reinterpret_cast<const float*>(h.data())[i];
+    }
+    if (scale) {
+        const size_t cnt = (size_t)P * ncnt;
+        std::vector<unsigned char> h(cnt * in->scale_elem);
+        const unsigned char* src = reinterpret_cast<const unsigned char*>(in->scale) +
+                                   (size_t)slot * in->Ppad * ncnt * in->scale_elem;
+        HIP_TRY(hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < cnt; i++)
+            scale[i] = in->scale_elem == 2 ? (int32_t)reinterpret_cast<const uint16_t*>(h.data())[i]
+                                           : reinterpret_cast<const int32_t*>(h.data())[i];
+    }
+    return PLK_OK;
+}
+
+}  // extern "C"

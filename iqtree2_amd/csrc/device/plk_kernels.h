@@ -1,0 +1,528 @@
+// HIP kernels of the likelihood engine (gfx950, wave64).
+//
+// Layout in HBM (one instance):
+//   partial[slot][p][c][x]   T (double or float), pattern-major, eigen basis
+//   scale[slot][p]           uint16 (FP64 form) / int32 exponent (FP32 form),
+//                            [slot][p][c] in safe mode
+//   theta[p][c][x]           T; buffer_scale[p] double
+//   states[taxon][p]         uint8
+// Thread mapping: one thread per (pattern, category); the C threads of a
+// pattern are adjacent lanes of one wavefront, so a pattern's block of C*n
+// values is read and written as one contiguous run, and cross-category
+// reductions are lane shuffles. A workgroup owns a tile of 256/C patterns
+// and walks the whole post-order op list for that tile: patterns are
+// independent, so no synchronisation crosses workgroups.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "plk.h"
+
+namespace plkdev {
+
+constexpr int kThreads = 256;
+constexpr double kScaleThreshold64 = 0x1p-256;          // tree/phylotree.h:74-80
+constexpr int kScaleExp64 = 256;
+constexpr double kLogScaleThreshold = -177.4456782233459932741;
+constexpr float kScaleThreshold32 = 0x1p-40f;           // FP32-storage rescale threshold
+constexpr double kLn2 = 0.6931471805599453094172321;
+
+// Precision traits: storage type, scale counter type and the rescale rule.
+template <typename T> struct Prec;
+template <> struct Prec<double> {
+    using scale_t = uint16_t;
+    // one rescale by 2^256 per underflowing node (tree/phylokernelnew.h:1993-2030)
+    __device__ static bool underflow(double m) { return m < kScaleThreshold64; }
+    __device__ static int exponent(double) { return 1; }
+    __device__ static double apply(double v, int) { return ldexp(v, kScaleExp64); }
+    // safe mode: a category scaled once more than the least-scaled one is
+    // multiplied by 2^-256, more than once is dropped (tree/phylokernelnew.h:2891-2915)
+    __device__ static double safe_adjust(int d) {
+        return d == 0 ? 1.0 : (d == 1 ? kScaleThreshold64 : 0.0);
+    }
+    static constexpr double log_unit = kLogScaleThreshold;   // ln(2^-256)
+};
+template <> struct Prec<float> {
+    using scale_t = int32_t;
+    // exact power-of-two rescale that brings the block max into [0.5, 1)
+    __device__ static bool underflow(float m) { return m < kScaleThreshold32 && m > 0.0f; }
+    __device__ static int exponent(float m) { return -(ilogbf(m) + 1); }
+    __device__ static float apply(float v, int k) { return ldexpf(v, k); }
+    // counters are exponents, so categories differ by an exact power of two
+    __device__ static double safe_adjust(int d) { return ldexp(1.0, -d); }
+    static constexpr double log_unit = -kLn2;
+};
+
+struct ModelDev {
+    const double* eval;      // [n]
+    const double* evec;      // [n][n]
+    const double* inv_evec;  // [n][n]
+    const double* tip;       // [SU+1][n]
+    const double* rates;     // [C]
+    const double* props;     // [C]
+};
+
+struct InstDev {
+    int P, Ppad, n, C, SU, safe;
+    const uint8_t* states;   // [ntaxa][Ppad]
+    const double* freq;      // [Ppad]
+    const double* invar;     // [Ppad]
+    void* partial;           // [nslots][Ppad*block]
+    void* scale;             // [nslots][Ppad*(safe?C:1)]
+    void* theta;             // [Ppad*block]
+    double* buffer_scale;    // [Ppad]
+    double* ptn_lh;          // [Ppad]
+    size_t slot_stride;      // Ppad*block
+    size_t scale_stride;     // Ppad*(safe?C:1)
+};
+
+enum TraverseMode { MODE_PARTIALS = 0, MODE_BRANCH = 1, MODE_THETA = 2 };
+enum BufferMode { BUF_DERV = 0, BUF_LNL = 1 };
+
+// ---------------------------------------------------------------------------
+// computePartialInfo (tree/phylokernelnew.h:1008-1109): per op and child, the
+// eigen-basis transition rows echild[c][x][i] = evec[x][i] exp(eval[i] r_c t)
+// for an internal child, or the tip table leaf[s][c][x] =
+// sum_i echild[c][x][i] tip[s][i] (leaf[SU] = 1) for a tip child.
+// One workgroup per (op, child); tables are written in storage precision.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_prep_ops(const plk_op* __restrict__ ops, int nops, ModelDev m, int n, int C, int SU,
+           int table_size, T* __restrict__ tables) {
+    extern __shared__ double ex[];   // [C][n]
+    const int o = blockIdx.x >> 1, k = blockIdx.x & 1;
+    if (o >= nops) return;
+    const plk_child ch = ops[o].child[k];
+    for (int t = threadIdx.x; t < C * n; t += blockDim.x) {
+        const int c = t / n, i = t % n;
+        ex[t] = exp(m.eval[i] * (m.rates[c] * ch.length));
+    }
+    __syncthreads();
+    T* out = tables + (size_t)(2 * o + k) * table_size;
+    if (!ch.is_tip) {
+        const int total = C * n * n;
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int i = t % n, x = (t / n) % n, c = t / (n * n);
+            out[t] = (T)(m.evec[x * n + i] * ex[c * n + i]);
+        }
+    } else {
+        const int total = (SU + 1) * C * n;
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int x = t % n, c = (t / n) % C, s = t / (n * C);
+            double v;
+            if (s == SU) {
+                v = 1.0;
+            } else {
+                const double* ev = m.evec + x * n;
+                const double* tp = m.tip + s * n;
+                const double* e = ex + c * n;
+                v = ev[0] * e[0] * tp[0];
+                for (int i = 1; i < n; i++) v += ev[i] * e[i] * tp[i];
+            }
+            out[t] = (T)v;
+        }
+    }
+}
+
+// Root tables for the branch likelihood (tree/phylokernelnew.h:2713-2806):
+// val[c][i] = exp(eval[i] r_c t) p_c, and for a tip dad
+// lut[s][c][i] = val[c][i] * tip[s][i] for every s in 0..SU.
+__global__ void __launch_bounds__(kThreads)
+k_prep_root(ModelDev m, int n, int C, int SU, double length, int tip_dad, double* __restrict__ out) {
+    const int nv = C * n;
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+        const int c = t / n, i = t % n;
+        out[t] = exp(m.eval[i] * (m.rates[c] * length)) * m.props[c];
+    }
+    if (!tip_dad) return;
+    const int total = (SU + 1) * nv;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int i = t % n, c = (t / n) % C, s = t / nv;
+        out[nv + t] = exp(m.eval[i] * (m.rates[c] * length)) * m.props[c] * m.tip[s * n + i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int C>
+__device__ inline double group_max(double v) {
+#pragma unroll
+    for (int d = 1; d < C; d <<= 1) v = fmax(v, __shfl_xor(v, d));
+    return v;
+}
+template <int C>
+__device__ inline int group_min_int(int v) {
+#pragma unroll
+    for (int d = 1; d < C; d <<= 1) v = min(v, __shfl_xor(v, d));
+    return v;
+}
+// sum over the C lanes of a pattern in category order 0..C-1 (the order the
+// reference accumulates lh_cat), result in every lane of the group
+template <int C>
+__device__ inline double group_sum_ordered(double v) {
+    const int lane = threadIdx.x & 63;
+    const int base = lane & ~(C - 1);
+    double s = __shfl(v, base);
+#pragma unroll
+    for (int c = 1; c < C; c++) s += __shfl(v, base + c);
+    return s;
+}
+// sumVec<VectorClass, true> grouping over categories (tree/phylokernelnew.h:43-71)
+template <int C>
+__device__ inline double group_sum_sumvec(double v) {
+    const int lane = threadIdx.x & 63;
+    const int base = lane & ~(C - 1);
+    double vals[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) vals[c] = __shfl(v, base + c);
+    double x = 0.0;
+    constexpr int cruft = C & 3;
+    if (cruft < C) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int c = 0; c < C - cruft; c += 4) {
+            a0 += vals[c]; a1 += vals[c + 1]; a2 += vals[c + 2]; a3 += vals[c + 3];
+        }
+        a0 += a2;
+        a1 += a3;
+        a0 += a1;
+        x += a0;
+    }
+#pragma unroll
+    for (int c = C - cruft; c < C; c++) x += vals[c];
+    return x;
+}
+
+// Deterministic workgroup sum: xor-tree inside each wavefront, then the
+// wavefront partials in index order. Result valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* sh) {
+#pragma unroll
+    for (int j = 0; j < NV; j++)
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v[j] += __shfl_xor(v[j], d);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < NV; j++) sh[wave * NV + j] = v[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            double s = sh[j];
+            for (int w = 1; w < nw; w++) s += sh[w * NV + j];
+            v[j] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused post-order traversal (computePartialLikelihoodSIMD, binary nodes,
+// tree/phylokernelnew.h:1632-2031) followed, per mode, by the branch lnL
+// (computeLikelihoodBranchSIMD 2753-3104) or the theta buffer
+// (computeLikelihoodBufferSIMD 2091-2230) of the branch (node, dad).
+template <typename T, int N, int C>
+__global__ void __launch_bounds__(kThreads)
+k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
+           const T* __restrict__ tables, int table_size, int mode, plk_end node, plk_end dad,
+           const double* __restrict__ root_tab, double* __restrict__ block_sums) {
+    using ST = typename Prec<T>::scale_t;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* inv_sh = reinterpret_cast<T*>(smem);                 // [N*N]
+    T* tab_sh = inv_sh + N * N;                              // [2][table_size]
+    constexpr int PB = kThreads / C;
+    constexpr int block = N * C;
+    const int c = threadIdx.x % C;
+    const int p = blockIdx.x * PB + threadIdx.x / C;
+    const bool live = p < I.Ppad;
+    const bool safe = I.safe != 0;
+    T* partial = reinterpret_cast<T*>(I.partial);
+    ST* scale = reinterpret_cast<ST*>(I.scale);
+
+    for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)m.inv_evec[t];
+
+    for (int o = 0; o < nops; o++) {
+        const plk_op op = ops[o];
+        int left = 0, right = 1;
+        if (!op.child[0].is_tip && op.child[1].is_tip) { left = 1; right = 0; }
+        const plk_child L = op.child[left], R = op.child[right];
+        const int szL = L.is_tip ? (I.SU + 1) * block : block * N;
+        const int szR = R.is_tip ? (I.SU + 1) * block : block * N;
+        __syncthreads();
+        const T* gL = tables + (size_t)(2 * o + left) * table_size;
+        const T* gR = tables + (size_t)(2 * o + right) * table_size;
+        for (int t = threadIdx.x; t < szL; t += kThreads) tab_sh[t] = gL[t];
+        for (int t = threadIdx.x; t < szR; t += kThreads) tab_sh[table_size + t] = gR[t];
+        __syncthreads();
+        if (!live) continue;
+        const T* tL = tab_sh;
+        const T* tR = tab_sh + table_size;
+
+        T tmp[N];
+        int sc_l = 0, sc_r = 0;
+        const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
+        if (L.is_tip) {
+            const int s = I.states[(size_t)L.id * I.Ppad + p];
+            const T* lut = tL + (s * C + c) * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) tmp[x] = lut[x];
+        } else {
+            const T* src = partial + (size_t)L.id * I.slot_stride + (size_t)p * block + c * N;
+            T cv[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) cv[i] = src[i];
+            const T* E = tL + c * N * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) {
+                T acc = E[x * N] * cv[0];
+#pragma unroll
+                for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
+                tmp[x] = acc;
+            }
+            sc_l = scale[(size_t)L.id * I.scale_stride + sidx];
+        }
+        if (R.is_tip) {
+            const int s = I.states[(size_t)R.id * I.Ppad + p];
+            const T* lut = tR + (s * C + c) * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) tmp[x] *= lut[x];
+        } else {
+            const T* src = partial + (size_t)R.id * I.slot_stride + (size_t)p * block + c * N;
+            T cv[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) cv[i] = src[i];
+            const T* E = tR + c * N * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) {
+                T acc = E[x * N] * cv[0];
+#pragma unroll
+                for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
+                tmp[x] *= acc;
+            }
+            sc_r = scale[(size_t)R.id * I.scale_stride + sidx];
+        }
+        // back to the eigen basis: dad[i] = sum_x inv_evec[i][x] tmp[x]
+        T dadv[N];
+        T lmax = (T)0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            T acc = inv_sh[i * N] * tmp[0];
+#pragma unroll
+            for (int x = 1; x < N; x++) acc = fma(inv_sh[i * N + x], tmp[x], acc);
+            dadv[i] = acc;
+            lmax = fmax(lmax, fabs(acc));
+        }
+        int sc = sc_l + sc_r;
+        const bool tiptip = L.is_tip && R.is_tip;
+        if (!tiptip) {
+            const bool no_invar = I.invar[p] == 0.0;
+            if (safe) {
+                if (Prec<T>::underflow(lmax) && no_invar) {
+                    const int k = Prec<T>::exponent(lmax);
+#pragma unroll
+                    for (int i = 0; i < N; i++) dadv[i] = Prec<T>::apply(dadv[i], k);
+                    sc += k;
+                }
+            } else {
+                const T gmax = (T)group_max<C>((double)lmax);
+                if (Prec<T>::underflow(gmax) && no_invar) {
+                    const int k = Prec<T>::exponent(gmax);
+#pragma unroll
+                    for (int i = 0; i < N; i++) dadv[i] = Prec<T>::apply(dadv[i], k);
+                    sc += k;
+                }
+            }
+        }
+        T* dst = partial + (size_t)op.dad_slot * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) dst[i] = dadv[i];
+        if (safe || c == 0) scale[(size_t)op.dad_slot * I.scale_stride + sidx] = (ST)sc;
+    }
+
+    if (mode == MODE_PARTIALS) return;
+
+    // ---- root stage ---------------------------------------------------------
+    double* root_sh = reinterpret_cast<double*>(tab_sh);
+    const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
+    __syncthreads();
+    if (mode == MODE_BRANCH) {
+        for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = root_tab[t];
+    } else if (dad.is_tip) {
+        for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = m.tip[t];
+    }
+    __syncthreads();
+    double contrib = 0.0;
+    if (live) {
+        const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
+        const T* nsrc = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
+        double nv[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) nv[i] = (double)nsrc[i];
+        int sc = scale[(size_t)node.id * I.scale_stride + sidx];
+        double dv[N];
+        if (dad.is_tip) {
+            const int s = I.states[(size_t)dad.id * I.Ppad + p];
+            if (mode == MODE_BRANCH) {
+                const double* lut = root_sh + block + (s * C + c) * N;
+#pragma unroll
+                for (int i = 0; i < N; i++) dv[i] = lut[i];
+            } else {
+                const double* tp = root_sh + s * N;
+#pragma unroll
+                for (int i = 0; i < N; i++) dv[i] = tp[i];
+            }
+        } else {
+            const T* dsrc = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) dv[i] = (double)dsrc[i];
+            sc += scale[(size_t)dad.id * I.scale_stride + sidx];
+        }
+        int minsc = sc;
+        double adjust = 1.0;
+        if (safe) {
+            minsc = group_min_int<C>(sc);
+            adjust = Prec<T>::safe_adjust(sc - minsc);
+        }
+        if (mode == MODE_BRANCH) {
+            double lhc;
+            if (dad.is_tip) {
+                lhc = dv[0] * nv[0];
+#pragma unroll
+                for (int i = 1; i < N; i++) lhc = fma(dv[i], nv[i], lhc);
+            } else {
+                const double* val = root_sh + c * N;
+                lhc = val[0] * dv[0] * nv[0];
+#pragma unroll
+                for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
+            }
+            double lh;
+            if (safe) {
+                lh = group_sum_sumvec<C>(lhc * adjust);
+            } else {
+                lh = group_sum_ordered<C>(lhc);
+            }
+            if (c == 0) {
+                lh = fabs(lh) + I.invar[p];
+                const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
+                if (p < I.P) {
+                    I.ptn_lh[p] = lnp;
+                    contrib = lnp * I.freq[p];
+                }
+            }
+        } else {
+            T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) th[i] = (T)(nv[i] * dv[i] * adjust);
+            if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
+        }
+    }
+    if (mode == MODE_BRANCH) {
+        double v[1] = {contrib};
+        block_sum<1>(v, reinterpret_cast<double*>(smem));
+        if (threadIdx.x == 0) block_sums[blockIdx.x] = v[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Derivatives and lnL on the theta buffer (computeLikelihoodDervSIMD
+// 2312-2551, computeLikelihoodFromBufferSIMD 3270-3343). Per pattern:
+// lh = |sum_{c,i} val0 theta| + invar, d1 = sum val1 theta, d2 = sum val2 theta;
+// df += f d1/lh, ddf += f (d2/lh - (d1/lh)^2), lnL += f (log lh + scale).
+template <typename T, int N, int C>
+__global__ void __launch_bounds__(kThreads)
+k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ block_sums) {
+    __shared__ double val[3 * N * C];
+    __shared__ double red[3 * (kThreads / 64)];
+    constexpr int PB = kThreads / C;
+    constexpr int block = N * C;
+    for (int t = threadIdx.x; t < N * C; t += kThreads) {
+        const int cc = t / N, i = t % N;
+        const double cof = m.eval[i] * m.rates[cc];
+        const double v0 = exp(cof * length) * m.props[cc];
+        const double v1 = cof * v0;
+        val[t] = v0;
+        val[N * C + t] = v1;
+        val[2 * N * C + t] = cof * v1;
+    }
+    __syncthreads();
+    const int c = threadIdx.x % C;
+    const int p = blockIdx.x * PB + threadIdx.x / C;
+    double lh = 0.0, d1 = 0.0, d2 = 0.0;
+    if (p < I.Ppad) {
+        const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
+        const double* v0 = val + c * N;
+        const double* v1 = val + N * C + c * N;
+        const double* v2 = val + 2 * N * C + c * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double t = (double)th[i];
+            lh = fma(v0[i], t, lh);
+            if (mode == BUF_DERV) {
+                d1 = fma(v1[i], t, d1);
+                d2 = fma(v2[i], t, d2);
+            }
+        }
+    }
+    lh = group_sum_ordered<C>(lh);
+    if (mode == BUF_DERV) {
+        d1 = group_sum_ordered<C>(d1);
+        d2 = group_sum_ordered<C>(d2);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (c == 0 && p < I.P) {
+        lh = fabs(lh) + I.invar[p];
+        const double f = I.freq[p];
+        const double lnp = log(lh) + I.buffer_scale[p];
+        if (mode == BUF_DERV) {
+            const double inv = 1.0 / lh;
+            const double df_frac = d1 * inv, ddf_frac = d2 * inv;
+            const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
+            acc[0] = tmp1;
+            acc[1] = tmp2 - tmp1 * df_frac;
+            acc[2] = lnp * f;
+        } else {
+            I.ptn_lh[p] = lnp;
+            acc[2] = lnp * f;
+        }
+    }
+    block_sum<3>(acc, red);
+    if (threadIdx.x == 0) {
+        block_sums[3 * blockIdx.x + 0] = acc[0];
+        block_sums[3 * blockIdx.x + 1] = acc[1];
+        block_sums[3 * blockIdx.x + 2] = acc[2];
+    }
+}
+
+// Fixed-order final reduction of per-workgroup partial sums: thread t adds
+// entries t, t+256, ... in index order, then a fixed tree over threads.
+__global__ void __launch_bounds__(kThreads)
+k_reduce(const double* __restrict__ in, int nblocks, int nvals, double* __restrict__ out) {
+    __shared__ double sh[kThreads];
+    for (int j = 0; j < nvals; j++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += kThreads) s += in[(size_t)b * nvals + j];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = kThreads / 2; w >= 1; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = sh[0];
+        __syncthreads();
+    }
+}
+
+// RELL: out[b] = sum_p w[b][p] * ptn_lh[p], one workgroup per replicate.
+__global__ void __launch_bounds__(kThreads)
+k_rell(const float* __restrict__ w, const double* __restrict__ ptn_lh, int P, double* __restrict__ out) {
+    __shared__ double sh[kThreads / 64];
+    const float* wb = w + (size_t)blockIdx.x * P;
+    double s = 0.0;
+    for (int p = threadIdx.x; p < P; p += kThreads) s = fma((double)wb[p], ptn_lh[p], s);
+    double v[1] = {s};
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+}  // namespace plkdev

@@ -1,0 +1,275 @@
+#include "phylotree_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace plk {
+
+PhyloTreeGPU::PhyloTreeGPU(plk_context* ctx, const Workload& w, int precision, int safe_numeric)
+    : tree_(w.tree), initial_len_(w.tree.len), ctx_(ctx) {
+    computed_.assign((size_t)tree_.nnodes * 3, 0);
+    owner_.assign(tree_.nnodes, -1);
+    size_.assign((size_t)tree_.nnodes * 3, 0);
+    if (!ctx_) return;
+    const ModelParams& m = w.model;
+    plk_instance_desc d{};
+    d.ntaxa = tree_.ntaxa;
+    d.nstates = m.nstates;
+    d.ncat = m.ncat;
+    d.npatterns = w.patterns.npattern;
+    d.nslots = num_slots();
+    d.state_unknown = seq_state_unknown(m.type);
+    d.precision = precision;
+    d.safe_numeric = safe_numeric;
+    status_ = plk_instance_create(ctx_, &d, &inst_);
+    if (status_ != PLK_OK) return;
+    status_ = plk_set_patterns(ctx_, inst_, w.patterns.states.data(), w.freq.data(), w.invar.data());
+    if (status_ != PLK_OK) return;
+    status_ = plk_set_model(ctx_, inst_, m.eval.data(), m.evec.data(), m.inv_evec.data(), m.tip.data(),
+                            m.rates.data(), m.props.data());
+}
+
+PhyloTreeGPU::~PhyloTreeGPU() {
+    if (ctx_ && inst_ >= 0) plk_instance_destroy(ctx_, inst_);
+}
+
+int PhyloTreeGPU::subtree_size(int node, int dad) {
+    if (node < tree_.ntaxa) return 1;
+    const int k = tree_.neighbor_index(node, dad);
+    int& s = size_[(size_t)node * 3 + k];
+    if (s > 0) return s;
+    int total = 0;
+    for (int j = 0; j < tree_.deg[node]; j++) {
+        const int ch = tree_.adj[3 * node + j];
+        if (ch != dad) total += subtree_size(ch, node);
+    }
+    s = total;
+    return s;
+}
+
+// computeTraversalInfo(dad_branch = dad->node, dad) (tree/phylotree.cpp:5918-6028)
+void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops) {
+    if (node < tree_.ntaxa) return;
+    const int k = tree_.neighbor_index(node, dad);
+    if (computed_[(size_t)node * 3 + k]) return;
+    int children[2], nc = 0;
+    for (int j = 0; j < tree_.deg[node]; j++) {
+        const int ch = tree_.adj[3 * node + j];
+        if (ch != dad && nc < 2) children[nc++] = ch;
+    }
+    // larger subtree first (5931-5941): keeps the live set of a post-order small
+    if (nc == 2 && subtree_size(children[0], node) < subtree_size(children[1], node))
+        std::swap(children[0], children[1]);
+    for (int i = 0; i < nc; i++) plan_rec(children[i], node, ops);
+    computed_[(size_t)node * 3 + k] = 1;
+    // one buffer per internal node, taken over by the orientation in use
+    // (reorientPartialLh 5898-5912, MemSlotVector::takeover)
+    if (owner_[node] != k) {
+        if (owner_[node] >= 0) computed_[(size_t)node * 3 + owner_[node]] = 0;
+        owner_[node] = k;
+    }
+    plk_op op{};
+    op.dad_slot = slot(node);
+    op.nchildren = nc;
+    for (int i = 0; i < nc; i++) {
+        const int ch = children[i];
+        op.child[i].is_tip = ch < tree_.ntaxa;
+        op.child[i].id = ch < tree_.ntaxa ? ch : slot(ch);
+        op.child[i].length = tree_.length(node, ch);
+    }
+    ops.push_back(op);
+}
+
+void PhyloTreeGPU::plan(int a, int b, std::vector<plk_op>& ops, plk_end& node, plk_end& dad,
+                        double& len) {
+    int nd = a, dd = b;
+    if (nd < tree_.ntaxa) std::swap(nd, dd);   // the leaf goes to dad (2654-2661)
+    ops.clear();
+    plan_rec(nd, dd, ops);
+    plan_rec(dd, nd, ops);
+    node = plk_end{0, slot(nd)};
+    dad = dd < tree_.ntaxa ? plk_end{1, dd} : plk_end{0, slot(dd)};
+    len = tree_.length(a, b);
+}
+
+void PhyloTreeGPU::clear_reverse(int node, int dad) {
+    for (int j = 0; j < tree_.deg[node]; j++) {
+        const int x = tree_.adj[3 * node + j];
+        if (x == dad) continue;
+        computed_[(size_t)node * 3 + j] = 0;
+        clear_reverse(x, node);
+    }
+}
+
+void PhyloTreeGPU::clear_all_partial() {
+    std::fill(computed_.begin(), computed_.end(), 0);
+    theta_a_ = theta_b_ = -1;
+}
+
+void PhyloTreeGPU::reset() {
+    tree_.len = initial_len_;
+    clear_all_partial();
+    cur_a_ = cur_b_ = -1;
+}
+
+void PhyloTreeGPU::set_length(int a, int b, double len) {
+    if (tree_.length(a, b) == len) return;
+    tree_.set_length(a, b, len);
+    clear_reverse(a, b);
+    clear_reverse(b, a);
+    theta_a_ = theta_b_ = -1;
+}
+
+int PhyloTreeGPU::farthest_leaf(int node, int dad, std::vector<int>& height) {
+    if (dad >= 0 && node < tree_.ntaxa) {
+        height[node] = 0;
+        return node;
+    }
+    int res = -1;
+    height[node] = 0;
+    for (int j = 0; j < tree_.deg[node]; j++) {
+        const int ch = tree_.adj[3 * node + j];
+        if (ch == dad) continue;
+        const int leaf = farthest_leaf(ch, node, height);
+        if (height[node] < height[ch] + 1) {
+            height[node] = height[ch] + 1;
+            res = leaf;
+        }
+    }
+    return res;
+}
+
+void PhyloTreeGPU::preorder(int node, int dad, const std::vector<int>& height,
+                            std::vector<int>& nodes, std::vector<int>& nodes2) {
+    if (dad >= 0) {
+        nodes.push_back(node);
+        nodes2.push_back(dad);
+    }
+    std::vector<int> nb(tree_.adj.begin() + 3 * node, tree_.adj.begin() + 3 * node + tree_.deg[node]);
+    std::stable_sort(nb.begin(), nb.end(), [&](int x, int y) { return height[x] < height[y]; });
+    for (int ch : nb)
+        if (ch != dad) preorder(ch, node, height, nodes, nodes2);
+}
+
+// MTree::findFarthestLeaf twice + getPreOrderBranches (tree/mtree.cpp:3040-3081)
+void PhyloTreeGPU::best_traversal(std::vector<int>& nodes, std::vector<int>& nodes2) {
+    std::vector<int> height(tree_.nnodes, 0);
+    const int far1 = farthest_leaf(0, -1, height);
+    const int far2 = farthest_leaf(far1, -1, height);
+    (void)far2;
+    nodes.clear();
+    nodes2.clear();
+    preorder(far1, -1, height, nodes, nodes2);
+}
+
+int PhyloTreeGPU::likelihood_branch(int a, int b, double* lnl, double* pattern_lh) {
+    if (!ctx_) return PLK_ERR_STATE;
+    std::vector<plk_op> ops;
+    plk_end node, dad;
+    double len;
+    plan(a, b, ops, node, dad, len);
+    return plk_likelihood_branch(ctx_, inst_, ops.data(), (int)ops.size(), node, dad, len, lnl,
+                                 pattern_lh);
+}
+
+int PhyloTreeGPU::compute_likelihood(double* lnl, double* pattern_lh) {
+    if (cur_a_ < 0) {
+        std::vector<int> height(tree_.nnodes, 0);
+        const int leaf = farthest_leaf(0, -1, height);
+        cur_a_ = leaf;
+        cur_b_ = tree_.adj[3 * leaf];
+    }
+    // computeLikelihoodBranch(current_it, current_it_back->node)
+    return likelihood_branch(cur_b_, cur_a_, lnl, pattern_lh);
+}
+
+int PhyloTreeGPU::likelihood_derv(int a, int b, double* df, double* ddf) {
+    if (!ctx_) return PLK_ERR_STATE;
+    int st;
+    if (!((theta_a_ == a && theta_b_ == b) || (theta_a_ == b && theta_b_ == a))) {
+        std::vector<plk_op> ops;
+        plk_end node, dad;
+        double len;
+        plan(b, a, ops, node, dad, len);
+        st = plk_likelihood_buffer(ctx_, inst_, ops.data(), (int)ops.size(), node, dad);
+        if (st != PLK_OK) return st;
+        theta_a_ = a;
+        theta_b_ = b;
+    }
+    return plk_likelihood_derv(ctx_, inst_, tree_.length(a, b), df, ddf, nullptr);
+}
+
+int PhyloTreeGPU::likelihood_from_buffer(double* lnl) {
+    if (!ctx_) return PLK_ERR_STATE;
+    if (theta_a_ < 0) return PLK_ERR_STATE;
+    return plk_likelihood_from_buffer(ctx_, inst_, tree_.length(theta_a_, theta_b_), lnl, nullptr);
+}
+
+// PhyloTree::optimizeOneBranch (tree/phylotree.cpp:2594-2644)
+int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, int* n_derv) {
+    if (!ctx_) return PLK_ERR_STATE;
+    cur_a_ = n1;
+    cur_b_ = n2;
+    std::vector<plk_op> ops;
+    plk_end node, dad;
+    double len;
+    plan(n2, n1, ops, node, dad, len);
+    double optx = len;
+    int nd = 0;
+    const int st = plk_optimize_branch(ctx_, inst_, ops.data(), (int)ops.size(), node, dad, len,
+                                       min_len, max_len, max_nr, &optx, nullptr, &nd);
+    if (st != PLK_OK) return st;
+    theta_a_ = n1;
+    theta_b_ = n2;
+    if (optx != len) {
+        tree_.set_length(n1, n2, optx);
+        clear_reverse(n1, n2);
+        clear_reverse(n2, n1);
+    }
+    if (new_len) *new_len = optx;
+    if (n_derv) *n_derv = nd;
+    return PLK_OK;
+}
+
+// PhyloTree::optimizeAllBranches(iterations, tolerance, maxNRStep) (2698-2773)
+int PhyloTreeGPU::optimize_all_branches(int iterations, double tolerance, int max_nr, double* lnl,
+                                        int* n_derv) {
+    if (!ctx_) return PLK_ERR_STATE;
+    std::vector<int> nodes, nodes2;
+    best_traversal(nodes, nodes2);
+    double tree_lh;
+    int st = likelihood_branch(nodes2[0], nodes[0], &tree_lh, nullptr);
+    if (st != PLK_OK) return st;
+    int total_derv = 0;
+    for (int it = 0; it < iterations; it++) {
+        std::vector<double> saved(tree_.len);
+        for (size_t j = 0; j < nodes.size(); j++) {
+            int nd = 0;
+            st = optimize_branch(nodes[j], nodes2[j], max_nr, nullptr, &nd);
+            total_derv += nd;
+            if (st != PLK_OK) return st;
+        }
+        double new_lh;
+        st = likelihood_from_buffer(&new_lh);
+        if (st != PLK_OK) return st;
+        if (new_lh < tree_lh - tolerance * 0.1) {
+            // revert branch lengths when the likelihood decreased (2737-2760)
+            clear_all_partial();
+            tree_.len = saved;
+            st = compute_likelihood(&new_lh, nullptr);
+            if (lnl) *lnl = new_lh;
+            if (n_derv) *n_derv = total_derv;
+            return st;
+        }
+        if (tree_lh <= new_lh && new_lh <= tree_lh + tolerance) {
+            tree_lh = new_lh;
+            break;
+        }
+        tree_lh = new_lh;
+    }
+    if (lnl) *lnl = tree_lh;
+    if (n_derv) *n_derv = total_derv;
+    return PLK_OK;
+}
+
+}  // namespace plk

@@ -1,0 +1,386 @@
+"""Python host mirror of the likelihood engine.
+
+Thin wrappers over libplk.so: `Context` / `Instance` expose the drop-in
+C-ABI (include/plk.h) whose entry points replace the PhyloTree kernel-set
+members computePartialLikelihood, computeLikelihoodBranch,
+computeLikelihoodDerv and computeLikelihoodFromBuffer
+(tree/phylotree.h:899-1001, 1340-1346); `PhyloTree` is the C++ driver that
+plans traversals and runs Newton-Raphson branch optimisation the way
+PhyloTree does (tree/phylotree.cpp:2594-2773, 5918-6028); `Workload` and
+`Patterns` expose the host data formats (site patterns, model eigensystem).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import (OP_DTYPE, PREC_F32, PREC_F64, PlkConfig, PlkEnd, PlkError,
+                    PlkhWorkloadSpec, PlkInstanceDesc, check, dptr, f32ptr, i32ptr, lib,
+                    u8ptr)
+
+__all__ = ["Context", "Instance", "PhyloTree", "Workload", "Patterns", "end", "ops_array",
+           "PREC_F32", "PREC_F64", "PlkError"]
+
+
+def end(is_tip, id_):
+    return PlkEnd(int(is_tip), int(id_))
+
+
+def ops_array(n=0):
+    return np.zeros(n, dtype=OP_DTYPE)
+
+
+def _ops_ptr(ops):
+    if ops is None or len(ops) == 0:
+        return None, 0
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    return ops.ctypes.data_as(ctypes.c_void_p), len(ops)
+
+
+class Workload:
+    """Seeded synthetic likelihood input (tree, model, patterns)."""
+
+    def __init__(self, seq_type=0, ntaxa=16, nsite=1000, ncat=4, mean_len=0.1, alpha=0.5,
+                 p_inv=0.0, gap_frac=0.0, seed=1, edges=None):
+        spec = PlkhWorkloadSpec(seq_type, ntaxa, nsite, ncat, mean_len, alpha, p_inv, gap_frac,
+                                seed)
+        st = ctypes.c_int(0)
+        L = lib()
+        if edges is None:
+            h = L.plkh_workload_create(ctypes.byref(spec), ctypes.byref(st))
+        else:
+            u, v, ln = (np.ascontiguousarray(edges[0], np.int32),
+                        np.ascontiguousarray(edges[1], np.int32),
+                        np.ascontiguousarray(edges[2], np.float64))
+            h = L.plkh_workload_create_on_tree(ctypes.byref(spec), len(u), i32ptr(u), i32ptr(v),
+                                               dptr(ln), ctypes.byref(st))
+        if not h:
+            raise PlkError(st.value, "plkh_workload_create")
+        self._h = h
+        self.seq_type = seq_type
+        dims = [ctypes.c_int() for _ in range(6)]
+        check(L.plkh_workload_dims(h, *[ctypes.byref(d) for d in dims]), "plkh_workload_dims")
+        (self.ntaxa, self.nstates, self.ncat, self.npatterns, self.state_unknown,
+         self.nsite) = [d.value for d in dims]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().plkh_workload_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get(self, name):
+        n, C, P, SU, N = self.nstates, self.ncat, self.npatterns, self.state_unknown, self.ntaxa
+        shapes = {
+            "states": ((N, P), np.uint8), "freq": ((P,), np.float64), "invar": ((P,), np.float64),
+            "eval": ((n,), np.float64), "evec": ((n, n), np.float64),
+            "inv_evec": ((n, n), np.float64), "Q": ((n, n), np.float64),
+            "pi": ((n,), np.float64), "tip": ((SU + 1, n), np.float64),
+            "rates": ((C,), np.float64), "props": ((C,), np.float64),
+            "exch": ((n * (n - 1) // 2,), np.float64),
+            "edges_u": ((2 * N - 3,), np.int32), "edges_v": ((2 * N - 3,), np.int32),
+            "edges_len": ((2 * N - 3,), np.float64),
+            "site_pattern": ((self.nsite,), np.int32), "const_char": ((P,), np.int32),
+            "params": ((4,), np.float64),
+        }
+        shape, dt = shapes[name]
+        out = np.zeros(shape, dtype=dt)
+        check(lib().plkh_workload_get(self._h, name.encode(), out.ctypes.data_as(ctypes.c_void_p)),
+              f"plkh_workload_get({name})")
+        return out
+
+    def model_arrays(self):
+        return {k: self.get(k) for k in ("eval", "evec", "inv_evec", "tip", "rates", "props")}
+
+    def edges(self):
+        return self.get("edges_u"), self.get("edges_v"), self.get("edges_len")
+
+
+class Patterns:
+    """Site-pattern compression of an alignment (alignment/alignment.cpp:1725-1929)."""
+
+    def __init__(self, seq_type, rows):
+        rows = [r.encode() if isinstance(r, str) else bytes(r) for r in rows]
+        nchar = len(rows[0])
+        if any(len(r) != nchar for r in rows):
+            raise ValueError("sequences differ in length")
+        buf = b"".join(rows)
+        st = ctypes.c_int(0)
+        h = lib().plkh_patterns_build(seq_type, len(rows), nchar, buf, ctypes.byref(st))
+        if not h:
+            raise PlkError(st.value, "plkh_patterns_build")
+        self._h = h
+        self.seq_type = seq_type
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().plkh_patterns_dims(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self.nseq, self.nsite, self.npatterns = a.value, b.value, c.value
+        self.states = np.zeros((self.nseq, self.npatterns), np.uint8)
+        self.freq = np.zeros(self.npatterns, np.int32)
+        self.const_char = np.zeros(self.npatterns, np.int32)
+        self.site_pattern = np.zeros(self.nsite, np.int32)
+        check(lib().plkh_patterns_get(h, u8ptr(self.states), i32ptr(self.freq),
+                                      i32ptr(self.const_char), i32ptr(self.site_pattern)),
+              "plkh_patterns_get")
+
+    def invar(self, pi, p_inv):
+        out = np.zeros(self.npatterns)
+        pi = np.ascontiguousarray(pi, np.float64)
+        check(lib().plkh_patterns_invar(self._h, dptr(pi), float(p_inv), dptr(out)),
+              "plkh_patterns_invar")
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().plkh_patterns_destroy(self._h)
+            self._h = None
+
+
+class Context:
+    """plk_context: one device, one stream, an HBM budget."""
+
+    def __init__(self, device=0, stream=None, hbm_budget=0):
+        cfg = PlkConfig(device, stream, hbm_budget)
+        h = ctypes.c_void_p()
+        check(lib().plk_create(ctypes.byref(cfg), ctypes.byref(h)), "plk_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return lib().plk_stream(self._h)
+
+    def memory(self):
+        u, b = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().plk_memory(self._h, ctypes.byref(u), ctypes.byref(b)), "plk_memory")
+        return u.value, b.value
+
+    def synchronize(self):
+        check(lib().plk_synchronize(self._h), "plk_synchronize")
+
+    def profile_start(self):
+        check(lib().plk_profile_start(self._h), "plk_profile_start")
+
+    def profile_stop(self):
+        """Per-kernel {launches, total_ms, bytes, flops} since profile_start."""
+        stats = (_capi.PlkKernelStat * 16)()
+        n = ctypes.c_int()
+        check(lib().plk_profile_stop(self._h, stats, 16, ctypes.byref(n)), "plk_profile_stop")
+        return {s.name.decode(): dict(launches=s.launches, total_ms=s.total_ms, bytes=s.bytes,
+                                      flops=s.flops) for s in stats[:n.value]}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().plk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Instance:
+    """One PhyloTree's device state (plk_instance_*)."""
+
+    def __init__(self, ctx, ntaxa, nstates, ncat, npatterns, nslots, state_unknown,
+                 precision=PREC_F64, safe_numeric=False):
+        self.ctx = ctx
+        self.desc = PlkInstanceDesc(ntaxa, nstates, ncat, npatterns, nslots, state_unknown,
+                                    precision, int(bool(safe_numeric)))
+        iid = ctypes.c_int(-1)
+        check(lib().plk_instance_create(ctx.handle, ctypes.byref(self.desc), ctypes.byref(iid)),
+              "plk_instance_create")
+        self.id = iid.value
+        self.P = npatterns
+        self.block = nstates * ncat
+        self.safe = bool(safe_numeric)
+        self.ncat = ncat
+
+    @staticmethod
+    def footprint(ntaxa, nstates, ncat, npatterns, nslots, state_unknown, precision=PREC_F64,
+                  safe_numeric=False):
+        d = PlkInstanceDesc(ntaxa, nstates, ncat, npatterns, nslots, state_unknown, precision,
+                            int(bool(safe_numeric)))
+        return lib().plk_instance_footprint(ctypes.byref(d))
+
+    @classmethod
+    def from_workload(cls, ctx, w, precision=PREC_F64, safe_numeric=False):
+        inst = cls(ctx, w.ntaxa, w.nstates, w.ncat, w.npatterns, w.ntaxa - 2, w.state_unknown,
+                   precision, safe_numeric)
+        inst.set_patterns(w.get("states"), w.get("freq"), w.get("invar"))
+        m = w.model_arrays()
+        inst.set_model(m["eval"], m["evec"], m["inv_evec"], m["tip"], m["rates"], m["props"])
+        return inst
+
+    def close(self):
+        if self.id >= 0 and self.ctx.handle:
+            lib().plk_instance_destroy(self.ctx.handle, self.id)
+        self.id = -1
+
+    def set_patterns(self, states, freq, invar):
+        s = np.ascontiguousarray(states, np.uint8)
+        f = np.ascontiguousarray(freq, np.float64)
+        v = np.ascontiguousarray(invar, np.float64)
+        check(lib().plk_set_patterns(self.ctx.handle, self.id, u8ptr(s), dptr(f), dptr(v)),
+              "plk_set_patterns")
+
+    def set_model(self, eval_, evec, inv_evec, tip, rates, props):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (eval_, evec, inv_evec, tip, rates,
+                                                                props)]
+        check(lib().plk_set_model(self.ctx.handle, self.id, *[dptr(a) for a in arrs]),
+              "plk_set_model")
+
+    def update_partials(self, ops):
+        p, n = _ops_ptr(ops)
+        check(lib().plk_update_partials(self.ctx.handle, self.id, p, n), "plk_update_partials")
+
+    def likelihood_branch(self, ops, node, dad, length, want_pattern_lh=False):
+        p, n = _ops_ptr(ops)
+        lnl = ctypes.c_double()
+        plh = np.zeros(self.P) if want_pattern_lh else None
+        check(lib().plk_likelihood_branch(self.ctx.handle, self.id, p, n, node, dad, float(length),
+                                          ctypes.byref(lnl), dptr(plh) if plh is not None else None),
+              "plk_likelihood_branch")
+        return (lnl.value, plh) if want_pattern_lh else lnl.value
+
+    def likelihood_buffer(self, ops, node, dad):
+        p, n = _ops_ptr(ops)
+        check(lib().plk_likelihood_buffer(self.ctx.handle, self.id, p, n, node, dad),
+              "plk_likelihood_buffer")
+
+    def likelihood_derv(self, length):
+        df, ddf, lnl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().plk_likelihood_derv(self.ctx.handle, self.id, float(length), ctypes.byref(df),
+                                        ctypes.byref(ddf), ctypes.byref(lnl)), "plk_likelihood_derv")
+        return df.value, ddf.value, lnl.value
+
+    def likelihood_from_buffer(self, length, want_pattern_lh=False):
+        lnl = ctypes.c_double()
+        plh = np.zeros(self.P) if want_pattern_lh else None
+        check(lib().plk_likelihood_from_buffer(self.ctx.handle, self.id, float(length),
+                                               ctypes.byref(lnl),
+                                               dptr(plh) if plh is not None else None),
+              "plk_likelihood_from_buffer")
+        return (lnl.value, plh) if want_pattern_lh else lnl.value
+
+    def optimize_branch(self, ops, node, dad, length, min_len=1e-6, max_len=10.0, max_nr=100):
+        p, n = _ops_ptr(ops)
+        out_len, out_lnl, nd = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(lib().plk_optimize_branch(self.ctx.handle, self.id, p, n, node, dad, float(length),
+                                        float(min_len), float(max_len), int(max_nr),
+                                        ctypes.byref(out_len), ctypes.byref(out_lnl),
+                                        ctypes.byref(nd)), "plk_optimize_branch")
+        return out_len.value, out_lnl.value, nd.value
+
+    def rell(self, weights):
+        w = np.ascontiguousarray(weights, np.float32)
+        out = np.zeros(w.shape[0])
+        check(lib().plk_rell(self.ctx.handle, self.id, f32ptr(w), w.shape[0], dptr(out)),
+              "plk_rell")
+        return out
+
+    def read_partial(self, slot):
+        part = np.zeros((self.P, self.block))
+        scale = np.zeros((self.P, self.ncat) if self.safe else (self.P,), np.int32)
+        check(lib().plk_read_partial(self.ctx.handle, self.id, int(slot), dptr(part),
+                                     i32ptr(scale)), "plk_read_partial")
+        return part, scale
+
+
+class PhyloTree:
+    """PhyloTree-shaped C++ driver (traversal planning + Newton branch optimisation)."""
+
+    def __init__(self, ctx, workload, precision=PREC_F64, safe_numeric=False):
+        st = ctypes.c_int(0)
+        h = lib().plkh_tree_create(ctx.handle if ctx is not None else None, workload.handle,
+                                   precision, int(bool(safe_numeric)), ctypes.byref(st))
+        if not h:
+            raise PlkError(st.value, "plkh_tree_create")
+        self._h = h
+        self.ctx = ctx
+        self.workload = workload
+        self.P = workload.npatterns
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().plkh_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def instance_id(self):
+        return lib().plkh_tree_instance(self._h)
+
+    def branches(self):
+        nb = lib().plkh_tree_num_branches(self._h)
+        a, b = np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+        check(lib().plkh_tree_branches(self._h, i32ptr(a), i32ptr(b)), "plkh_tree_branches")
+        return list(zip(a.tolist(), b.tolist()))
+
+    def get_length(self, a, b):
+        x = ctypes.c_double()
+        check(lib().plkh_tree_get_length(self._h, a, b, ctypes.byref(x)), "plkh_tree_get_length")
+        return x.value
+
+    def set_length(self, a, b, length):
+        check(lib().plkh_tree_set_length(self._h, a, b, float(length)), "plkh_tree_set_length")
+
+    def clear_all_partial(self):
+        check(lib().plkh_tree_clear_all_partial(self._h), "plkh_tree_clear_all_partial")
+
+    def reset(self):
+        check(lib().plkh_tree_reset(self._h), "plkh_tree_reset")
+
+    def plan(self, a, b):
+        cap = 2 * self.workload.ntaxa
+        ops = ops_array(cap)
+        n = ctypes.c_int()
+        node, dad = PlkEnd(), PlkEnd()
+        length = ctypes.c_double()
+        check(lib().plkh_tree_plan(self._h, a, b, ops.ctypes.data_as(ctypes.c_void_p), cap,
+                                   ctypes.byref(n), ctypes.byref(node), ctypes.byref(dad),
+                                   ctypes.byref(length)), "plkh_tree_plan")
+        return ops[:n.value].copy(), node, dad, length.value
+
+    def compute_likelihood(self, want_pattern_lh=False):
+        lnl = ctypes.c_double()
+        plh = np.zeros(self.P) if want_pattern_lh else None
+        check(lib().plkh_tree_compute_likelihood(self._h, ctypes.byref(lnl),
+                                                 dptr(plh) if plh is not None else None),
+              "plkh_tree_compute_likelihood")
+        return (lnl.value, plh) if want_pattern_lh else lnl.value
+
+    def likelihood_branch(self, a, b):
+        lnl = ctypes.c_double()
+        check(lib().plkh_tree_likelihood_branch(self._h, a, b, ctypes.byref(lnl), None),
+              "plkh_tree_likelihood_branch")
+        return lnl.value
+
+    def likelihood_derv(self, a, b):
+        df, ddf = ctypes.c_double(), ctypes.c_double()
+        check(lib().plkh_tree_likelihood_derv(self._h, a, b, ctypes.byref(df), ctypes.byref(ddf)),
+              "plkh_tree_likelihood_derv")
+        return df.value, ddf.value
+
+    def optimize_branch(self, a, b, max_nr=100):
+        x, nd = ctypes.c_double(), ctypes.c_int()
+        check(lib().plkh_tree_optimize_branch(self._h, a, b, max_nr, ctypes.byref(x),
+                                              ctypes.byref(nd)), "plkh_tree_optimize_branch")
+        return x.value, nd.value
+
+    def optimize_all_branches(self, iterations=1, tolerance=0.001, max_nr=100):
+        lnl, nd = ctypes.c_double(), ctypes.c_int()
+        check(lib().plkh_tree_optimize_all_branches(self._h, iterations, tolerance, max_nr,
+                                                    ctypes.byref(lnl), ctypes.byref(nd)),
+              "plkh_tree_optimize_all_branches")
+        return lnl.value, nd.value
+
+    def set_branch_bounds(self, min_len, max_len):
+        check(lib().plkh_tree_set_branch_bounds(self._h, min_len, max_len),
+              "plkh_tree_set_branch_bounds")

@@ -1,0 +1,945 @@
+/*
+ * oracle.c - FP64 CPU restatement of IQ-TREE 2's reversible likelihood
+ * kernels. TEST INFRASTRUCTURE ONLY (parity oracle and CPU baseline); the
+ * product library never links or calls this file.
+ *
+ * Every function cites the reference code it restates. Arithmetic follows the
+ * reference helpers' grouping (tree/phylokernelnew.h:81-796) on 4-wide
+ * pattern vectors, so results agree with the reference FMA kernel to a few
+ * ulps; exp/log come from libm instead of VCL 1.30.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double v4d __attribute__((vector_size(32), aligned(32)));
+#define VS 4
+#define SCALING_THRESHOLD 0x1p-256           /* tree/phylotree.h:74-80 */
+#define SCALING_THRESHOLD_EXP 256
+#define LOG_SCALING_THRESHOLD (-177.4456782233459932741)
+#define INLINE static inline __attribute__((always_inline))
+
+struct orc_tree {
+    int ntaxa, n, C, P, Ppad, nslots, SU, safe, nthreads, npackets;
+    size_t block;
+    uint8_t* states;       /* [ntaxa][Ppad], padding = SU */
+    double *freq, *invar;  /* [Ppad] */
+    double *eval, *evec, *inv_evec, *tip, *rates, *props;
+    double* partial;       /* [nslots][Ppad*block] interleaved */
+    uint16_t* scale;       /* [nslots][Ppad*(safe?C:1)] */
+    double* theta;         /* [Ppad*block] */
+    double* buffer_scale;  /* [Ppad] */
+    double* pattern_lh;    /* [Ppad] */
+    int theta_ready;
+    double* info;          /* per-op echild + leaf tables */
+    size_t info_cap;
+    double* tipbuf;        /* per-packet tip staging [npackets][2*block*VS] */
+};
+
+static void* amalloc(size_t bytes) {
+    void* p = NULL;
+    if (bytes == 0) bytes = 64;
+    if (posix_memalign(&p, 64, (bytes + 63) & ~(size_t)63) != 0) return NULL;
+    memset(p, 0, bytes);
+    return p;
+}
+
+INLINE v4d vset1(double a) { return (v4d){a, a, a, a}; }
+INLINE v4d vabs(v4d a) {
+    v4d r;
+    for (int k = 0; k < VS; k++) r[k] = fabs(a[k]);
+    return r;
+}
+INLINE v4d vmax(v4d a, v4d b) {
+    v4d r;
+    for (int k = 0; k < VS; k++) r[k] = a[k] > b[k] ? a[k] : b[k];
+    return r;
+}
+INLINE double hadd(v4d a) { return (a[0] + a[1]) + (a[2] + a[3]); }
+
+/* ---- helpers restating tree/phylokernelnew.h:81-796 ----------------- */
+
+/* dotProductVec, JAMES_VERSION (81-126): A scalar row, B pattern vectors */
+INLINE v4d dot_vec_s(const double* A, const v4d* B, const int N) {
+    if (N == 4) return (vset1(A[0]) * B[0] + vset1(A[1]) * B[1]) + (vset1(A[2]) * B[2] + vset1(A[3]) * B[3]);
+    const int cruft = N & 3;
+    if (N == cruft) {
+        v4d X = vset1(0.0);
+        for (int i = 0; i < cruft; i++) X = vset1(A[i]) * B[i] + X;
+        return X;
+    }
+    v4d X = vset1(A[0]) * B[0], v1 = vset1(A[1]) * B[1], v2 = vset1(A[2]) * B[2],
+        v3 = vset1(A[3]) * B[3];
+    int i = 4;
+    for (; i < N - cruft; i += 4) {
+        X = vset1(A[i]) * B[i] + X;
+        v1 = vset1(A[i + 1]) * B[i + 1] + v1;
+        v2 = vset1(A[i + 2]) * B[i + 2] + v2;
+        v3 = vset1(A[i + 3]) * B[i + 3] + v3;
+    }
+    if (cruft > 0) {
+        X = vset1(A[i]) * B[i] + X;
+        if (cruft > 1) {
+            v1 = vset1(A[i + 1]) * B[i + 1] + v1;
+            if (cruft > 2) v2 = vset1(A[i + 2]) * B[i + 2] + v2;
+        }
+    }
+    return (X + v1) + (v2 + v3);
+}
+
+/* same with both operands pattern vectors */
+INLINE v4d dot_vec_v(const v4d* A, const v4d* B, const int N) {
+    if (N == 4) return (A[0] * B[0] + A[1] * B[1]) + (A[2] * B[2] + A[3] * B[3]);
+    const int cruft = N & 3;
+    if (N == cruft) {
+        v4d X = vset1(0.0);
+        for (int i = 0; i < cruft; i++) X = A[i] * B[i] + X;
+        return X;
+    }
+    v4d X = A[0] * B[0], v1 = A[1] * B[1], v2 = A[2] * B[2], v3 = A[3] * B[3];
+    int i = 4;
+    for (; i < N - cruft; i += 4) {
+        X = A[i] * B[i] + X;
+        v1 = A[i + 1] * B[i + 1] + v1;
+        v2 = A[i + 2] * B[i + 2] + v2;
+        v3 = A[i + 3] * B[i + 3] + v3;
+    }
+    if (cruft > 0) {
+        X = A[i] * B[i] + X;
+        if (cruft > 1) {
+            v1 = A[i + 1] * B[i + 1] + v1;
+            if (cruft > 2) v2 = A[i + 2] * B[i + 2] + v2;
+        }
+    }
+    return (X + v1) + (v2 + v3);
+}
+
+/* dotProductDualVec (214-314): (A.B) * (C.D), A and C scalar rows */
+INLINE v4d dot_dual(const double* A, const v4d* B, const double* Cm, const v4d* D, const int N) {
+    if (N == 4) {
+        return ((vset1(A[1]) * B[1] + vset1(A[0]) * B[0]) + (vset1(A[3]) * B[3] + vset1(A[2]) * B[2])) *
+               ((vset1(Cm[1]) * D[1] + vset1(Cm[0]) * D[0]) + (vset1(Cm[3]) * D[3] + vset1(Cm[2]) * D[2]));
+    }
+    if (N < 4) {
+        v4d ab = vset1(A[0]) * B[0], cd = vset1(Cm[0]) * D[0];
+        for (int i = 1; i < N; i++) {
+            ab = vset1(A[i]) * B[i] + ab;
+            cd = vset1(Cm[i]) * D[i] + cd;
+        }
+        return ab * cd;
+    }
+    switch (N % 4) {
+    case 0: {
+        v4d AB[4], CD[4];
+        for (int j = 0; j < 4; j++) { AB[j] = vset1(A[j]) * B[j]; CD[j] = vset1(Cm[j]) * D[j]; }
+        for (int i = 4; i < N; i += 4)
+            for (int j = 0; j < 4; j++) {
+                AB[j] = vset1(A[i + j]) * B[i + j] + AB[j];
+                CD[j] = vset1(Cm[i + j]) * D[i + j] + CD[j];
+            }
+        return ((AB[0] + AB[1]) + (AB[2] + AB[3])) * (((CD[0] + CD[1]) + CD[2]) + CD[3]);
+    }
+    case 1: {
+        const int Nm1 = N - 1;
+        v4d AB[4], CD[4];
+        for (int j = 0; j < 4; j++) { AB[j] = vset1(A[j]) * B[j]; CD[j] = vset1(Cm[j]) * D[j]; }
+        for (int i = 4; i < Nm1; i += 4)
+            for (int j = 0; j < 4; j++) {
+                AB[j] = vset1(A[i + j]) * B[i + j] + AB[j];
+                CD[j] = vset1(Cm[i + j]) * D[i + j] + CD[j];
+            }
+        return (vset1(A[Nm1]) * B[Nm1] + ((AB[0] + AB[1]) + (AB[2] + AB[3]))) *
+               (vset1(Cm[Nm1]) * D[Nm1] + (((CD[0] + CD[1]) + CD[2]) + CD[3]));
+    }
+    case 2: {
+        v4d AB[2], CD[2];
+        for (int j = 0; j < 2; j++) { AB[j] = vset1(A[j]) * B[j]; CD[j] = vset1(Cm[j]) * D[j]; }
+        for (int i = 2; i < N; i += 2)
+            for (int j = 0; j < 2; j++) {
+                AB[j] = vset1(A[i + j]) * B[i + j] + AB[j];
+                CD[j] = vset1(Cm[i + j]) * D[i + j] + CD[j];
+            }
+        return (AB[0] + AB[1]) * (CD[0] + CD[1]);
+    }
+    default: {
+        v4d AB[2], CD[2];
+        for (int j = 0; j < 2; j++) { AB[j] = vset1(A[j]) * B[j]; CD[j] = vset1(Cm[j]) * D[j]; }
+        for (int i = 2; i < N - 1; i += 2)
+            for (int j = 0; j < 2; j++) {
+                AB[j] = vset1(A[i + j]) * B[i + j] + AB[j];
+                CD[j] = vset1(Cm[i + j]) * D[i + j] + CD[j];
+            }
+        return (vset1(A[N - 1]) * B[N - 1] + (AB[0] + AB[1])) *
+               (vset1(Cm[N - 1]) * D[N - 1] + (CD[0] + CD[1]));
+    }
+    }
+}
+
+/* productVecMat with max (458-567): X[i] = sum_j A[j] M[i][j]; Xmax = max|X| */
+INLINE void prod_vec_mat(const v4d* A, const double* M, v4d* X, v4d* Xmax, const int N) {
+    if (N == 4) {
+        for (int i = 0; i < 4; i++) {
+            const double* m = M + 4 * i;
+            X[i] = (A[1] * vset1(m[1]) + A[0] * vset1(m[0])) + (A[3] * vset1(m[3]) + A[2] * vset1(m[2]));
+            if (Xmax) *Xmax = vmax(*Xmax, vabs(X[i]));
+        }
+        return;
+    }
+    for (int i = 0; i < N; i++, M += N) {
+        v4d r;
+        if (N < 4) {
+            r = A[0] * vset1(M[0]);
+            for (int j = 1; j < N; j++) r = A[j] * vset1(M[j]) + r;
+        } else if (N % 4 == 0 || N % 4 == 1) {
+            const int lim = (N % 4 == 0) ? N : N - 1;
+            v4d Vv[4];
+            for (int j = 0; j < 4; j++) Vv[j] = A[j] * vset1(M[j]);
+            for (int x = 4; x < lim; x += 4)
+                for (int j = 0; j < 4; j++) Vv[j] = A[x + j] * vset1(M[x + j]) + Vv[j];
+            r = (Vv[0] + Vv[1]) + (Vv[2] + Vv[3]);
+            if (N % 4 == 1) r = A[N - 1] * vset1(M[N - 1]) + r;
+        } else {
+            const int lim = (N % 4 == 2) ? N : N - 1;
+            v4d Vv[2];
+            for (int j = 0; j < 2; j++) Vv[j] = A[j] * vset1(M[j]);
+            for (int x = 2; x < lim; x += 2)
+                for (int j = 0; j < 2; j++) Vv[j] = A[x + j] * vset1(M[x + j]) + Vv[j];
+            r = Vv[0] + Vv[1];
+            if (N % 4 == 3) r = A[N - 1] * vset1(M[N - 1]) + r;
+        }
+        X[i] = r;
+        if (Xmax) *Xmax = vmax(*Xmax, vabs(r));
+    }
+}
+
+/* dotProduct3Vec (737-796): sum A[i]*B[i]*C[i], A scalar row */
+INLINE v4d dot3(const double* A, const v4d* B, const v4d* Cv, const int N) {
+    switch (N % 4) {
+    case 0:
+    case 1: {
+        const int lim = (N % 4 == 0) ? N : N - 1;
+        v4d Vv[4];
+        for (int j = 0; j < 4; j++) Vv[j] = vset1(A[j]) * B[j] * Cv[j];
+        for (int i = 4; i < lim; i += 4)
+            for (int j = 0; j < 4; j++) Vv[j] = (vset1(A[i + j]) * B[i + j]) * Cv[i + j] + Vv[j];
+        v4d X = (Vv[0] + Vv[1]) + (Vv[2] + Vv[3]);
+        if (N % 4 == 1) X = (vset1(A[N - 1]) * B[N - 1]) * Cv[N - 1] + X;
+        return X;
+    }
+    default: {
+        const int lim = (N % 4 == 2) ? N : N - 1;
+        v4d Vv[2];
+        for (int j = 0; j < 2; j++) Vv[j] = vset1(A[j]) * B[j] * Cv[j];
+        for (int i = 2; i < lim; i += 2)
+            for (int j = 0; j < 2; j++) Vv[j] = (vset1(A[i + j]) * B[i + j]) * Cv[i + j] + Vv[j];
+        v4d X = Vv[0] + Vv[1];
+        if (N % 4 == 3) X = (vset1(A[N - 1]) * B[N - 1]) * Cv[N - 1] + X;
+        return X;
+    }
+    }
+}
+
+/* dotProductTriple (666-725) over N = block entries; pairwise accumulators */
+INLINE void dot_triple(const double* A, const double* B, const double* Cm, const v4d* D,
+                       v4d* X, v4d* Y, v4d* Z, const int N, const int nstates) {
+    v4d AD[2], BD[2], CD[2];
+    for (int j = 0; j < 2; j++) {
+        AD[j] = vset1(A[j]) * D[j];
+        BD[j] = vset1(B[j]) * D[j];
+        CD[j] = vset1(Cm[j]) * D[j];
+    }
+    const int lim = ((nstates & 1) == 0) ? N : N - 1;
+    for (int i = 2; i < lim; i += 2)
+        for (int j = 0; j < 2; j++) {
+            AD[j] = vset1(A[i + j]) * D[i + j] + AD[j];
+            BD[j] = vset1(B[i + j]) * D[i + j] + BD[j];
+            CD[j] = vset1(Cm[i + j]) * D[i + j] + CD[j];
+        }
+    if ((nstates & 1) == 0) {
+        *X = AD[0] + AD[1];
+        *Y = BD[0] + BD[1];
+        *Z = CD[0] + CD[1];
+    } else {
+        *X = vset1(A[N - 1]) * D[N - 1] + (AD[0] + AD[1]);
+        *Y = vset1(B[N - 1]) * D[N - 1] + (BD[0] + BD[1]);
+        *Z = vset1(Cm[N - 1]) * D[N - 1] + (CD[0] + CD[1]);
+    }
+}
+
+/* ---- tree object --------------------------------------------------------- */
+
+orc_tree* orc_create(int ntaxa, int nstates, int ncat, int npatterns, int nslots,
+                     int state_unknown, int safe_numeric, int nthreads) {
+    orc_tree* t = (orc_tree*)calloc(1, sizeof(orc_tree));
+    t->ntaxa = ntaxa;
+    t->n = nstates;
+    t->C = ncat;
+    t->P = npatterns;
+    t->Ppad = (npatterns + VS - 1) / VS * VS;
+    t->nslots = nslots;
+    t->SU = state_unknown;
+    t->safe = safe_numeric;
+    t->block = (size_t)nstates * ncat;
+    const size_t Pp = (size_t)t->Ppad;
+    t->states = (uint8_t*)amalloc((size_t)ntaxa * Pp);
+    memset(t->states, state_unknown, (size_t)ntaxa * Pp);
+    t->freq = (double*)amalloc(Pp * 8);
+    t->invar = (double*)amalloc(Pp * 8);
+    const size_t nn = (size_t)nstates * nstates;
+    t->eval = (double*)amalloc(nstates * 8);
+    t->evec = (double*)amalloc(nn * 8);
+    t->inv_evec = (double*)amalloc(nn * 8);
+    t->tip = (double*)amalloc((size_t)(state_unknown + 1) * nstates * 8);
+    t->rates = (double*)amalloc(ncat * 8);
+    t->props = (double*)amalloc(ncat * 8);
+    t->partial = (double*)amalloc((size_t)nslots * Pp * t->block * 8);
+    t->scale = (uint16_t*)amalloc((size_t)nslots * Pp * (safe_numeric ? ncat : 1) * 2);
+    t->theta = (double*)amalloc(Pp * t->block * 8);
+    t->buffer_scale = (double*)amalloc(Pp * 8);
+    t->pattern_lh = (double*)amalloc(Pp * 8);
+    t->tipbuf = NULL;
+    orc_set_threads(t, nthreads);
+    return t;
+}
+
+void orc_set_threads(orc_tree* t, int nthreads) {
+    /* PhyloTree::setNumThreads (tree/phylotreesse.cpp:42-52) */
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 1 && nthreads > t->P / 8) nthreads = t->P / 8 > 1 ? t->P / 8 : 1;
+    t->nthreads = nthreads;
+    t->npackets = nthreads == 1 ? 1 : nthreads * 2;
+    free(t->tipbuf);
+    t->tipbuf = (double*)amalloc((size_t)t->npackets * 3 * t->block * VS * 8);
+}
+
+void orc_destroy(orc_tree* t) {
+    if (!t) return;
+    free(t->states); free(t->freq); free(t->invar); free(t->eval); free(t->evec);
+    free(t->inv_evec); free(t->tip); free(t->rates); free(t->props); free(t->partial);
+    free(t->scale); free(t->theta); free(t->buffer_scale); free(t->pattern_lh); free(t->info);
+    free(t->tipbuf);
+    free(t);
+}
+
+void orc_set_patterns(orc_tree* t, const uint8_t* states, const double* freq, const double* invar) {
+    for (int s = 0; s < t->ntaxa; s++)
+        memcpy(t->states + (size_t)s * t->Ppad, states + (size_t)s * t->P, t->P);
+    memcpy(t->freq, freq, (size_t)t->P * 8);
+    memcpy(t->invar, invar, (size_t)t->P * 8);
+    for (int p = t->P; p < t->Ppad; p++) { t->freq[p] = 0.0; t->invar[p] = 0.0; }
+}
+
+void orc_set_model(orc_tree* t, const double* eval, const double* evec, const double* inv_evec,
+                   const double* tip_eigen, const double* rates, const double* props) {
+    const size_t nn = (size_t)t->n * t->n;
+    memcpy(t->eval, eval, t->n * 8);
+    memcpy(t->evec, evec, nn * 8);
+    memcpy(t->inv_evec, inv_evec, nn * 8);
+    memcpy(t->tip, tip_eigen, (size_t)(t->SU + 1) * t->n * 8);
+    memcpy(t->rates, rates, t->C * 8);
+    memcpy(t->props, props, t->C * 8);
+}
+
+/* computeBounds (tree/phylokernelnew.h:1112-1143) */
+static void compute_bounds(int threads, int packets, size_t elements, size_t* limits) {
+    elements = (elements + VS - 1) / VS * VS;
+    size_t start = 0;
+    int k = 0;
+    for (int wave = packets / threads; wave >= 1; --wave) {
+        size_t this_wave = elements - start;
+        if (1 < wave) this_wave = (this_wave * 3) / 4;
+        this_wave = (this_wave + VS - 1) / VS * VS;
+        const size_t stop = start + this_wave;
+        for (int togo = threads; togo >= 1; --togo) {
+            limits[k++] = start;
+            size_t bs = (stop - start) / togo;
+            bs = (bs + VS - 1) / VS * VS;
+            start += bs;
+        }
+    }
+    limits[k] = elements;
+    for (int i = 1; i <= packets; i++)
+        if (limits[i] > elements) limits[i] = elements;
+}
+
+/* per-op precomputed tables (computePartialInfo, tree/phylokernelnew.h:1008-1109):
+ * echild[child][c][x][i] = evec[x][i] * exp(eval[i] * r_c * t_child) and, for
+ * tip children, leaf[child][s][c][x] = sum_i echild[c][x][i] tip[s][i] with
+ * leaf[SU] = 1. */
+typedef struct {
+    const double* echild[2];
+    const double* leaf[2];
+} op_info;
+
+static void partial_info(orc_tree* t, const orc_op* op, double* ech, double* leaf) {
+    const int n = t->n, C = t->C;
+    const size_t block = t->block;
+    for (int k = 0; k < 2; k++) {
+        double* E = ech + (size_t)k * block * n;
+        for (int c = 0; c < C; c++) {
+            const double len = t->rates[c] * op->child[k].length;
+            for (int x = 0; x < n; x++)
+                for (int i = 0; i < n; i++)
+                    E[((size_t)c * n + x) * n + i] = t->evec[x * n + i] * exp(t->eval[i] * len);
+        }
+        if (op->child[k].is_tip) {
+            double* L = leaf + (size_t)k * (t->SU + 1) * block;
+            for (int s = 0; s < t->SU; s++)
+                for (int c = 0; c < C; c++)
+                    for (int x = 0; x < n; x++) {
+                        const double* e = E + ((size_t)c * n + x) * n;
+                        double v = e[0] * t->tip[s * n + 0];
+                        for (int i = 1; i < n; i++) v += e[i] * t->tip[s * n + i];
+                        L[(size_t)s * block + c * n + x] = v;
+                    }
+            for (size_t x = 0; x < block; x++) L[(size_t)t->SU * block + x] = 1.0;
+        }
+    }
+}
+
+static void ensure_info(orc_tree* t, int nops) {
+    const size_t per = 2 * t->block * t->n + 2 * (size_t)(t->SU + 1) * t->block;
+    const size_t need = (size_t)(nops > 0 ? nops : 1) * per;
+    if (need > t->info_cap) {
+        free(t->info);
+        t->info = (double*)amalloc(need * 8);
+        t->info_cap = need;
+    }
+}
+
+static double* op_echild(orc_tree* t, int i) {
+    const size_t per = 2 * t->block * t->n + 2 * (size_t)(t->SU + 1) * t->block;
+    return t->info + (size_t)i * per;
+}
+static double* op_leaf(orc_tree* t, int i) { return op_echild(t, i) + 2 * t->block * t->n; }
+
+INLINE double* slot_ptr(orc_tree* t, int slot) {
+    return t->partial + (size_t)slot * t->Ppad * t->block;
+}
+INLINE uint16_t* scale_ptr(orc_tree* t, int slot) {
+    return t->scale + (size_t)slot * t->Ppad * (t->safe ? t->C : 1);
+}
+
+/* load interleaved tip vectors of 4 patterns from a [SU+1][block] table */
+INLINE void load_tip(orc_tree* t, const double* table, int taxon, size_t ptn, double* vec) {
+    const size_t block = t->block;
+    const uint8_t* row = t->states + (size_t)taxon * t->Ppad;
+    for (int x = 0; x < VS; x++) {
+        const double* src = table + (size_t)row[ptn + x] * block;
+        for (size_t i = 0; i < block; i++) vec[i * VS + x] = src[i];
+    }
+}
+
+/* scaling of one 4-pattern vector block (scaleLikelihood semantics,
+ * tree/phylokernelnew.h:1993-2030): rescale a lane when its max |partial| is
+ * below 2^-256 and its pattern has no +I mass */
+INLINE void scale_lanes(double* dad, uint16_t* sc, const double* invar, v4d lh_max,
+                        size_t count, size_t stride_sc) {
+    for (int x = 0; x < VS; x++)
+        if (lh_max[x] < SCALING_THRESHOLD && invar[x] == 0.0) {
+            for (size_t i = 0; i < count; i++)
+                dad[i * VS + x] = ldexp(dad[i * VS + x], SCALING_THRESHOLD_EXP);
+            sc[x * stride_sc] += 1;
+        }
+}
+
+/* computePartialLikelihoodSIMD for binary nodes (tree/phylokernelnew.h:1632-2031) */
+INLINE void partial_body(orc_tree* t, const orc_op* op, const op_info* inf, size_t lo, size_t hi,
+                         double* tipbuf, const int n) {
+    const int C = t->C, safe = t->safe;
+    const size_t block = (size_t)n * C;
+    const int nsc = safe ? C : 1;
+    int left = 0, right = 1;
+    /* a tip child, when there is one, goes left (1513-1524) */
+    if (!op->child[0].is_tip && op->child[1].is_tip) { left = 1; right = 0; }
+    const orc_child* L = &op->child[left];
+    const orc_child* R = &op->child[right];
+    double* dad_base = slot_ptr(t, op->dad_slot);
+    uint16_t* sdad = scale_ptr(t, op->dad_slot);
+    v4d tmp[64];
+    if (L->is_tip && R->is_tip) {
+        /* tip-tip: scale counters zero, no rescale (1632-1745) */
+        memset(sdad + lo * nsc, 0, (hi - lo) * nsc * sizeof(uint16_t));
+        double* vl = tipbuf;
+        double* vr = tipbuf + block * VS;
+        for (size_t ptn = lo; ptn < hi; ptn += VS) {
+            load_tip(t, inf->leaf[left], L->id, ptn, vl);
+            load_tip(t, inf->leaf[right], R->id, ptn, vr);
+            v4d* dad = (v4d*)(dad_base + ptn * block);
+            const v4d* a = (const v4d*)vl;
+            const v4d* b = (const v4d*)vr;
+            for (int c = 0; c < C; c++) {
+                for (int x = 0; x < n; x++) tmp[x] = a[c * n + x] * b[c * n + x];
+                prod_vec_mat(tmp, t->inv_evec, dad + c * n, NULL, n);
+            }
+        }
+        return;
+    }
+    if (L->is_tip) {
+        /* tip-internal (1748-1905) */
+        const double* rbase = slot_ptr(t, R->id);
+        const uint16_t* sright = scale_ptr(t, R->id);
+        memcpy(sdad + lo * nsc, sright + lo * nsc, (hi - lo) * nsc * sizeof(uint16_t));
+        double* vl = tipbuf;
+        const double* ER = inf->echild[right];
+        for (size_t ptn = lo; ptn < hi; ptn += VS) {
+            load_tip(t, inf->leaf[left], L->id, ptn, vl);
+            v4d* dad = (v4d*)(dad_base + ptn * block);
+            const v4d* pr = (const v4d*)(rbase + ptn * block);
+            const v4d* a = (const v4d*)vl;
+            v4d lh_max = vset1(0.0);
+            for (int c = 0; c < C; c++) {
+                if (safe) lh_max = vset1(0.0);
+                const double* e = ER + (size_t)c * n * n;
+                for (int x = 0; x < n; x++) tmp[x] = a[c * n + x] * dot_vec_s(e + x * n, pr + c * n, n);
+                prod_vec_mat(tmp, t->inv_evec, dad + c * n, &lh_max, n);
+                if (safe)
+                    scale_lanes((double*)(dad + c * n), sdad + ptn * C + c, t->invar + ptn, lh_max, n, C);
+            }
+            if (!safe) scale_lanes((double*)dad, sdad + ptn, t->invar + ptn, lh_max, block, 1);
+        }
+        return;
+    }
+    /* internal-internal (1907-2031) */
+    const double* lbase = slot_ptr(t, L->id);
+    const double* rbase = slot_ptr(t, R->id);
+    const uint16_t* sl = scale_ptr(t, L->id);
+    const uint16_t* sr = scale_ptr(t, R->id);
+    const double* EL = inf->echild[left];
+    const double* ER = inf->echild[right];
+    for (size_t ptn = lo; ptn < hi; ptn += VS) {
+        v4d* dad = (v4d*)(dad_base + ptn * block);
+        const v4d* pl = (const v4d*)(lbase + ptn * block);
+        const v4d* pr = (const v4d*)(rbase + ptn * block);
+        v4d lh_max = vset1(0.0);
+        if (!safe)
+            for (int x = 0; x < VS; x++) sdad[ptn + x] = sl[ptn + x] + sr[ptn + x];
+        for (int c = 0; c < C; c++) {
+            if (safe) {
+                lh_max = vset1(0.0);
+                for (int x = 0; x < VS; x++)
+                    sdad[(ptn + x) * C + c] = sl[(ptn + x) * C + c] + sr[(ptn + x) * C + c];
+            }
+            const double* el = EL + (size_t)c * n * n;
+            const double* er = ER + (size_t)c * n * n;
+            for (int x = 0; x < n; x++)
+                tmp[x] = dot_dual(el + x * n, pl + c * n, er + x * n, pr + c * n, n);
+            prod_vec_mat(tmp, t->inv_evec, dad + c * n, &lh_max, n);
+            if (safe) scale_lanes((double*)(dad + c * n), sdad + ptn * C + c, t->invar + ptn, lh_max, n, C);
+        }
+        if (!safe) scale_lanes((double*)dad, sdad + ptn, t->invar + ptn, lh_max, block, 1);
+    }
+}
+
+static void partial_dispatch(orc_tree* t, const orc_op* op, const op_info* inf, size_t lo, size_t hi,
+                             double* tipbuf) {
+    if (t->n == 4) partial_body(t, op, inf, lo, hi, tipbuf, 4);
+    else if (t->n == 20) partial_body(t, op, inf, lo, hi, tipbuf, 20);
+    else partial_body(t, op, inf, lo, hi, tipbuf, t->n);
+}
+
+static void prepare_ops(orc_tree* t, const orc_op* ops, int nops, op_info* infos) {
+    ensure_info(t, nops);
+#pragma omp parallel for schedule(static) num_threads(t->nthreads) if (nops >= 3)
+    for (int i = 0; i < nops; i++) {
+        double* ech = op_echild(t, i);
+        double* leaf = op_leaf(t, i);
+        partial_info(t, &ops[i], ech, leaf);
+        for (int k = 0; k < 2; k++) {
+            infos[i].echild[k] = ech + (size_t)k * t->block * t->n;
+            infos[i].leaf[k] = leaf + (size_t)k * (t->SU + 1) * t->block;
+        }
+    }
+}
+
+static void run_ops_packet(orc_tree* t, const orc_op* ops, int nops, const op_info* infos,
+                           size_t lo, size_t hi, int packet) {
+    double* tipbuf = t->tipbuf + (size_t)packet * 3 * t->block * VS;
+    for (int i = 0; i < nops; i++) partial_dispatch(t, &ops[i], &infos[i], lo, hi, tipbuf);
+}
+
+void orc_update_partials(orc_tree* t, const orc_op* ops, int nops) {
+    if (nops <= 0) return;
+    op_info* infos = (op_info*)malloc(sizeof(op_info) * nops);
+    prepare_ops(t, ops, nops, infos);
+    size_t limits[1024];
+    compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads)
+    for (int pk = 0; pk < t->npackets; pk++) run_ops_packet(t, ops, nops, infos, limits[pk], limits[pk + 1], pk);
+    free(infos);
+}
+
+/* computeLikelihoodBranchSIMD (tree/phylokernelnew.h:2641-3223), save_log_value = true */
+INLINE double branch_body(orc_tree* t, const orc_op* ops, int nops, const op_info* infos,
+                          orc_end node, orc_end dad, const double* val, const double* tiplut,
+                          const int n) {
+    const int C = t->C, safe = t->safe;
+    const size_t block = (size_t)n * C;
+    size_t limits[1024];
+    compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
+    double all = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads) reduction(+ : all)
+    for (int pk = 0; pk < t->npackets; pk++) {
+        const size_t lo = limits[pk], hi = limits[pk + 1];
+        run_ops_packet(t, ops, nops, infos, lo, hi, pk);
+        double* vtip = t->tipbuf + (size_t)pk * 3 * t->block * VS + 2 * t->block * VS;
+        const double* pn = slot_ptr(t, node.id);
+        const uint16_t* sn = scale_ptr(t, node.id);
+        v4d tree_lh = vset1(0.0);
+        for (size_t ptn = lo; ptn < hi; ptn += VS) {
+            const v4d* vnode = (const v4d*)(pn + ptn * block);
+            v4d lh_cat[16];
+            v4d lh = vset1(0.0);
+            if (dad.is_tip) {
+                load_tip(t, tiplut, dad.id, ptn, vtip);
+                const v4d* vt = (const v4d*)vtip;
+                for (int c = 0; c < C; c++) {
+                    lh_cat[c] = dot_vec_v(vt + c * n, vnode + c * n, n);
+                    if (!safe) lh += lh_cat[c];
+                }
+            } else {
+                const v4d* vdad = (const v4d*)(slot_ptr(t, dad.id) + ptn * block);
+                for (int c = 0; c < C; c++) {
+                    lh_cat[c] = dot3(val + c * n, vdad + c * n, vnode + c * n, n);
+                    if (!safe) lh += lh_cat[c];
+                }
+            }
+            v4d minsc;
+            if (safe) {
+                const uint16_t* sd = dad.is_tip ? NULL : scale_ptr(t, dad.id);
+                for (int x = 0; x < VS; x++) {
+                    uint16_t sum[16], mn = 0;
+                    for (int c = 0; c < C; c++) {
+                        sum[c] = (uint16_t)(sn[(ptn + x) * C + c] + (sd ? sd[(ptn + x) * C + c] : 0));
+                        if (c == 0 || sum[c] < mn) mn = sum[c];
+                    }
+                    minsc[x] = mn;
+                    for (int c = 0; c < C; c++) {
+                        if (sum[c] == mn + 1) lh_cat[c][x] *= SCALING_THRESHOLD;
+                        else if (sum[c] > mn + 1) lh_cat[c][x] = 0.0;
+                    }
+                }
+                /* sumVec<VectorClass, true> grouping (43-71) */
+                const int cruft = C & 3;
+                if (cruft < C) {
+                    v4d acc[4] = {vset1(0), vset1(0), vset1(0), vset1(0)};
+                    for (int c = 0; c < C - cruft; c += 4)
+                        for (int j = 0; j < 4; j++) acc[j] += lh_cat[c + j];
+                    acc[0] += acc[2];
+                    acc[1] += acc[3];
+                    acc[0] += acc[1];
+                    lh += acc[0];
+                }
+                for (int c = C - cruft; c < C; c++) lh += lh_cat[c];
+            } else {
+                const uint16_t* sd = dad.is_tip ? NULL : scale_ptr(t, dad.id);
+                for (int x = 0; x < VS; x++) minsc[x] = (double)(sn[ptn + x] + (sd ? sd[ptn + x] : 0));
+            }
+            minsc *= vset1(LOG_SCALING_THRESHOLD);
+            lh = vabs(lh) + *(const v4d*)(t->invar + ptn);
+            for (int x = 0; x < VS; x++) {
+                if (ptn + x < (size_t)t->P) {
+                    const double lp = log(lh[x]) + minsc[x];
+                    t->pattern_lh[ptn + x] = lp;
+                    tree_lh[x] = lp * t->freq[ptn + x] + tree_lh[x];
+                }
+            }
+        }
+        all += hadd(tree_lh);
+    }
+    return all;
+}
+
+double orc_likelihood_branch(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                             double length, double* pattern_lh, int* status) {
+    const int n = t->n, C = t->C;
+    const size_t block = t->block;
+    op_info* infos = (op_info*)malloc(sizeof(op_info) * (nops > 0 ? nops : 1));
+    prepare_ops(t, ops, nops, infos);
+    double* val = (double*)amalloc(block * 8);
+    for (int c = 0; c < C; c++) {
+        const double len = t->rates[c] * length;
+        for (int i = 0; i < n; i++) val[c * n + i] = exp(t->eval[i] * len) * t->props[c];
+    }
+    double* tiplut = NULL;
+    if (dad.is_tip) {
+        tiplut = (double*)amalloc((size_t)(t->SU + 1) * block * 8);
+        for (int s = 0; s <= t->SU; s++)
+            for (int c = 0; c < C; c++)
+                for (int i = 0; i < n; i++)
+                    tiplut[(size_t)s * block + c * n + i] = val[c * n + i] * t->tip[s * n + i];
+    }
+    double lnl;
+    if (n == 4) lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, 4);
+    else if (n == 20) lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, 20);
+    else lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, n);
+    int st = 0;
+    if (!isfinite(lnl)) {
+        /* per-pattern fix-up (tree/phylokernelnew.h:3115-3132) */
+        st = 4;
+        lnl = 0.0;
+        for (int p = 0; p < t->P; p++) {
+            if (!isfinite(t->pattern_lh[p])) t->pattern_lh[p] = LOG_SCALING_THRESHOLD * 4;
+            lnl += t->pattern_lh[p] * t->freq[p];
+        }
+    }
+    if (status) *status = st;
+    if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)t->P * 8);
+    free(val);
+    free(tiplut);
+    free(infos);
+    return lnl;
+}
+
+/* computeLikelihoodBufferSIMD (tree/phylokernelnew.h:2048-2231) */
+void orc_likelihood_buffer(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad) {
+    const int n = t->n, C = t->C, safe = t->safe;
+    const size_t block = t->block;
+    op_info* infos = (op_info*)malloc(sizeof(op_info) * (nops > 0 ? nops : 1));
+    prepare_ops(t, ops, nops, infos);
+    /* tip table: tip eigen vector repeated per category */
+    double* tipcat = NULL;
+    if (dad.is_tip) {
+        tipcat = (double*)amalloc((size_t)(t->SU + 1) * block * 8);
+        for (int s = 0; s <= t->SU; s++)
+            for (int c = 0; c < C; c++)
+                for (int i = 0; i < n; i++) tipcat[(size_t)s * block + c * n + i] = t->tip[s * n + i];
+    }
+    size_t limits[1024];
+    compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads)
+    for (int pk = 0; pk < t->npackets; pk++) {
+        const size_t lo = limits[pk], hi = limits[pk + 1];
+        run_ops_packet(t, ops, nops, infos, lo, hi, pk);
+        double* vtip = t->tipbuf + (size_t)pk * 3 * t->block * VS + 2 * t->block * VS;
+        const double* pn = slot_ptr(t, node.id);
+        const uint16_t* sn = scale_ptr(t, node.id);
+        const uint16_t* sd = dad.is_tip ? NULL : scale_ptr(t, dad.id);
+        for (size_t ptn = lo; ptn < hi; ptn += VS) {
+            v4d* th = (v4d*)(t->theta + ptn * block);
+            const v4d* vn = (const v4d*)(pn + ptn * block);
+            if (dad.is_tip) {
+                load_tip(t, tipcat, dad.id, ptn, vtip);
+                const v4d* vt = (const v4d*)vtip;
+                for (size_t i = 0; i < block; i++) th[i] = vt[i] * vn[i];
+            } else {
+                const v4d* vd = (const v4d*)(slot_ptr(t, dad.id) + ptn * block);
+                for (size_t i = 0; i < block; i++) th[i] = vn[i] * vd[i];
+            }
+            for (int x = 0; x < VS; x++) {
+                if (safe) {
+                    uint16_t sum[16], mn = 0;
+                    for (int c = 0; c < C; c++) {
+                        sum[c] = (uint16_t)(sn[(ptn + x) * C + c] + (sd ? sd[(ptn + x) * C + c] : 0));
+                        if (c == 0 || sum[c] < mn) mn = sum[c];
+                    }
+                    t->buffer_scale[ptn + x] = mn;
+                    double* thx = t->theta + ptn * block + x;
+                    for (int c = 0; c < C; c++) {
+                        if (sum[c] == mn + 1) {
+                            for (int i = 0; i < n; i++) thx[(c * n + i) * VS] *= SCALING_THRESHOLD;
+                        } else if (sum[c] > mn + 1) {
+                            for (int i = 0; i < n; i++) thx[(c * n + i) * VS] = 0.0;
+                        }
+                    }
+                } else {
+                    t->buffer_scale[ptn + x] = (double)(sn[ptn + x] + (sd ? sd[ptn + x] : 0));
+                }
+                t->buffer_scale[ptn + x] *= LOG_SCALING_THRESHOLD;
+            }
+        }
+    }
+    t->theta_ready = 1;
+    free(tipcat);
+    free(infos);
+}
+
+/* computeLikelihoodDervSIMD, normal branch-length model (tree/phylokernelnew.h:2233-2633) */
+int orc_likelihood_derv(orc_tree* t, double length, double* df, double* ddf) {
+    const int n = t->n, C = t->C;
+    const size_t block = t->block;
+    double* val0 = (double*)amalloc(block * 8);
+    double* val1 = (double*)amalloc(block * 8);
+    double* val2 = (double*)amalloc(block * 8);
+    for (int c = 0; c < C; c++) {
+        const double rate = t->rates[c], prop = t->props[c];
+        for (int i = 0; i < n; i++) {
+            const double cof = t->eval[i] * rate;
+            const double v = exp(cof * length) * prop;
+            const double v1 = cof * v;
+            val0[c * n + i] = v;
+            val1[c * n + i] = v1;
+            val2[c * n + i] = cof * v1;
+        }
+    }
+    size_t limits[1024];
+    compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
+    double all_df = 0.0, all_ddf = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads) reduction(+ : all_df, all_ddf)
+    for (int pk = 0; pk < t->npackets; pk++) {
+        v4d my_df = vset1(0.0), my_ddf = vset1(0.0);
+        for (size_t ptn = limits[pk]; ptn < limits[pk + 1]; ptn += VS) {
+            const v4d* th = (const v4d*)(t->theta + ptn * block);
+            v4d lh, d1, d2;
+            dot_triple(val0, val1, val2, th, &lh, &d1, &d2, (int)block, n);
+            lh = vabs(lh) + *(const v4d*)(t->invar + ptn);
+            /* padding patterns carry freq 0 and contribute nothing */
+            const v4d inv = vset1(1.0) / lh;
+            const v4d df_frac = d1 * inv;
+            const v4d ddf_frac = d2 * inv;
+            const v4d freq = *(const v4d*)(t->freq + ptn);
+            const v4d tmp1 = df_frac * freq;
+            const v4d tmp2 = ddf_frac * freq;
+            my_df += tmp1;
+            my_ddf += tmp2 - tmp1 * df_frac;
+        }
+        all_df += hadd(my_df);
+        all_ddf += hadd(my_ddf);
+    }
+    free(val0); free(val1); free(val2);
+    *df = all_df;
+    *ddf = all_ddf;
+    if (!isfinite(*df)) {
+        *df = *ddf = 0.0;
+        return 4;
+    }
+    return 0;
+}
+
+/* computeLikelihoodFromBufferSIMD (tree/phylokernelnew.h:3232-3428) */
+double orc_likelihood_from_buffer(orc_tree* t, double length, double* pattern_lh) {
+    const int n = t->n, C = t->C;
+    const size_t block = t->block;
+    double* val0 = (double*)amalloc(block * 8);
+    for (int c = 0; c < C; c++) {
+        const double len = t->rates[c] * length;
+        for (int i = 0; i < n; i++) val0[c * n + i] = exp(t->eval[i] * len) * t->props[c];
+    }
+    double all = 0.0;
+#pragma omp parallel for num_threads(t->nthreads) reduction(+ : all)
+    for (size_t ptn = 0; ptn < (size_t)t->Ppad; ptn += VS) {
+        const v4d* th = (const v4d*)(t->theta + ptn * block);
+        v4d lh = dot_vec_s(val0, th, (int)block);
+        lh = vabs(lh) + *(const v4d*)(t->invar + ptn);
+        v4d acc = vset1(0.0);
+        for (int x = 0; x < VS; x++) {
+            if (ptn + x < (size_t)t->P) {
+                const double lp = log(fabs(lh[x])) + t->buffer_scale[ptn + x];
+                t->pattern_lh[ptn + x] = lp;
+                acc[x] = lp * t->freq[ptn + x];
+            }
+        }
+        all += hadd(acc);
+    }
+    free(val0);
+    if (!isfinite(all)) {
+        all = 0.0;
+        for (int p = 0; p < t->P; p++) {
+            if (!isfinite(t->pattern_lh[p])) t->pattern_lh[p] = LOG_SCALING_THRESHOLD * 4;
+            all += t->pattern_lh[p] * t->freq[p];
+        }
+    }
+    if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)t->P * 8);
+    return all;
+}
+
+/* Optimization::minimizeNewton (utils/optimization.cpp:422-499) */
+double orc_minimize_newton(double x1, double xguess, double x2, double xacc, double* d2l,
+                           int max_nr, orc_fderv fn, void* user, int* n_eval, int* status) {
+    double df, dx, dxold, f, temp, xh, xl, rts, rts_old;
+    int evals = 0;
+    *status = 0;
+    rts = xguess;
+    if (rts < x1) rts = x1;
+    if (rts > x2) rts = x2;
+    fn(user, rts, &f, &df);
+    evals++;
+    *d2l = df;
+    if (!isfinite(f) || !isfinite(df)) { *status = 4; if (n_eval) *n_eval = evals; return rts; }
+    if (df >= 0.0 && fabs(f) < xacc) { if (n_eval) *n_eval = evals; return rts; }
+    if (f < 0.0) { xl = rts; xh = x2; } else { xh = rts; xl = x1; }
+    dx = dxold = fabs(xh - xl);
+    for (int j = 1; j <= max_nr; j++) {
+        rts_old = rts;
+        if ((df <= 0.0) || (((rts - xh) * df - f) * ((rts - xl) * df - f) >= 0.0)) {
+            dxold = dx;
+            dx = 0.5 * (xh - xl);
+            rts = xl + dx;
+            *d2l = df;
+            if (xl == rts) { if (n_eval) *n_eval = evals; return rts; }
+        } else {
+            dxold = dx;
+            dx = f / df;
+            temp = rts;
+            rts -= dx;
+            *d2l = df;
+            if (temp == rts) { if (n_eval) *n_eval = evals; return rts; }
+        }
+        if (fabs(dx) < xacc || (j == max_nr)) { if (n_eval) *n_eval = evals; return rts_old; }
+        fn(user, rts, &f, &df);
+        evals++;
+        if (!isfinite(f) || !isfinite(df)) { *status = 4; if (n_eval) *n_eval = evals; return rts; }
+        if (df > 0.0 && fabs(f) < xacc) {
+            *d2l = df;
+            if (n_eval) *n_eval = evals;
+            return rts;
+        }
+        if (f < 0.0) xl = rts;
+        else if (f > 0.0) xh = rts;
+    }
+    *status = 7;
+    *d2l = 0.0;
+    if (n_eval) *n_eval = evals;
+    return 0.0;
+}
+
+/* PhyloTree::computeFuncDerv (tree/phylotree.cpp:2529-2536): f = -dlnL/dt */
+static void tree_fderv(void* user, double x, double* f, double* df) {
+    orc_tree* t = (orc_tree*)user;
+    double d1, d2;
+    orc_likelihood_derv(t, x, &d1, &d2);
+    *f = -d1;
+    *df = -d2;
+}
+
+/* PhyloTree::optimizeOneBranch, Newton branch (tree/phylotree.cpp:2594-2644) */
+double orc_optimize_branch(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                           double length, double min_len, double max_len, int max_nr,
+                           double* lnl, int* n_derv, int* status) {
+    orc_likelihood_buffer(t, ops, nops, node, dad);
+    double d2l;
+    int st = 0, evals = 0;
+    double optx = orc_minimize_newton(min_len, length, max_len, min_len, &d2l, max_nr, tree_fderv, t,
+                                      &evals, &st);
+    if (optx > max_len * 0.95) {
+        const double opt_lh = orc_likelihood_from_buffer(t, optx, NULL);
+        const double orig_lh = orc_likelihood_from_buffer(t, length, NULL);
+        if (orig_lh > opt_lh) optx = length;
+    }
+    if (lnl) *lnl = orc_likelihood_from_buffer(t, optx, NULL);
+    if (n_derv) *n_derv = evals;
+    if (status) *status = st;
+    return optx;
+}
+
+void orc_read_partial(orc_tree* t, int slot, double* partial, int32_t* scale) {
+    const size_t block = t->block;
+    const double* src = slot_ptr(t, slot);
+    const uint16_t* sc = scale_ptr(t, slot);
+    for (int p = 0; p < t->P; p++) {
+        const size_t base = (size_t)(p / VS) * block * VS + (p % VS);
+        if (partial)
+            for (size_t i = 0; i < block; i++) partial[(size_t)p * block + i] = src[base + i * VS];
+        if (scale) {
+            if (t->safe)
+                for (int c = 0; c < t->C; c++) scale[(size_t)p * t->C + c] = sc[(size_t)p * t->C + c];
+            else
+                scale[p] = sc[p];
+        }
+    }
+}

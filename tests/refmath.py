@@ -1,0 +1,148 @@
+"""Independent likelihood mathematics used to pin the oracle.
+
+Nothing here shares code or basis with the oracle: transition matrices come
+from scipy.linalg.expm(Q r t) in the real basis (no eigen-decomposition),
+partials are pruned with per-pattern log scaling in numpy, and tiny trees are
+summed over every assignment of internal states.
+"""
+import itertools
+
+import numpy as np
+from scipy.linalg import expm
+
+
+def adjacency(u, v, t):
+    adj = {}
+    for a, b, ln in zip(u.tolist(), v.tolist(), t.tolist()):
+        adj.setdefault(a, []).append((b, ln))
+        adj.setdefault(b, []).append((a, ln))
+    return adj
+
+
+class Pruner:
+    """Real-basis Felsenstein pruning with expm transition matrices."""
+
+    def __init__(self, ntaxa, edges, states, Q, pi, rates, props, invar, freq, indicators):
+        self.N = ntaxa
+        self.adj = adjacency(*edges)
+        self.states = states
+        self.Q = Q
+        self.pi = pi
+        self.rates = np.asarray(rates)
+        self.props = np.asarray(props)
+        self.invar = invar
+        self.freq = freq
+        self.ind = indicators
+
+    def P(self, t):
+        return np.stack([expm(self.Q * r * t) for r in self.rates])   # (C, n, n)
+
+    def side(self, node, dad):
+        """Partial of `node` seen from `dad`: (C, P, n) array and log scale (P,)."""
+        C = len(self.rates)
+        if node < self.N:
+            v = self.ind[self.states[node]]                     # (P, n)
+            return np.broadcast_to(v, (C,) + v.shape).copy(), np.zeros(v.shape[0])
+        acc, ls = None, None
+        for ch, t in self.adj[node]:
+            if ch == dad:
+                continue
+            L, s = self.side(ch, node)
+            Pt = self.P(t)
+            M = np.einsum("cxy,cpy->cpx", Pt, L)
+            acc = M if acc is None else acc * M
+            ls = s if ls is None else ls + s
+        m = acc.max(axis=(0, 2))
+        m = np.where(m > 0, m, 1.0)
+        return acc / m[None, :, None], ls + np.log(m)
+
+    def branch_terms(self, a, b, t):
+        """Per-pattern (log lh, d lh/dt / lh, d2 lh/dt2 / lh) on branch (a, b) at length t."""
+        A, sa = self.side(a, b)
+        B, sb = self.side(b, a)
+        Pt = self.P(t)
+        rQ = self.rates[:, None, None] * self.Q[None]
+        P1 = np.einsum("cxz,czy->cxy", rQ, Pt)
+        P2 = np.einsum("cxz,czy->cxy", rQ, P1)
+
+        def site(Pm):
+            return np.einsum("c,x,cpx,cxy,cpy->p", self.props, self.pi, A, Pm, B)
+
+        s0, s1, s2 = site(Pt), site(P1), site(P2)
+        ls = sa + sb
+        # lh = exp(ls) s0 + invar, kept in log space
+        with np.errstate(divide="ignore"):
+            log_inv = np.log(self.invar)
+        log_lh = np.logaddexp(ls + np.log(s0), log_inv)
+        scale = np.exp(ls - log_lh)          # exp(ls) / lh
+        return log_lh, s1 * scale, s2 * scale
+
+    def lnl(self, a, b, t):
+        log_lh, _, _ = self.branch_terms(a, b, t)
+        return float(np.dot(self.freq, log_lh)), log_lh
+
+    def derivs(self, a, b, t):
+        log_lh, d1, d2 = self.branch_terms(a, b, t)
+        df = float(np.dot(self.freq, d1))
+        ddf = float(np.dot(self.freq, d2 - d1 * d1))
+        return float(np.dot(self.freq, log_lh)), df, ddf
+
+
+def brute_force_lnl(ntaxa, edges, states, Q, pi, rates, props, invar, freq, indicators):
+    """Sum over every assignment of internal-node states (tiny trees only)."""
+    adj = adjacency(*edges)
+    nodes = sorted(adj)
+    internal = [x for x in nodes if x >= ntaxa]
+    n = Q.shape[0]
+    root = internal[0]
+    # parent pointers from the root
+    parent, order, stack = {root: None}, [], [root]
+    while stack:
+        x = stack.pop()
+        order.append(x)
+        for y, t in adj[x]:
+            if y not in parent:
+                parent[y] = (x, t)
+                stack.append(y)
+    P = states.shape[1]
+    total = np.zeros(P)
+    for c, (r, pc) in enumerate(zip(rates, props)):
+        lh_c = np.zeros(P)
+        mats = {x: expm(Q * r * parent[x][1]) for x in order if parent[x] is not None}
+        for assign in itertools.product(range(n), repeat=len(internal)):
+            st = dict(zip(internal, assign))
+            w = pi[st[root]]
+            for x in internal:
+                if parent[x] is not None:
+                    w = w * mats[x][st[parent[x][0]], st[x]]
+            if w == 0.0:
+                continue
+            site = np.full(P, w)
+            for leaf in range(ntaxa):
+                px = st[parent[leaf][0]]
+                site = site * (indicators[states[leaf]] @ mats[leaf][px])
+            lh_c += site
+        total += pc * lh_c
+    lh = total + invar
+    return float(np.dot(freq, np.log(lh))), np.log(lh)
+
+
+def indicators(seq_type, n, state_unknown):
+    """Observed-state indicator vectors [SU+1][n] from the alphabet definitions:
+    DNA ambiguity = bitmask + 3 over A,C,G,T; protein B/Z/J = N|D, Q|E, I|L;
+    STATE_UNKNOWN = every state."""
+    ind = np.zeros((state_unknown + 1, n))
+    for s in range(state_unknown + 1):
+        if s < n:
+            ind[s, s] = 1.0
+        elif s == state_unknown:
+            ind[s, :] = 1.0
+        elif seq_type == 0:
+            mask = s - 3
+            for x in range(4):
+                if mask & (1 << x):
+                    ind[s, x] = 1.0
+        elif seq_type == 1:
+            pair = {20: (2, 3), 21: (5, 6), 22: (9, 10)}[s]
+            ind[s, list(pair)] = 1.0
+    return ind

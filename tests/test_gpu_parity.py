@@ -1,0 +1,198 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the CPU oracle on
+identical inputs. Tolerances (FP64 form): lnL 1e-9 relative, per-pattern
+lnL 1e-9 absolute, derivatives 1e-7 relative, scale counters bit-exact,
+partials 1e-10 relative to their pattern's largest entry."""
+import numpy as np
+import pytest
+
+import iqtree2_amd as ia
+from oracle.binding import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+LNL_RTOL = 1e-9
+DERV_RTOL = 1e-7
+
+
+def setup(ctx, w, precision=ia.PREC_F64, safe=False, nthreads=4):
+    planner = ia.PhyloTree(None, w)
+    inst = ia.Instance.from_workload(ctx, w, precision=precision, safe_numeric=safe)
+    orc = OracleTree.from_workload(w, safe_numeric=safe, nthreads=nthreads)
+    return planner, inst, orc
+
+
+def compare_partials(inst, orc, ops):
+    for op in ops:
+        slot = int(op["dad_slot"])
+        pg, sg = inst.read_partial(slot)
+        po, so = orc.read_partial(slot)
+        mism = np.flatnonzero((sg != so).reshape(sg.shape[0], -1).any(axis=1))
+        assert mism.size == 0, f"slot {slot}: scale counters differ at patterns {mism[:10]}"
+        denom = np.maximum(np.abs(po).max(axis=1, keepdims=True), 1e-300)
+        err = np.max(np.abs(pg - po) / denom)
+        assert err <= 1e-10, f"slot {slot}: partial error {err}"
+
+
+def all_branch_parity(w, planner, inst, orc, check_derv=True, stride=1):
+    for a, b in planner.branches()[::stride]:
+        ops, node, dad, length = planner.plan(a, b)
+        lg, pg = inst.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+        lo, po = orc.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+        assert abs(lg - lo) <= LNL_RTOL * abs(lo), (a, b, lg, lo)
+        np.testing.assert_allclose(pg, po, rtol=0, atol=1e-9)
+        compare_partials(inst, orc, ops)
+        if check_derv:
+            inst.likelihood_buffer(None, node, dad)
+            orc.likelihood_buffer(None, node, dad)
+            for t in (length, 2.0 * length + 1e-3):
+                dg, ddg, lng = inst.likelihood_derv(t)
+                do, ddo = orc.likelihood_derv(t)
+                assert abs(dg - do) <= DERV_RTOL * max(1.0, abs(do)), (a, b, t, dg, do)
+                assert abs(ddg - ddo) <= DERV_RTOL * max(1.0, abs(ddo)), (a, b, t, ddg, ddo)
+                lfb = orc.likelihood_from_buffer(t)
+                assert abs(lng - lfb) <= LNL_RTOL * abs(lfb)
+                assert abs(inst.likelihood_from_buffer(t) - lfb) <= LNL_RTOL * abs(lfb)
+
+
+def test_config1_dna_gtr_g4_all_branches(ctx):
+    """16 taxa x 1,000 sites, GTR+G4, lnL and df/ddf on every branch."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=16, nsite=1000, ncat=4, mean_len=0.1, alpha=0.5,
+                    seed=20261203)
+    all_branch_parity(w, *setup(ctx, w))
+
+
+def test_dna_gtr_i_g4_gaps(ctx):
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=60, nsite=5000, ncat=4, mean_len=0.1, alpha=0.5,
+                    p_inv=0.2, gap_frac=0.02, seed=20261204)
+    all_branch_parity(w, *setup(ctx, w), stride=7)
+
+
+def test_protein_g4(ctx):
+    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=30, nsite=2000, ncat=4, mean_len=0.2, alpha=0.8,
+                    seed=20261205)
+    all_branch_parity(w, *setup(ctx, w), stride=5)
+
+
+@pytest.mark.parametrize("safe", [False, True])
+def test_scaling_counts_bit_exact(ctx, safe):
+    """Deep tree with long branches: partials cross 2^-256 and get rescaled."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=300, nsite=600, ncat=4, mean_len=0.6, alpha=1.0,
+                    seed=5)
+    planner, inst, orc = setup(ctx, w, safe=safe)
+    a, b = planner.branches()[0]
+    ops, node, dad, length = planner.plan(a, b)
+    lg = inst.likelihood_branch(ops, node, dad, length)
+    lo = orc.likelihood_branch(ops, node, dad, length)
+    assert abs(lg - lo) <= LNL_RTOL * abs(lo)
+    assert max(orc.read_partial(int(op["dad_slot"]))[1].max() for op in ops) >= 1
+    compare_partials(inst, orc, ops)
+    inst.likelihood_buffer(None, node, dad)
+    orc.likelihood_buffer(None, node, dad)
+    dg, ddg, _ = inst.likelihood_derv(length)
+    do, ddo = orc.likelihood_derv(length)
+    assert abs(dg - do) <= DERV_RTOL * max(1.0, abs(do))
+    assert abs(ddg - ddo) <= DERV_RTOL * max(1.0, abs(ddo))
+
+
+@pytest.mark.parametrize("ncat", [1, 2, 8])
+def test_category_counts(ctx, ncat):
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=24, nsite=1500, ncat=ncat, mean_len=0.15, alpha=0.6,
+                    p_inv=0.1, seed=30 + ncat)
+    all_branch_parity(w, *setup(ctx, w), stride=9)
+
+
+def test_three_taxa(ctx):
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=3, nsite=200, ncat=4, seed=77)
+    all_branch_parity(w, *setup(ctx, w))
+
+
+def test_newton_branch_optimisation(ctx):
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=40, nsite=3000, ncat=4, mean_len=0.1, alpha=0.5,
+                    p_inv=0.2, gap_frac=0.02, seed=1234)
+    planner, inst, orc = setup(ctx, w)
+    for a, b in planner.branches()[::4]:
+        ops, node, dad, length = planner.plan(a, b)
+        xg, lg, ng = inst.optimize_branch(ops, node, dad, length * 3.0)
+        xo, lo, no, st = orc.optimize_branch(ops, node, dad, length * 3.0)
+        assert st == 0
+        assert abs(xg - xo) <= 1e-7 * max(xo, 1e-6), (a, b, xg, xo)
+        assert abs(lg - lo) <= LNL_RTOL * abs(lo)
+        assert ng == no
+
+
+def test_optimize_all_branches_driver(ctx):
+    """PhyloTree::optimizeAllBranches through the C++ driver vs the oracle
+    driven by the same plans."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=20, nsite=1500, ncat=4, mean_len=0.1, alpha=0.5,
+                    seed=99)
+    gpu_tree = ia.PhyloTree(ctx, w)
+    lnl0 = gpu_tree.compute_likelihood()
+    lnl, nd = gpu_tree.optimize_all_branches(iterations=1, tolerance=1e-3, max_nr=100)
+    assert lnl >= lnl0 - 1e-6 and nd > 0
+    # the oracle replays optimizeAllBranches with the same planner
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w, nthreads=4)
+    branches = planner.branches()
+    a0, b0 = branches[0]
+    ops, node, dad, length = planner.plan(b0, a0)
+    orc.likelihood_branch(ops, node, dad, length)
+    for a, b in branches:
+        ops, node, dad, length = planner.plan(b, a)
+        x, _, _, st = orc.optimize_branch(ops, node, dad, length)
+        assert st == 0
+        planner.set_length(a, b, x)
+        assert abs(gpu_tree.get_length(a, b) - x) <= 1e-7 * max(x, 1e-6)
+    lo = orc.likelihood_from_buffer(planner.get_length(*branches[-1]))
+    assert abs(lnl - lo) <= LNL_RTOL * abs(lo)
+
+
+def test_rell(ctx):
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=16, nsite=2000, ncat=4, seed=3)
+    planner, inst, orc = setup(ctx, w)
+    ops, node, dad, length = planner.plan(*planner.branches()[0])
+    _, plh = inst.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+    rng = np.random.default_rng(0)
+    freq = w.get("freq")
+    weights = np.stack([rng.multinomial(int(freq.sum()), freq / freq.sum())
+                        for _ in range(50)]).astype(np.float32)
+    out = inst.rell(weights)
+    np.testing.assert_allclose(out, weights.astype(np.float64) @ plh, rtol=1e-12)
+
+
+def test_fp32_storage_form(ctx):
+    """FP32 partials/contractions, FP64 log, sums and derivatives."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=60, nsite=5000, ncat=4, mean_len=0.1, alpha=0.5,
+                    p_inv=0.2, gap_frac=0.02, seed=20261204)
+    planner, inst32, orc = setup(ctx, w, precision=ia.PREC_F32)
+    for a, b in planner.branches()[::11]:
+        ops, node, dad, length = planner.plan(a, b)
+        lg = inst32.likelihood_branch(ops, node, dad, length)
+        lo = orc.likelihood_branch(ops, node, dad, length)
+        assert abs(lg - lo) <= 1e-6 * abs(lo), (a, b, lg, lo, abs(lg - lo))
+        inst32.likelihood_buffer(None, node, dad)
+        orc.likelihood_buffer(None, node, dad)
+        dg, ddg, _ = inst32.likelihood_derv(length)
+        do, ddo = orc.likelihood_derv(length)
+        assert abs(dg - do) <= 1e-4 * max(1.0, abs(do))
+
+
+def test_error_paths(ctx):
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=8, nsite=100, seed=1)
+    planner, inst, orc = setup(ctx, w)
+    with pytest.raises(ia.PlkError) as e:
+        inst.likelihood_derv(0.1)
+    assert e.value.status == 5          # theta not computed
+    ops, node, dad, length = planner.plan(*planner.branches()[0])
+    bad = ops.copy()
+    bad[0]["dad_slot"] = 999
+    with pytest.raises(ia.PlkError) as e:
+        inst.likelihood_branch(bad, node, dad, length)
+    assert e.value.status == 1
+    small = ia.Context(0, hbm_budget=1 << 20)
+    big = ia.Workload(seq_type=ia.DNA, ntaxa=64, nsite=20000, seed=2)
+    with pytest.raises(ia.PlkError) as e:
+        ia.Instance.from_workload(small, big)
+    assert e.value.status == 2
+    small.close()
+    # the engine still works after the failures
+    assert np.isfinite(inst.likelihood_branch(ops, node, dad, length))

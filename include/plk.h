@@ -148,6 +148,46 @@ plk_status plk_optimize_branch(plk_context* ctx, int instance, const plk_op* ops
                                double max_len, int max_nr, double* out_length, double* out_lnl,
                                int* n_derv);
 
+/* ---- device-resident branch optimisation ------------------------------- */
+/* A traversal step whose children carry edge ids instead of lengths: the
+ * lengths are read from the device copy of the branch-length array, which
+ * the sweep updates as it optimises branches. */
+typedef struct {
+    int is_tip;
+    int id;
+    int edge;
+} plk_sweep_child;
+
+typedef struct {
+    int dad_slot;
+    int nchildren;
+    plk_sweep_child child[PLK_MAX_CHILDREN];
+} plk_sweep_op;
+
+/* one branch of the sweep: ops[op_begin .. op_begin+op_count) refresh the
+ * partials at its ends, then theta(node, dad) and Newton on edge `edge` */
+typedef struct {
+    int op_begin;
+    int op_count;
+    plk_end node;
+    plk_end dad;
+    int edge;
+} plk_sweep_branch;
+
+/* One pass of PhyloTree::optimizeAllBranches over `branches` in order
+ * (tree/phylotree.cpp:2698-2736): for each branch optimizeOneBranch's
+ * minimizeNewton(min_len, len, max_len, xacc = min_len, max_nr) and the
+ * divergence guard, entirely on the device. lengths[nedges] is read and
+ * overwritten with the optimised lengths; n_derv (nullable, [nbranches])
+ * receives derivative evaluations per branch; lnl receives
+ * computeLikelihoodFromBuffer after the last branch. The op list must be the
+ * one a host planner produces when every optimised branch invalidates the
+ * partials that contain it (clearReversePartialLh). */
+plk_status plk_optimize_sweep(plk_context* ctx, int instance, const plk_sweep_op* ops, int nops,
+                              const plk_sweep_branch* branches, int nbranches, double* lengths,
+                              int nedges, double min_len, double max_len, int max_nr,
+                              int* n_derv, double* lnl);
+
 /* RELL (tree/iqtree.cpp:3568-3625, tree/phylokernel.h:64-70): out[b] =
  * sum_p weights[b][p] * pattern_lh[p] over the last evaluated pattern_lh. */
 plk_status plk_rell(plk_context* ctx, int instance, const float* weights, int nboot,

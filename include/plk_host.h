@@ -117,6 +117,10 @@ int plkh_tree_optimize_branch(plkh_tree* t, int a, int b, int max_nr, double* ne
  * lnl = final tree lnL, n_derv = derivative evaluations */
 int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double tolerance, int max_nr,
                                     double* lnl, int* n_derv);
+/* 1 (default): optimize_all_branches runs each pass as one device-resident
+ * Newton sweep (plk_optimize_sweep); 0: minimizeNewton on the host, one
+ * engine call per derivative evaluation */
+int plkh_tree_set_device_newton(plkh_tree* t, int on);
 /* min/max branch length used by optimizeOneBranch (defaults 1e-6, 10) */
 int plkh_tree_set_branch_bounds(plkh_tree* t, double min_len, double max_len);
 

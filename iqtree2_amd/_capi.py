@@ -89,6 +89,8 @@ _PROTOS = {
     "plk_optimize_branch": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd, c_double,
                                     c_double, c_double, c_int, _DP, _DP, P(c_int)]),
     "plk_rell": (c_int, [c_void_p, c_int, _F32, c_int, _DP]),
+    "plk_optimize_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
+                                   c_double, c_double, c_int, _I32, _DP]),
     "plk_read_partial": (c_int, [c_void_p, c_int, c_int, _DP, _I32]),
     "plk_profile_start": (c_int, [c_void_p]),
     "plk_profile_stop": (c_int, [c_void_p, P(PlkKernelStat), c_int, P(c_int)]),
@@ -128,6 +130,7 @@ _PROTOS = {
     "plkh_tree_optimize_branch": (c_int, [c_void_p, c_int, c_int, c_int, _DP, P(c_int)]),
     "plkh_tree_optimize_all_branches": (c_int, [c_void_p, c_int, c_double, c_int, _DP, P(c_int)]),
     "plkh_tree_set_branch_bounds": (c_int, [c_void_p, c_double, c_double]),
+    "plkh_tree_set_device_newton": (c_int, [c_void_p, c_int]),
 }
 
 _lib = None

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -13,6 +14,7 @@
 
 #include "plk.h"
 #include "plk_kernels.h"
+#include "plk_sweep.h"
 
 using namespace plkdev;
 
@@ -64,6 +66,20 @@ struct Instance {
     double* root_tab = nullptr;
     plk_op* ops = nullptr;
     int ops_cap = 0;
+    // device-resident Newton sweep
+    plk_sweep_op* sw_ops = nullptr;
+    int sw_ops_cap = 0;
+    plk_sweep_branch* sw_br = nullptr;
+    int sw_br_cap = 0;
+    double* sw_len = nullptr;
+    int sw_len_cap = 0;
+    int* sw_nderv = nullptr;
+    double* sw_part = nullptr;
+    int sw_part_cap = 0;
+    unsigned* sw_flags = nullptr;     // arrival flags, release flag, abort flag
+    int sw_flags_cap = 0;
+    double* sw_out = nullptr;         // [0..1] lnL, status; [2..3] reduced sums
+    long long* sw_timing = nullptr;   // phase ticks of the last sweeps (PLK_SWEEP_TIMING)
     void* tables = nullptr;
     bool patterns_set = false, model_set = false, theta_ready = false, ptn_lh_ready = false;
     double theta_length = 0.0;
@@ -89,10 +105,10 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 }  // namespace
 
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
-                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_NKIND };
+                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
-                                          "reduce", "rell"};
+                                          "reduce", "rell", "newton_sweep"};
 
 struct ProfRec {
     int kind;
@@ -102,6 +118,8 @@ struct ProfRec {
 
 struct plk_context {
     int device = 0;
+    int num_cu = 0;
+    int sweep_blocks_per_cu = 2;
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev;
     size_t prof_next = 0;
@@ -459,6 +477,8 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
     }
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    HIP_TRY(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (const char* e = std::getenv("PLK_SWEEP_BLOCKS_PER_CU")) ctx->sweep_blocks_per_cu = std::max(1, std::atoi(e));
     ctx->budget = (cfg && cfg->hbm_budget) ? cfg->hbm_budget : (size_t)(0.9 * (double)free_b);
     hipError_t e = hipHostMalloc((void**)&ctx->h_out, 64 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -572,7 +592,8 @@ plk_status plk_instance_destroy(plk_context* ctx, int id) {
     (void)hipStreamSynchronize(ctx->stream);
     void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
                     in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
-                    in->root_tab, in->ops, in->tables};
+                    in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
+                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     ctx->used = ctx->used >= in->bytes ? ctx->used - in->bytes : 0;
@@ -818,6 +839,204 @@ plk_status plk_optimize_branch(plk_context* ctx, int id, const plk_op* ops, int 
     if (out_lnl) {
         if ((st = plk_likelihood_from_buffer(ctx, id, optx, out_lnl, nullptr)) != PLK_OK) return st;
     }
+    return PLK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+plk_status grow(plk_context* ctx, Instance* in, T** p, int& cap, int need) {
+    if (need <= cap && *p) return PLK_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dfree(ctx, in, *p, sizeof(T) * (size_t)cap);
+    *p = nullptr;
+    cap = std::max(need, 2 * cap);
+    plk_status st = dmalloc(ctx, in, p, sizeof(T) * (size_t)cap);
+    if (st != PLK_OK) cap = 0;
+    return st;
+}
+
+template <typename T, int N, int C>
+plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& grid_out) {
+    const SweepSmem L(N, C, in->d.state_unknown, in->table_size, a.nedges, (int)sizeof(T));
+    if (L.total > 64 * 1024) return fail(PLK_ERR_UNSUPPORTED, "sweep LDS need %d bytes", L.total);
+    int nb = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, N, C>, kThreads, L.total));
+    if (nb < 1) return fail(PLK_ERR_UNSUPPORTED, "sweep kernel does not fit a CU");
+    const int PB = kThreads / C;
+    const int ntiles = in->Ppad / PB;
+    int grid = ctx->num_cu * std::min(nb, ctx->sweep_blocks_per_cu);
+    grid = std::max(1, std::min(grid, ntiles));
+    plk_status st = grow(ctx, in, &in->sw_part, in->sw_part_cap, 2 * grid);
+    if (st != PLK_OK) return st;
+    const int nflags = (grid + 1) * kFlagStride;
+    if ((st = grow(ctx, in, &in->sw_flags, in->sw_flags_cap, nflags)) != PLK_OK) return st;
+    HIP_TRY(hipMemsetAsync(in->sw_flags, 0, sizeof(unsigned) * nflags, ctx->stream));
+    a.part = in->sw_part;
+    a.arrive = in->sw_flags + kFlagStride;
+    a.release = in->sw_flags;
+    a.result = in->sw_out + 2;
+    void* args[] = {&a};
+    HIP_TRY(hipLaunchCooperativeKernel((const void*)k_sweep<T, N, C>, dim3(grid), dim3(kThreads),
+                                       args, (unsigned)L.total, ctx->stream));
+    grid_out = grid;
+    return PLK_OK;
+}
+
+template <typename T, int N>
+plk_status launch_sweep_n(plk_context* ctx, Instance* in, SweepArgs& a, int& grid) {
+    switch (in->d.ncat) {
+    case 1: return launch_sweep_t<T, N, 1>(ctx, in, a, grid);
+    case 2: return launch_sweep_t<T, N, 2>(ctx, in, a, grid);
+    case 4: return launch_sweep_t<T, N, 4>(ctx, in, a, grid);
+    case 8: return launch_sweep_t<T, N, 8>(ctx, in, a, grid);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+}
+
+plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid) {
+    const bool f32 = in->d.precision == PLK_PREC_F32;
+    switch (in->d.nstates) {
+    case 4: return f32 ? launch_sweep_n<float, 4>(ctx, in, a, grid) : launch_sweep_n<double, 4>(ctx, in, a, grid);
+    case 20: return f32 ? launch_sweep_n<float, 20>(ctx, in, a, grid) : launch_sweep_n<double, 20>(ctx, in, a, grid);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
+}
+
+}  // namespace
+
+extern "C" {
+
+plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops, int nops,
+                              const plk_sweep_branch* branches, int nbranches, double* lengths,
+                              int nedges, double min_len, double max_len, int max_nr,
+                              int* n_derv, double* lnl) {
+    Instance* in = get_instance(ctx, id);
+    plk_status st = ready(in);
+    if (st != PLK_OK) return st;
+    if (nops < 0 || (nops > 0 && !ops) || nbranches < 1 || !branches || !lengths || nedges < 1 ||
+        max_nr < 1 || !(min_len > 0.0) || !(max_len > min_len))
+        return fail(PLK_ERR_ARG, "invalid sweep arguments");
+    for (int e = 0; e < nedges; e++)
+        if (!(lengths[e] >= 0.0) || !std::isfinite(lengths[e])) return fail(PLK_ERR_ARG, "invalid length %d", e);
+    for (int o = 0; o < nops; o++) {
+        const plk_sweep_op& op = ops[o];
+        if (op.dad_slot < 0 || op.dad_slot >= in->d.nslots || op.nchildren != 2)
+            return fail(PLK_ERR_ARG, "sweep op %d: bad dad slot / arity", o);
+        for (int k = 0; k < 2; k++) {
+            const plk_sweep_child& ch = op.child[k];
+            const int lim = ch.is_tip ? in->d.ntaxa : in->d.nslots;
+            if (ch.id < 0 || ch.id >= lim || ch.edge < 0 || ch.edge >= nedges ||
+                (!ch.is_tip && ch.id == op.dad_slot))
+                return fail(PLK_ERR_ARG, "sweep op %d child %d out of range", o, k);
+        }
+    }
+    for (int b = 0; b < nbranches; b++) {
+        const plk_sweep_branch& B = branches[b];
+        if (B.op_begin < 0 || B.op_count < 0 || B.op_begin + B.op_count > nops || B.edge < 0 ||
+            B.edge >= nedges)
+            return fail(PLK_ERR_ARG, "sweep branch %d: bad op range / edge", b);
+        if ((st = check_end(in, B.node, false)) != PLK_OK) return st;
+        if ((st = check_end(in, B.dad, true)) != PLK_OK) return st;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if ((st = grow(ctx, in, &in->sw_ops, in->sw_ops_cap, std::max(nops, 1))) != PLK_OK) return st;
+    if ((st = grow(ctx, in, &in->sw_br, in->sw_br_cap, nbranches)) != PLK_OK) return st;
+    if ((st = grow(ctx, in, &in->sw_len, in->sw_len_cap, nedges)) != PLK_OK) return st;
+    int nd_cap = in->sw_br_cap;
+    if (!in->sw_nderv) {
+        if ((st = dmalloc(ctx, in, &in->sw_nderv, sizeof(int) * (size_t)std::max(nd_cap, 4096))) != PLK_OK) return st;
+        if ((st = dmalloc(ctx, in, &in->sw_out, sizeof(double) * 8)) != PLK_OK) return st;
+    }
+    if (nbranches > 4096) return fail(PLK_ERR_UNSUPPORTED, "more than 4096 branches in one sweep");
+    if (nops > 0)
+        HIP_TRY(hipMemcpyAsync(in->sw_ops, ops, sizeof(plk_sweep_op) * nops, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->sw_br, branches, sizeof(plk_sweep_branch) * nbranches, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->sw_len, lengths, sizeof(double) * nedges, hipMemcpyHostToDevice, ctx->stream));
+    SweepArgs a;
+    a.I = inst_dev(in);
+    a.m = model_dev(in);
+    a.ops = in->sw_ops;
+    a.br = in->sw_br;
+    a.nbranches = nbranches;
+    a.nedges = nedges;
+    a.len = in->sw_len;
+    a.min_len = min_len;
+    a.max_len = max_len;
+    a.max_nr = max_nr;
+    a.table_size = in->table_size;
+    a.part = nullptr;
+    a.arrive = nullptr;
+    a.release = nullptr;
+    a.result = nullptr;
+    a.nderv = in->sw_nderv;
+    a.out = in->sw_out;
+    a.timing = nullptr;
+    static const bool want_timing = std::getenv("PLK_SWEEP_TIMING") != nullptr;
+    if (want_timing) {
+        if (!in->sw_timing) {
+            if ((st = dmalloc(ctx, in, &in->sw_timing, sizeof(long long) * PH_COUNT)) != PLK_OK) return st;
+            HIP_TRY(hipMemsetAsync(in->sw_timing, 0, sizeof(long long) * PH_COUNT, ctx->stream));
+        }
+        a.timing = in->sw_timing;
+    }
+    int grid = 0;
+    const int pe = prof_begin(ctx);
+    st = launch_sweep(ctx, in, a, grid);
+    if (st != PLK_OK) return st;
+    prof_end(ctx, pe, K_SWEEP, 0.0, 0.0);
+    std::vector<int> nd(nbranches);
+    double out[2];
+    HIP_TRY(hipMemcpyAsync(lengths, in->sw_len, sizeof(double) * nedges, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(nd.data(), in->sw_nderv, sizeof(int) * nbranches, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, in->sw_out, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    in->theta_ready = true;
+    in->ptn_lh_ready = true;
+    if (a.timing) {
+        long long t[PH_COUNT];
+        int khz = 0;
+        HIP_TRY(hipMemcpy(t, a.timing, sizeof(t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+        const double us = 1e3 / (double)khz;
+        std::fprintf(stderr, "[plk sweep timing] grid %d: ops %.1f us, theta %.1f us, eval work %.1f us, "
+                     "exchange %.1f us, total %.1f us (cumulative)\n", grid, t[PH_OPS] * us,
+                     t[PH_THETA] * us, t[PH_EVAL_WORK] * us, t[PH_EXCHANGE] * us, t[PH_TOTAL] * us);
+    }
+    if (n_derv) std::memcpy(n_derv, nd.data(), sizeof(int) * nbranches);
+    if (lnl) *lnl = out[0];
+    if (pe >= 0) {
+        // algorithmic bytes: the ops, theta per branch, and every evaluation
+        // pass over theta (+invar, freq; lnL passes also buffer_scale)
+        const double P = in->d.npatterns, s = (double)in->elem, blk = (double)in->block;
+        double bytes = 0.0, flops = 0.0;
+        for (int o = 0; o < nops; o++) {
+            plk_op tmp{};
+            tmp.nchildren = 2;
+            for (int k = 0; k < 2; k++) tmp.child[k].is_tip = ops[o].child[k].is_tip;
+            double b, f;
+            op_cost(in, tmp, b, f);
+            bytes += b;
+            flops += f;
+        }
+        long evals = 0;
+        for (int b = 0; b < nbranches; b++) {
+            evals += nd[b];
+            const int nint = 1 + (branches[b].dad.is_tip ? 0 : 1);
+            bytes += blk * s * nint + (branches[b].dad.is_tip ? 1 : 0) + in->scale_elem * nint +
+                     blk * s + 8.0;
+        }
+        bytes += evals * (blk * s + 16.0) + (blk * s + 32.0);
+        flops += evals * 6.0 * blk;
+        ctx->prof_recs.back().bytes = bytes * P;
+        ctx->prof_recs.back().flops = flops * P;
+    }
+    const int status = (int)out[1];
+    if (status == PLK_ERR_NONFINITE) return fail(PLK_ERR_NONFINITE, "non-finite derivative in the Newton sweep");
+    if (status == PLK_ERR_DEVICE) return fail(PLK_ERR_DEVICE, "grid reduction in the Newton sweep timed out");
+    if (status != 0) return fail((plk_status)status, "Newton sweep failed (status %d)", status);
     return PLK_OK;
 }
 

@@ -218,16 +218,151 @@ __device__ inline void block_sum(double (&v)[NV], double* sh) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused post-order traversal (computePartialLikelihoodSIMD, binary nodes,
-// tree/phylokernelnew.h:1632-2031) followed, per mode, by the branch lnL
-// (computeLikelihoodBranchSIMD 2753-3104) or the theta buffer
-// (computeLikelihoodBufferSIMD 2091-2230) of the branch (node, dad).
+// One binary traversal step for the pattern/category (p, c) of this thread
+// (computePartialLikelihoodSIMD, tree/phylokernelnew.h:1632-2031):
+// tmp[x] = prod_children (tip: leaf[s][c][x]; internal: sum_i E[c][x][i] child[i]),
+// dad = U^-1 tmp, then the reference's rescaling rule. tL / tR are the
+// children's tables (tip lookup or transition rows), a tip child goes left.
+template <typename T, int N, int C>
+__device__ inline void op_apply(const InstDev& I, const T* inv_sh, const T* tL, const T* tR,
+                                int l_tip, int l_id, int r_tip, int r_id, int dad_slot, int p,
+                                int c) {
+    using ST = typename Prec<T>::scale_t;
+    constexpr int block = N * C;
+    T* partial = reinterpret_cast<T*>(I.partial);
+    ST* scale = reinterpret_cast<ST*>(I.scale);
+    const bool safe = I.safe != 0;
+    T tmp[N];
+    int sc_l = 0, sc_r = 0;
+    const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
+    if (l_tip) {
+        const int s = I.states[(size_t)l_id * I.Ppad + p];
+        const T* lut = tL + (s * C + c) * N;
+#pragma unroll
+        for (int x = 0; x < N; x++) tmp[x] = lut[x];
+    } else {
+        const T* src = partial + (size_t)l_id * I.slot_stride + (size_t)p * block + c * N;
+        T cv[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) cv[i] = src[i];
+        const T* E = tL + c * N * N;
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            T acc = E[x * N] * cv[0];
+#pragma unroll
+            for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
+            tmp[x] = acc;
+        }
+        sc_l = scale[(size_t)l_id * I.scale_stride + sidx];
+    }
+    if (r_tip) {
+        const int s = I.states[(size_t)r_id * I.Ppad + p];
+        const T* lut = tR + (s * C + c) * N;
+#pragma unroll
+        for (int x = 0; x < N; x++) tmp[x] *= lut[x];
+    } else {
+        const T* src = partial + (size_t)r_id * I.slot_stride + (size_t)p * block + c * N;
+        T cv[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) cv[i] = src[i];
+        const T* E = tR + c * N * N;
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            T acc = E[x * N] * cv[0];
+#pragma unroll
+            for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
+            tmp[x] *= acc;
+        }
+        sc_r = scale[(size_t)r_id * I.scale_stride + sidx];
+    }
+    // back to the eigen basis: dad[i] = sum_x inv_evec[i][x] tmp[x]
+    T dadv[N];
+    T lmax = (T)0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        T acc = inv_sh[i * N] * tmp[0];
+#pragma unroll
+        for (int x = 1; x < N; x++) acc = fma(inv_sh[i * N + x], tmp[x], acc);
+        dadv[i] = acc;
+        lmax = fmax(lmax, fabs(acc));
+    }
+    int sc = sc_l + sc_r;
+    if (!(l_tip && r_tip)) {   // tip-tip steps are never rescaled
+        const bool no_invar = I.invar[p] == 0.0;
+        const T m = safe ? lmax : (T)group_max<C>((double)lmax);
+        if (Prec<T>::underflow(m) && no_invar) {
+            const int k = Prec<T>::exponent(m);
+#pragma unroll
+            for (int i = 0; i < N; i++) dadv[i] = Prec<T>::apply(dadv[i], k);
+            sc += k;
+        }
+    }
+    T* dst = partial + (size_t)dad_slot * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+    for (int i = 0; i < N; i++) dst[i] = dadv[i];
+    if (safe || c == 0) scale[(size_t)dad_slot * I.scale_stride + sidx] = (ST)sc;
+}
+
+// Branch ends for (p, c): node partial nv, dad partial (or tip vector from
+// tip_sh) dv, the summed scale counter and, in safe mode, the per-category
+// adjustment relative to the least-scaled category.
+template <typename T, int N, int C>
+__device__ inline void load_branch(const InstDev& I, plk_end node, plk_end dad, const double* tip_sh,
+                                   int p, int c, double* nv, double* dv, int& minsc,
+                                   double& adjust) {
+    using ST = typename Prec<T>::scale_t;
+    constexpr int block = N * C;
+    const T* partial = reinterpret_cast<const T*>(I.partial);
+    const ST* scale = reinterpret_cast<const ST*>(I.scale);
+    const bool safe = I.safe != 0;
+    const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
+    const T* nsrc = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+    for (int i = 0; i < N; i++) nv[i] = (double)nsrc[i];
+    int sc = scale[(size_t)node.id * I.scale_stride + sidx];
+    if (dad.is_tip) {
+        if (tip_sh) {   // callers that use a val-weighted lookup pass nullptr
+            const int s = I.states[(size_t)dad.id * I.Ppad + p];
+            const double* tp = tip_sh + s * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) dv[i] = tp[i];
+        }
+    } else {
+        const T* dsrc = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) dv[i] = (double)dsrc[i];
+        sc += scale[(size_t)dad.id * I.scale_stride + sidx];
+    }
+    minsc = sc;
+    adjust = 1.0;
+    if (safe) {
+        minsc = group_min_int<C>(sc);
+        adjust = Prec<T>::safe_adjust(sc - minsc);
+    }
+}
+
+// theta = node * dad (tip dad: its eigen-basis tip vector) and buffer_scale
+// (computeLikelihoodBufferSIMD, tree/phylokernelnew.h:2091-2230)
+template <typename T, int N, int C>
+__device__ inline void theta_apply(const InstDev& I, plk_end node, plk_end dad, const double* tip_sh,
+                                   int p, int c) {
+    double nv[N], dv[N], adjust;
+    int minsc;
+    load_branch<T, N, C>(I, node, dad, tip_sh, p, c, nv, dv, minsc, adjust);
+    T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * (N * C) + c * N;
+#pragma unroll
+    for (int i = 0; i < N; i++) th[i] = (T)(nv[i] * dv[i] * adjust);
+    if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
+}
+
+// Fused post-order traversal over the op list followed, per mode, by the
+// branch lnL (computeLikelihoodBranchSIMD 2753-3104) or the theta buffer of
+// the branch (node, dad).
 template <typename T, int N, int C>
 __global__ void __launch_bounds__(kThreads)
 k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
            const T* __restrict__ tables, int table_size, int mode, plk_end node, plk_end dad,
            const double* __restrict__ root_tab, double* __restrict__ block_sums) {
-    using ST = typename Prec<T>::scale_t;
     extern __shared__ __align__(16) unsigned char smem[];
     T* inv_sh = reinterpret_cast<T*>(smem);                 // [N*N]
     T* tab_sh = inv_sh + N * N;                              // [2][table_size]
@@ -236,9 +371,6 @@ k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
     const int c = threadIdx.x % C;
     const int p = blockIdx.x * PB + threadIdx.x / C;
     const bool live = p < I.Ppad;
-    const bool safe = I.safe != 0;
-    T* partial = reinterpret_cast<T*>(I.partial);
-    ST* scale = reinterpret_cast<ST*>(I.scale);
 
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)m.inv_evec[t];
 
@@ -255,98 +387,20 @@ k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
         for (int t = threadIdx.x; t < szL; t += kThreads) tab_sh[t] = gL[t];
         for (int t = threadIdx.x; t < szR; t += kThreads) tab_sh[table_size + t] = gR[t];
         __syncthreads();
-        if (!live) continue;
-        const T* tL = tab_sh;
-        const T* tR = tab_sh + table_size;
-
-        T tmp[N];
-        int sc_l = 0, sc_r = 0;
-        const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
-        if (L.is_tip) {
-            const int s = I.states[(size_t)L.id * I.Ppad + p];
-            const T* lut = tL + (s * C + c) * N;
-#pragma unroll
-            for (int x = 0; x < N; x++) tmp[x] = lut[x];
-        } else {
-            const T* src = partial + (size_t)L.id * I.slot_stride + (size_t)p * block + c * N;
-            T cv[N];
-#pragma unroll
-            for (int i = 0; i < N; i++) cv[i] = src[i];
-            const T* E = tL + c * N * N;
-#pragma unroll
-            for (int x = 0; x < N; x++) {
-                T acc = E[x * N] * cv[0];
-#pragma unroll
-                for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
-                tmp[x] = acc;
-            }
-            sc_l = scale[(size_t)L.id * I.scale_stride + sidx];
-        }
-        if (R.is_tip) {
-            const int s = I.states[(size_t)R.id * I.Ppad + p];
-            const T* lut = tR + (s * C + c) * N;
-#pragma unroll
-            for (int x = 0; x < N; x++) tmp[x] *= lut[x];
-        } else {
-            const T* src = partial + (size_t)R.id * I.slot_stride + (size_t)p * block + c * N;
-            T cv[N];
-#pragma unroll
-            for (int i = 0; i < N; i++) cv[i] = src[i];
-            const T* E = tR + c * N * N;
-#pragma unroll
-            for (int x = 0; x < N; x++) {
-                T acc = E[x * N] * cv[0];
-#pragma unroll
-                for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
-                tmp[x] *= acc;
-            }
-            sc_r = scale[(size_t)R.id * I.scale_stride + sidx];
-        }
-        // back to the eigen basis: dad[i] = sum_x inv_evec[i][x] tmp[x]
-        T dadv[N];
-        T lmax = (T)0;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            T acc = inv_sh[i * N] * tmp[0];
-#pragma unroll
-            for (int x = 1; x < N; x++) acc = fma(inv_sh[i * N + x], tmp[x], acc);
-            dadv[i] = acc;
-            lmax = fmax(lmax, fabs(acc));
-        }
-        int sc = sc_l + sc_r;
-        const bool tiptip = L.is_tip && R.is_tip;
-        if (!tiptip) {
-            const bool no_invar = I.invar[p] == 0.0;
-            if (safe) {
-                if (Prec<T>::underflow(lmax) && no_invar) {
-                    const int k = Prec<T>::exponent(lmax);
-#pragma unroll
-                    for (int i = 0; i < N; i++) dadv[i] = Prec<T>::apply(dadv[i], k);
-                    sc += k;
-                }
-            } else {
-                const T gmax = (T)group_max<C>((double)lmax);
-                if (Prec<T>::underflow(gmax) && no_invar) {
-                    const int k = Prec<T>::exponent(gmax);
-#pragma unroll
-                    for (int i = 0; i < N; i++) dadv[i] = Prec<T>::apply(dadv[i], k);
-                    sc += k;
-                }
-            }
-        }
-        T* dst = partial + (size_t)op.dad_slot * I.slot_stride + (size_t)p * block + c * N;
-#pragma unroll
-        for (int i = 0; i < N; i++) dst[i] = dadv[i];
-        if (safe || c == 0) scale[(size_t)op.dad_slot * I.scale_stride + sidx] = (ST)sc;
+        if (live)
+            op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + table_size, L.is_tip, L.id, R.is_tip, R.id,
+                              op.dad_slot, p, c);
     }
 
     if (mode == MODE_PARTIALS) return;
 
     // ---- root stage ---------------------------------------------------------
+    // MODE_BRANCH: root_tab = val[c][i] then, for a tip dad, lut[s][c][i] = val * tip
+    // MODE_THETA:  the eigen-basis tip vectors
     double* root_sh = reinterpret_cast<double*>(tab_sh);
-    const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
     __syncthreads();
     if (mode == MODE_BRANCH) {
+        const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
         for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = root_tab[t];
     } else if (dad.is_tip) {
         for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = m.tip[t];
@@ -354,67 +408,35 @@ k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
     __syncthreads();
     double contrib = 0.0;
     if (live) {
-        const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
-        const T* nsrc = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
-        double nv[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) nv[i] = (double)nsrc[i];
-        int sc = scale[(size_t)node.id * I.scale_stride + sidx];
-        double dv[N];
-        if (dad.is_tip) {
-            const int s = I.states[(size_t)dad.id * I.Ppad + p];
-            if (mode == MODE_BRANCH) {
-                const double* lut = root_sh + block + (s * C + c) * N;
-#pragma unroll
-                for (int i = 0; i < N; i++) dv[i] = lut[i];
-            } else {
-                const double* tp = root_sh + s * N;
-#pragma unroll
-                for (int i = 0; i < N; i++) dv[i] = tp[i];
-            }
+        if (mode == MODE_THETA) {
+            theta_apply<T, N, C>(I, node, dad, root_sh, p, c);
         } else {
-            const T* dsrc = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
-#pragma unroll
-            for (int i = 0; i < N; i++) dv[i] = (double)dsrc[i];
-            sc += scale[(size_t)dad.id * I.scale_stride + sidx];
-        }
-        int minsc = sc;
-        double adjust = 1.0;
-        if (safe) {
-            minsc = group_min_int<C>(sc);
-            adjust = Prec<T>::safe_adjust(sc - minsc);
-        }
-        if (mode == MODE_BRANCH) {
+            double nv[N], dv[N], adjust;
+            int minsc;
+            // a tip dad reads the val-weighted lookup lut[s][c][i] below instead of dv
+            load_branch<T, N, C>(I, node, dad, nullptr, p, c, nv, dv, minsc, adjust);
             double lhc;
             if (dad.is_tip) {
-                lhc = dv[0] * nv[0];
+                const int s = I.states[(size_t)dad.id * I.Ppad + p];
+                const double* lut = root_sh + block + (s * C + c) * N;
+                lhc = lut[0] * nv[0];
 #pragma unroll
-                for (int i = 1; i < N; i++) lhc = fma(dv[i], nv[i], lhc);
+                for (int i = 1; i < N; i++) lhc = fma(lut[i], nv[i], lhc);
             } else {
                 const double* val = root_sh + c * N;
                 lhc = val[0] * dv[0] * nv[0];
 #pragma unroll
                 for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
             }
-            double lh;
-            if (safe) {
-                lh = group_sum_sumvec<C>(lhc * adjust);
-            } else {
-                lh = group_sum_ordered<C>(lhc);
-            }
+            const double lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
             if (c == 0) {
-                lh = fabs(lh) + I.invar[p];
+                const double lh = fabs(lh0) + I.invar[p];
                 const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
                 if (p < I.P) {
                     I.ptn_lh[p] = lnp;
                     contrib = lnp * I.freq[p];
                 }
             }
-        } else {
-            T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
-#pragma unroll
-            for (int i = 0; i < N; i++) th[i] = (T)(nv[i] * dv[i] * adjust);
-            if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
         }
     }
     if (mode == MODE_BRANCH) {

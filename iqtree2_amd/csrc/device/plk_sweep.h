@@ -1,0 +1,413 @@
+// Device-resident branch-length optimisation: one cooperative launch runs an
+// optimizeAllBranches sweep (tree/phylotree.cpp:2698-2773) - for every branch
+// in order, the stale partials (transition tables built in-kernel from the
+// device copy of the branch lengths), theta, and minimizeNewton
+// (utils/optimization.cpp:422-499) with its exact control flow.
+//
+// Every workgroup owns a fixed set of pattern tiles for the whole launch, so
+// partial updates and theta only read values the same thread wrote; the only
+// grid-wide exchange is one reduction per likelihood evaluation:
+//   - each workgroup stores its partial sums and then its own arrival flag
+//     (one cache line each, no shared counter);
+//   - workgroup 0 polls every flag in parallel, adds the partial sums in a
+//     fixed order, stores the result and then a release flag;
+//   - every workgroup polls the release flag and reads the result.
+// Everything exchanged goes through device-scope atomic loads and stores, so
+// it is coherent across the XCDs' L2 caches without writing the caches back;
+// a store of data is acknowledged (s_waitcnt) before the flag that publishes
+// it is stored, and a reader issues its data load only after it has seen the
+// flag. Flags carry an epoch that grows by one per reduction. All workgroups
+// therefore hold bit-identical (df, ddf) and take identical Newton decisions.
+// Every spin is bounded: a workgroup that waits too long raises a shared
+// abort flag, every spinning workgroup checks it, and all of them leave the
+// kernel, which then reports an error.
+#pragma once
+#include "plk_kernels.h"
+
+namespace plkdev {
+
+constexpr long kSpinLimit = 1L << 24;   // ~0.5 s of polling before giving up
+constexpr int kFlagStride = 32;         // one 128-byte line per arrival flag
+
+struct SweepArgs {
+    InstDev I;
+    ModelDev m;
+    const plk_sweep_op* ops;
+    const plk_sweep_branch* br;
+    int nbranches;
+    int nedges;
+    double* len;          // [nedges], written by workgroup 0
+    double min_len, max_len;
+    int max_nr;
+    int table_size;
+    double* part;         // [gridDim.x][2] per-workgroup partial sums
+    unsigned* arrive;     // [gridDim.x * kFlagStride] arrival epochs (zeroed per launch)
+    unsigned* release;    // [kFlagStride] release epoch, [1] abort flag
+    double* result;       // [2] reduced sums
+    int* nderv;           // [nbranches]
+    double* out;          // [0] lnL from buffer after the sweep, [1] status
+    long long* timing;    // nullable: wall-clock ticks of workgroup 0 per phase
+};
+
+// phases accumulated in SweepArgs::timing (workgroup 0, thread 0)
+enum SweepPhase { PH_OPS = 0, PH_THETA, PH_EVAL_WORK, PH_EXCHANGE, PH_TOTAL, PH_COUNT };
+
+__device__ inline unsigned load_flag(const unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_flag(unsigned* p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_value(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                       (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_value(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+// every store this thread issued has been acknowledged
+__device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0); }
+
+// bounded wait until *flag == epoch; false on timeout or abort
+__device__ inline bool wait_epoch(const unsigned* flag, unsigned epoch, unsigned* abort_flag) {
+    for (long it = 0; it < kSpinLimit; it++) {
+        if (load_flag(flag) == epoch) return true;
+        if ((it & 31) == 31 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
+}
+
+// Grid-wide fixed-order sum of two per-workgroup values (valid in thread 0
+// on entry); every workgroup gets the same res[0..1]. Returns false when the
+// exchange was aborted.
+__device__ inline bool grid_sum2(const SweepArgs& a, double v0, double v1, unsigned epoch,
+                                 double* red_sh, double* res) {
+    __shared__ int ok_sh;
+    if (threadIdx.x == 0) {
+        store_value(&a.part[2 * blockIdx.x + 0], v0);
+        store_value(&a.part[2 * blockIdx.x + 1], v1);
+        stores_done();
+        store_flag(&a.arrive[blockIdx.x * kFlagStride], epoch);
+        ok_sh = 1;
+    }
+    __syncthreads();
+    unsigned* abort_flag = &a.release[1];
+    if (blockIdx.x == 0) {
+        bool ok = true;
+        double s0 = 0.0, s1 = 0.0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
+            if (!wait_epoch(&a.arrive[b * kFlagStride], epoch, abort_flag)) { ok = false; break; }
+            s0 += load_value(&a.part[2 * b + 0]);
+            s1 += load_value(&a.part[2 * b + 1]);
+        }
+        if (!ok) ok_sh = 0;
+        __syncthreads();
+        if (ok_sh) {
+            red_sh[threadIdx.x] = s0;
+            red_sh[kThreads + threadIdx.x] = s1;
+            __syncthreads();
+            for (int w = kThreads / 2; w >= 1; w >>= 1) {
+                if ((int)threadIdx.x < w) {
+                    red_sh[threadIdx.x] += red_sh[threadIdx.x + w];
+                    red_sh[kThreads + threadIdx.x] += red_sh[kThreads + threadIdx.x + w];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                store_value(&a.result[0], red_sh[0]);
+                store_value(&a.result[1], red_sh[kThreads]);
+                stores_done();
+                store_flag(&a.release[0], epoch);
+            }
+        }
+    }
+    if (threadIdx.x == 0 && ok_sh) {
+        if (wait_epoch(&a.release[0], epoch, abort_flag)) {
+            res[0] = load_value(&a.result[0]);
+            res[1] = load_value(&a.result[1]);
+        } else {
+            ok_sh = 0;
+        }
+    }
+    __syncthreads();
+    return ok_sh != 0;
+}
+
+// LDS layout shared by the host (size) and the kernel (offsets)
+struct SweepSmem {
+    int tip, ex, val, red, lens, ctl, inv, tab, total;
+    __host__ __device__ SweepSmem(int N, int C, int SU, int table_size, int nedges, int tsize) {
+        int off = 0;
+        tip = off;  off += (SU + 1) * N * 8;
+        ex = off;   off += 2 * C * N * 8;
+        val = off;  off += 3 * C * N * 8;
+        red = off;  off += 2 * kThreads * 8;
+        lens = off; off += nedges * 8;
+        ctl = off;  off += 8 * 8;
+        inv = off;  off += (N * N * tsize + 15) / 16 * 16;
+        tab = off;  off += 2 * table_size * tsize;
+        total = (off + 15) / 16 * 16;
+    }
+};
+
+// Transition tables of both children of an op, built by the whole workgroup
+// with the same arithmetic as k_prep_ops.
+template <typename T, int N, int C>
+__device__ inline void build_tables(const ModelDev& m, int SU, int table_size, const int* is_tip,
+                                    const double* len, T* tab_sh, double* ex_sh) {
+    for (int t = threadIdx.x; t < 2 * C * N; t += kThreads) {
+        const int k = t / (C * N), r = t % (C * N), c = r / N, i = r % N;
+        ex_sh[t] = exp(m.eval[i] * (m.rates[c] * len[k]));
+    }
+    __syncthreads();
+    for (int k = 0; k < 2; k++) {
+        T* out = tab_sh + k * table_size;
+        const double* ex = ex_sh + k * C * N;
+        if (!is_tip[k]) {
+            for (int t = threadIdx.x; t < C * N * N; t += kThreads) {
+                const int i = t % N, x = (t / N) % N, c = t / (N * N);
+                out[t] = (T)(m.evec[x * N + i] * ex[c * N + i]);
+            }
+        } else {
+            for (int t = threadIdx.x; t < (SU + 1) * C * N; t += kThreads) {
+                const int x = t % N, c = (t / N) % C, s = t / (N * C);
+                double v;
+                if (s == SU) {
+                    v = 1.0;
+                } else {
+                    const double* ev = m.evec + x * N;
+                    const double* tp = m.tip + s * N;
+                    const double* e = ex + c * N;
+                    v = ev[0] * e[0] * tp[0];
+                    for (int i = 1; i < N; i++) v += ev[i] * e[i] * tp[i];
+                }
+                out[t] = (T)v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Grid-wide evaluation on theta. mode 0: (df, ddf) at x; mode 1: lnL at x and
+// at x2 (x2 < 0: only x), writing pattern lnL for x when store_ptn. Results
+// land in res[0..1] of every workgroup.
+template <typename T, int N, int C>
+__device__ inline bool sweep_eval(const SweepArgs& a, int mode, double x, double x2, bool store_ptn,
+                                  unsigned& epoch, double* val_sh, double* red_sh, double* res) {
+    const long long t0 = wall_clock64();
+    constexpr int PB = kThreads / C;
+    constexpr int block = N * C;
+    const InstDev& I = a.I;
+    for (int t = threadIdx.x; t < N * C; t += kThreads) {
+        const int cc = t / N, i = t % N;
+        const double cof = a.m.eval[i] * a.m.rates[cc];
+        if (mode == 0) {
+            const double v0 = exp(cof * x) * a.m.props[cc];
+            const double v1 = cof * v0;
+            val_sh[t] = v0;
+            val_sh[N * C + t] = v1;
+            val_sh[2 * N * C + t] = cof * v1;
+        } else {
+            val_sh[t] = exp(cof * x) * a.m.props[cc];
+            val_sh[N * C + t] = x2 >= 0.0 ? exp(cof * x2) * a.m.props[cc] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int c = threadIdx.x % C;
+    const int ntiles = I.Ppad / PB;
+    double acc[2] = {0.0, 0.0};
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int p = tile * PB + threadIdx.x / C;
+        const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
+        const double* v0 = val_sh + c * N;
+        const double* v1 = val_sh + N * C + c * N;
+        const double* v2 = val_sh + 2 * N * C + c * N;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double t = (double)th[i];
+            s0 = fma(v0[i], t, s0);
+            s1 = fma(v1[i], t, s1);
+            if (mode == 0) s2 = fma(v2[i], t, s2);
+        }
+        s0 = group_sum_ordered<C>(s0);
+        s1 = group_sum_ordered<C>(s1);
+        if (mode == 0) s2 = group_sum_ordered<C>(s2);
+        if (c == 0 && p < I.P) {
+            const double f = I.freq[p];
+            if (mode == 0) {
+                const double lh = fabs(s0) + I.invar[p];
+                const double inv = 1.0 / lh;
+                const double df_frac = s1 * inv, ddf_frac = s2 * inv;
+                const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
+                acc[0] += tmp1;
+                acc[1] += tmp2 - tmp1 * df_frac;
+            } else {
+                const double lnp = log(fabs(s0) + I.invar[p]) + I.buffer_scale[p];
+                if (store_ptn) I.ptn_lh[p] = lnp;
+                acc[0] += lnp * f;
+                if (x2 >= 0.0) acc[1] += (log(fabs(s1) + I.invar[p]) + I.buffer_scale[p]) * f;
+            }
+        }
+    }
+    block_sum<2>(acc, red_sh);
+    epoch++;
+    const long long t1 = wall_clock64();
+    const bool ok = grid_sum2(a, acc[0], acc[1], epoch, red_sh, res);
+    if (a.timing && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long t2 = wall_clock64();
+        a.timing[PH_EVAL_WORK] += t1 - t0;
+        a.timing[PH_EXCHANGE] += t2 - t1;
+    }
+    return ok;
+}
+
+template <typename T, int N, int C>
+__global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const InstDev& I = a.I;
+    const SweepSmem L(N, C, I.SU, a.table_size, a.nedges, (int)sizeof(T));
+    double* tip_sh = reinterpret_cast<double*>(smem + L.tip);
+    double* ex_sh = reinterpret_cast<double*>(smem + L.ex);
+    double* val_sh = reinterpret_cast<double*>(smem + L.val);
+    double* red_sh = reinterpret_cast<double*>(smem + L.red);
+    double* lens = reinterpret_cast<double*>(smem + L.lens);
+    double* ctl = reinterpret_cast<double*>(smem + L.ctl);
+    T* inv_sh = reinterpret_cast<T*>(smem + L.inv);
+    T* tab_sh = reinterpret_cast<T*>(smem + L.tab);
+    constexpr int PB = kThreads / C;
+    const int c = threadIdx.x % C;
+    const int ntiles = I.Ppad / PB;
+
+    for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)a.m.inv_evec[t];
+    for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) tip_sh[t] = a.m.tip[t];
+    for (int t = threadIdx.x; t < a.nedges; t += kThreads) lens[t] = a.len[t];
+    __syncthreads();
+
+    unsigned epoch = 0;
+    int status = 0;
+    const bool timed = a.timing && blockIdx.x == 0 && threadIdx.x == 0;
+    const long long t_start = wall_clock64();
+    const double x1 = a.min_len, x2 = a.max_len, xacc = a.min_len;
+    for (int j = 0; j < a.nbranches && status == 0; j++) {
+        const plk_sweep_branch B = a.br[j];
+        long long tp = wall_clock64();
+        // stale partials of this branch
+        for (int o = B.op_begin; o < B.op_begin + B.op_count; o++) {
+            const plk_sweep_op op = a.ops[o];
+            int left = 0, right = 1;
+            if (!op.child[0].is_tip && op.child[1].is_tip) { left = 1; right = 0; }
+            const int tips[2] = {op.child[left].is_tip, op.child[right].is_tip};
+            const double ln[2] = {lens[op.child[left].edge], lens[op.child[right].edge]};
+            build_tables<T, N, C>(a.m, I.SU, a.table_size, tips, ln, tab_sh, ex_sh);
+            for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+                const int p = tile * PB + threadIdx.x / C;
+                op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + a.table_size, tips[0],
+                                  op.child[left].id, tips[1], op.child[right].id, op.dad_slot, p, c);
+            }
+            __syncthreads();
+        }
+        if (timed) {
+            const long long now = wall_clock64();
+            a.timing[PH_OPS] += now - tp;
+            tp = now;
+        }
+        for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+            theta_apply<T, N, C>(I, B.node, B.dad, tip_sh, tile * PB + threadIdx.x / C, c);
+        if (timed) a.timing[PH_THETA] += wall_clock64() - tp;
+
+        // minimizeNewton(min_len, len, max_len, xacc = min_len, max_nr) on
+        // f = -dlnL/dt (PhyloTree::computeFuncDerv); every thread holds the
+        // same scalars
+        const double len0 = lens[B.edge];
+        int nd = 0;
+        double f, df, dx, dxold, xl, xh, rts, rts_old, result = 0.0;
+        bool done = false;
+        rts = len0;
+        if (rts < x1) rts = x1;
+        if (rts > x2) rts = x2;
+        if (!sweep_eval<T, N, C>(a, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+            status = PLK_ERR_DEVICE;
+            break;
+        }
+        nd++;
+        f = -ctl[0];
+        df = -ctl[1];
+        if (!isfinite(f) || !isfinite(df)) {
+            status = PLK_ERR_NONFINITE;
+            break;
+        }
+        if (df >= 0.0 && fabs(f) < xacc) {
+            result = rts;
+            done = true;
+        }
+        if (!done) {
+            if (f < 0.0) { xl = rts; xh = x2; } else { xh = rts; xl = x1; }
+            dx = dxold = fabs(xh - xl);
+            for (int it = 1; it <= a.max_nr; it++) {
+                rts_old = rts;
+                if ((df <= 0.0) || (((rts - xh) * df - f) * ((rts - xl) * df - f) >= 0.0)) {
+                    dxold = dx;
+                    dx = 0.5 * (xh - xl);
+                    rts = xl + dx;
+                    if (xl == rts) { result = rts; done = true; break; }
+                } else {
+                    dxold = dx;
+                    dx = f / df;
+                    const double temp = rts;
+                    rts -= dx;
+                    if (temp == rts) { result = rts; done = true; break; }
+                }
+                if (fabs(dx) < xacc || it == a.max_nr) { result = rts_old; done = true; break; }
+                if (!sweep_eval<T, N, C>(a, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+                    status = PLK_ERR_DEVICE;
+                    break;
+                }
+                nd++;
+                f = -ctl[0];
+                df = -ctl[1];
+                if (!isfinite(f) || !isfinite(df)) { status = PLK_ERR_NONFINITE; break; }
+                if (df > 0.0 && fabs(f) < xacc) { result = rts; done = true; break; }
+                if (f < 0.0) xl = rts;
+                else if (f > 0.0) xh = rts;
+            }
+            (void)dxold;
+            if (status != 0) break;
+            if (!done) { status = PLK_ERR_NEWTON; break; }
+        }
+        double optx = result;
+        // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625)
+        if (optx > x2 * 0.95) {
+            if (!sweep_eval<T, N, C>(a, 1, optx, len0, false, epoch, val_sh, red_sh, ctl)) {
+                status = PLK_ERR_DEVICE;
+                break;
+            }
+            if (ctl[1] > ctl[0]) optx = len0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) lens[B.edge] = optx;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.len[B.edge] = optx;
+            a.nderv[j] = nd;
+        }
+        __syncthreads();
+    }
+    // computeLikelihoodFromBuffer at the last branch
+    if (status == 0 && a.nbranches > 0) {
+        const double last = lens[a.br[a.nbranches - 1].edge];
+        if (!sweep_eval<T, N, C>(a, 1, last, -1.0, true, epoch, val_sh, red_sh, ctl))
+            status = PLK_ERR_DEVICE;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.out[0] = status == 0 ? ctl[0] : 0.0;
+        a.out[1] = (double)status;
+        if (a.timing) a.timing[PH_TOTAL] += wall_clock64() - t_start;
+    }
+}
+
+}  // namespace plkdev

@@ -210,17 +210,13 @@ int plkh_workload_get(const plkh_workload* wl, const char* name, void* out) {
         const double p[4] = {w.model.alpha, w.model.p_inv, w.model.kappa, w.model.omega};
         std::memcpy(out, p, sizeof(p));
     } else if (n == "edges_u" || n == "edges_v" || n == "edges_len") {
+        // in edge-id order
         const Tree& t = w.tree;
-        int e = 0;
-        for (int x = 0; x < t.nnodes; x++)
-            for (int k = 0; k < t.deg[x]; k++) {
-                const int y = t.adj[3 * x + k];
-                if (y < x) continue;
-                if (n == "edges_u") static_cast<int32_t*>(out)[e] = x;
-                else if (n == "edges_v") static_cast<int32_t*>(out)[e] = y;
-                else static_cast<double*>(out)[e] = t.len[3 * x + k];
-                e++;
-            }
+        for (int e = 0; e < t.nbranch(); e++) {
+            if (n == "edges_u") static_cast<int32_t*>(out)[e] = t.eu[e];
+            else if (n == "edges_v") static_cast<int32_t*>(out)[e] = t.ev[e];
+            else static_cast<double*>(out)[e] = t.length(t.eu[e], t.ev[e]);
+        }
     } else {
         return PLK_ERR_ARG;
     }
@@ -332,6 +328,12 @@ int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double toleran
                                     double* lnl, int* n_derv) {
     if (!t || iterations < 1 || max_nr < 1) return PLK_ERR_ARG;
     return t->t->optimize_all_branches(iterations, tolerance, max_nr, lnl, n_derv);
+}
+
+int plkh_tree_set_device_newton(plkh_tree* t, int on) {
+    if (!t) return PLK_ERR_ARG;
+    t->t->device_newton = on != 0;
+    return PLK_OK;
 }
 
 int plkh_tree_set_branch_bounds(plkh_tree* t, double min_len, double max_len) {

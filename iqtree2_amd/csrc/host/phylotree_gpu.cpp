@@ -48,7 +48,7 @@ int PhyloTreeGPU::subtree_size(int node, int dad) {
 }
 
 // computeTraversalInfo(dad_branch = dad->node, dad) (tree/phylotree.cpp:5918-6028)
-void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops) {
+void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops, std::vector<int>* edges) {
     if (node < tree_.ntaxa) return;
     const int k = tree_.neighbor_index(node, dad);
     if (computed_[(size_t)node * 3 + k]) return;
@@ -60,7 +60,7 @@ void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops) {
     // larger subtree first (5931-5941): keeps the live set of a post-order small
     if (nc == 2 && subtree_size(children[0], node) < subtree_size(children[1], node))
         std::swap(children[0], children[1]);
-    for (int i = 0; i < nc; i++) plan_rec(children[i], node, ops);
+    for (int i = 0; i < nc; i++) plan_rec(children[i], node, ops, edges);
     computed_[(size_t)node * 3 + k] = 1;
     // one buffer per internal node, taken over by the orientation in use
     // (reorientPartialLh 5898-5912, MemSlotVector::takeover)
@@ -76,8 +76,47 @@ void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops) {
         op.child[i].is_tip = ch < tree_.ntaxa;
         op.child[i].id = ch < tree_.ntaxa ? ch : slot(ch);
         op.child[i].length = tree_.length(node, ch);
+        if (edges) edges->push_back(tree_.edge_id(node, ch));
     }
     ops.push_back(op);
+}
+
+void PhyloTreeGPU::plan_sweep(const std::vector<int>& nodes, const std::vector<int>& nodes2,
+                              std::vector<plk_sweep_op>& sops,
+                              std::vector<plk_sweep_branch>& branches) {
+    sops.clear();
+    branches.clear();
+    for (size_t j = 0; j < nodes.size(); j++) {
+        const int n1 = nodes[j], n2 = nodes2[j];
+        // computeLikelihoodDerv(current_it = n1->n2, n1): node = n2, dad = n1
+        int nd = n2, dd = n1;
+        if (nd < tree_.ntaxa) std::swap(nd, dd);
+        std::vector<plk_op> ops;
+        std::vector<int> edges;
+        plan_rec(nd, dd, ops, &edges);
+        plan_rec(dd, nd, ops, &edges);
+        plk_sweep_branch B{};
+        B.op_begin = (int)sops.size();
+        B.op_count = (int)ops.size();
+        B.node = plk_end{0, slot(nd)};
+        B.dad = dd < tree_.ntaxa ? plk_end{1, dd} : plk_end{0, slot(dd)};
+        B.edge = tree_.edge_id(n1, n2);
+        for (size_t o = 0; o < ops.size(); o++) {
+            plk_sweep_op so{};
+            so.dad_slot = ops[o].dad_slot;
+            so.nchildren = ops[o].nchildren;
+            for (int k = 0; k < ops[o].nchildren; k++) {
+                so.child[k].is_tip = ops[o].child[k].is_tip;
+                so.child[k].id = ops[o].child[k].id;
+                so.child[k].edge = edges[2 * o + k];
+            }
+            sops.push_back(so);
+        }
+        branches.push_back(B);
+        // optimizeOneBranch(clearLH = true) after a length change
+        clear_reverse(n1, n2);
+        clear_reverse(n2, n1);
+    }
 }
 
 void PhyloTreeGPU::plan(int a, int b, std::vector<plk_op>& ops, plk_end& node, plk_end& dad,
@@ -231,9 +270,59 @@ int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, i
     return PLK_OK;
 }
 
-// PhyloTree::optimizeAllBranches(iterations, tolerance, maxNRStep) (2698-2773)
+// PhyloTree::optimizeAllBranches(iterations, tolerance, maxNRStep) (2698-2773),
+// every pass over the branches one device-resident Newton sweep
 int PhyloTreeGPU::optimize_all_branches(int iterations, double tolerance, int max_nr, double* lnl,
                                         int* n_derv) {
+    if (!ctx_) return PLK_ERR_STATE;
+    if (!device_newton) return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
+    std::vector<int> nodes, nodes2;
+    best_traversal(nodes, nodes2);
+    double tree_lh;
+    int st = likelihood_branch(nodes2[0], nodes[0], &tree_lh, nullptr);
+    if (st != PLK_OK) return st;
+    int total_derv = 0;
+    const int ne = tree_.nbranch();
+    for (int it = 0; it < iterations; it++) {
+        std::vector<double> saved(tree_.len);
+        std::vector<plk_sweep_op> sops;
+        std::vector<plk_sweep_branch> br;
+        plan_sweep(nodes, nodes2, sops, br);
+        std::vector<double> lens(ne);
+        for (int e = 0; e < ne; e++) lens[e] = tree_.length(tree_.eu[e], tree_.ev[e]);
+        std::vector<int> nd(br.size());
+        double new_lh = 0.0;
+        st = plk_optimize_sweep(ctx_, inst_, sops.data(), (int)sops.size(), br.data(), (int)br.size(),
+                                lens.data(), ne, min_len, max_len, max_nr, nd.data(), &new_lh);
+        if (st != PLK_OK) return st;
+        for (int e = 0; e < ne; e++) tree_.set_length(tree_.eu[e], tree_.ev[e], lens[e]);
+        for (int x : nd) total_derv += x;
+        theta_a_ = nodes.back();
+        theta_b_ = nodes2.back();
+        cur_a_ = theta_a_;
+        cur_b_ = theta_b_;
+        if (new_lh < tree_lh - tolerance * 0.1) {
+            clear_all_partial();
+            tree_.len = saved;
+            st = compute_likelihood(&new_lh, nullptr);
+            if (lnl) *lnl = new_lh;
+            if (n_derv) *n_derv = total_derv;
+            return st;
+        }
+        if (tree_lh <= new_lh && new_lh <= tree_lh + tolerance) {
+            tree_lh = new_lh;
+            break;
+        }
+        tree_lh = new_lh;
+    }
+    if (lnl) *lnl = tree_lh;
+    if (n_derv) *n_derv = total_derv;
+    return PLK_OK;
+}
+
+// the same pass with minimizeNewton on the host, one engine call per branch
+int PhyloTreeGPU::optimize_all_branches_host(int iterations, double tolerance, int max_nr,
+                                             double* lnl, int* n_derv) {
     if (!ctx_) return PLK_ERR_STATE;
     std::vector<int> nodes, nodes2;
     best_traversal(nodes, nodes2);

@@ -25,6 +25,13 @@ public:
     void best_traversal(std::vector<int>& nodes, std::vector<int>& nodes2);
     // computeTraversalInfo for branch (a, b)
     void plan(int a, int b, std::vector<plk_op>& ops, plk_end& node, plk_end& dad, double& len);
+    // the whole optimizeAllBranches pass as one device sweep: ops of every
+    // branch in order, assuming each optimised branch changes its length
+    void plan_sweep(const std::vector<int>& nodes, const std::vector<int>& nodes2,
+                    std::vector<plk_sweep_op>& sops, std::vector<plk_sweep_branch>& branches);
+    int optimize_all_branches_host(int iterations, double tolerance, int max_nr, double* lnl,
+                                   int* n_derv);
+    bool device_newton = true;
     void set_length(int a, int b, double len);
     void clear_all_partial();
     void reset();
@@ -41,7 +48,7 @@ public:
     double min_len = 1e-6, max_len = 10.0;
 
 private:
-    void plan_rec(int node, int dad, std::vector<plk_op>& ops);
+    void plan_rec(int node, int dad, std::vector<plk_op>& ops, std::vector<int>* edges = nullptr);
     int subtree_size(int node, int dad);
     int farthest_leaf(int node, int dad, std::vector<int>& height);
     void preorder(int node, int dad, const std::vector<int>& height, std::vector<int>& nodes,

@@ -9,11 +9,16 @@
 namespace plk {
 
 static void add_edge(Tree& t, int u, int v, double len) {
+    const int e = (int)t.eu.size();
+    t.eu.push_back(u);
+    t.ev.push_back(v);
     t.adj[3 * u + t.deg[u]] = v;
     t.len[3 * u + t.deg[u]] = len;
+    t.edge[3 * u + t.deg[u]] = e;
     t.deg[u]++;
     t.adj[3 * v + t.deg[v]] = u;
     t.len[3 * v + t.deg[v]] = len;
+    t.edge[3 * v + t.deg[v]] = e;
     t.deg[v]++;
 }
 
@@ -24,6 +29,7 @@ Tree tree_from_edges(int ntaxa, int nedges, const int* u, const int* v, const do
     t.deg.assign(t.nnodes, 0);
     t.adj.assign(3 * t.nnodes, -1);
     t.len.assign(3 * t.nnodes, 0.0);
+    t.edge.assign(3 * t.nnodes, -1);
     for (int e = 0; e < nedges; e++) add_edge(t, u[e], v[e], tl[e]);
     return t;
 }

@@ -20,7 +20,10 @@ struct Tree {
     std::vector<int> deg;                    // [nnodes]
     std::vector<int> adj;                    // [nnodes][3]
     std::vector<double> len;                 // [nnodes][3] (symmetric)
+    std::vector<int> edge;                   // [nnodes][3] edge id of each adjacency slot
+    std::vector<int> eu, ev;                 // [nnodes-1] endpoints of each edge id
     int nbranch() const { return nnodes - 1; }
+    int edge_id(int u, int v) const { return edge[3 * u + neighbor_index(u, v)]; }
     int neighbor_index(int v, int u) const {
         for (int k = 0; k < deg[v]; k++)
             if (adj[3 * v + k] == u) return k;

@@ -381,6 +381,10 @@ class PhyloTree:
               "plkh_tree_optimize_all_branches")
         return lnl.value, nd.value
 
+    def set_device_newton(self, on):
+        check(lib().plkh_tree_set_device_newton(self._h, int(bool(on))),
+              "plkh_tree_set_device_newton")
+
     def set_branch_bounds(self, min_len, max_len):
         check(lib().plkh_tree_set_branch_bounds(self._h, min_len, max_len),
               "plkh_tree_set_branch_bounds")

@@ -93,6 +93,103 @@ def cpu_baseline(ia, w, budget_s, threads):
                        f"({evals} tree evaluations x {w.npatterns} patterns, {elapsed:.1f} s)")
 
 
+def make_partitions(ia, nparts, seed=20261202 + 3):
+    """Config 4: one 64-taxon tree (Exp mean 0.1), partition lengths
+    lognormal(ln 800, 0.8) clipped to [100, 5000], 70% DNA GTR+G4 / 30%
+    protein +G4, independently seeded parameters (alpha ~ U(0.2, 2))."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    base = ia.Workload(seq_type=ia.DNA, ntaxa=64, nsite=10, mean_len=0.1, seed=seed)
+    edges = base.edges()
+    parts = []
+    for k in range(nparts):
+        nsite = int(np.clip(round(rng.lognormal(np.log(800.0), 0.8)), 100, 5000))
+        st = ia.PROTEIN if rng.random() < 0.3 else ia.DNA
+        parts.append(ia.Workload(seq_type=st, ntaxa=64, nsite=nsite, ncat=4, alpha=-1.0,
+                                 seed=seed * 1000 + k, edges=edges))
+    return parts
+
+
+def bench_partitions(ia, ctx, nparts, reps, cpu_seconds, threads):
+    import numpy as np
+    t0 = time.perf_counter()
+    parts = make_partitions(ia, nparts)
+    plans = []
+    for w in parts:
+        planner = ia.PhyloTree(None, w)
+        a, b = planner.branches()[0]
+        plans.append(planner.plan(a, b))
+    total_p = sum(w.npatterns for w in parts)
+    total_sites = sum(w.nsite for w in parts)
+    log(f"partitions: {nparts}, {total_sites} sites, {total_p} patterns "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    res = {"partitions": nparts, "sites": total_sites, "patterns": total_p}
+    lnl = {}
+    for prec, name in ((ia.PREC_F64, "fp64"), (ia.PREC_F32, "fp32")):
+        insts = [ia.Instance.from_workload(ctx, w, precision=prec) for w in parts]
+        jobs = [(inst, *plan) for inst, plan in zip(insts, plans)]
+        if prec == ia.PREC_F64:
+            # plain form: one computeLikelihoodBranch launch per partition
+            for inst, ops, node, dad, length in jobs:
+                inst.likelihood_branch(ops, node, dad, length)
+            ctx.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                single = [inst.likelihood_branch(ops, node, dad, length)
+                          for inst, ops, node, dad, length in jobs]
+            ctx.synchronize()
+            dt = (time.perf_counter() - t) / reps
+            res["plain_fp64_per_partition_evals_per_s"] = total_p / dt
+            res["plain_fp64_ms"] = 1e3 * dt
+            lnl["plain"] = np.array(single)
+        ia.likelihood_branch_batch(ctx, jobs)       # autotunes the launch shape once
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            out = ia.likelihood_branch_batch(ctx, jobs)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        res[f"batched_{name}_evals_per_s"] = total_p / dt
+        res[f"batched_{name}_ms"] = 1e3 * dt
+        lnl[name] = out
+        for inst in insts:
+            inst.close()
+    res["batched_fp64_vs_plain"] = res["batched_fp64_evals_per_s"] / res["plain_fp64_per_partition_evals_per_s"]
+    res["batched_fp32_vs_plain_fp64"] = res["batched_fp32_evals_per_s"] / res["plain_fp64_per_partition_evals_per_s"]
+    res["max_abs_dlnl_fp32_per_tree"] = float(np.max(np.abs(lnl["fp32"] - lnl["fp64"])))
+    res["max_rel_dlnl_fp32_per_tree"] = float(np.max(np.abs(lnl["fp32"] - lnl["fp64"]) / np.abs(lnl["fp64"])))
+    res["max_abs_dlnl_batched_vs_plain_fp64"] = float(np.max(np.abs(lnl["fp64"] - lnl["plain"])))
+    if cpu_seconds > 0:
+        res["cpu_baseline"] = cpu_partitions(ia, parts, plans, cpu_seconds, threads)
+        res["batched_fp32_vs_cpu"] = res["batched_fp32_evals_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
+def cpu_partitions(ia, parts, plans, budget_s, threads):
+    """Oracle over partitions the way PhyloSuperTree runs them: partitions in
+    parallel, one thread each, largest first, until the time budget is spent."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.binding import OracleTree
+    order = sorted(range(len(parts)), key=lambda k: -parts[k].npatterns * parts[k].nstates ** 2)
+    t0 = time.perf_counter()
+    done = []
+
+    def run(k):
+        if time.perf_counter() - t0 > budget_s:
+            return 0
+        orc = OracleTree.from_workload(parts[k], nthreads=1)
+        ops, node, dad, length = plans[k]
+        orc.likelihood_branch(ops, node, dad, length)
+        return parts[k].npatterns
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        done = list(ex.map(run, order))
+    elapsed = time.perf_counter() - t0
+    n = sum(1 for d in done if d)
+    return dict(value=sum(done) / elapsed, unit="site-lik evals/s", cores=threads, kind="port",
+                sample=f"{n} of {len(parts)} partitions, full traversal + lnL each ({elapsed:.1f} s)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +200,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--partitions", type=int, default=500)
+    ap.add_argument("--no-partitions", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -149,6 +248,13 @@ def main():
         log(f"fp32: {json.dumps(fp32)}")
         tree32.close()
 
+    parts = None
+    if not args.no_partitions:
+        parts = bench_partitions(ia, ctx, args.partitions, max(1, args.steps),
+                                 0.0 if args.no_cpu else args.cpu_seconds,
+                                 min(16, os.cpu_count() or 1))
+        log(f"partitions: {json.dumps(parts)}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -180,6 +286,7 @@ def main():
                      "bytes_per_launch": dom["bytes"] / dom["launches"]},
         "kernels": kernels,
         "fp32_storage": fp32,
+        "partition_batch": parts,
         "cpu_baseline": cpu,
     }
     tree.close()

@@ -148,6 +148,25 @@ plk_status plk_optimize_branch(plk_context* ctx, int instance, const plk_op* ops
                                double max_len, int max_nr, double* out_length, double* out_lnl,
                                int* n_derv);
 
+/* ---- batched evaluation (partitions, requests) -------------------------- */
+/* one computeLikelihoodBranch per item, items on distinct instances */
+typedef struct {
+    int instance;
+    const plk_op* ops;
+    int nops;
+    plk_end node;
+    plk_end dad;
+    double length;
+} plk_batch_item;
+
+/* PhyloSuperTree::computeLikelihood's partition loop
+ * (tree/phylosupertree.cpp:701-735) as one persistent launch: every item's
+ * traversal and branch lnL, instances of 4 and 20 states and either precision
+ * together, pattern tiles handed out largest-partition first. lnl[i] gets
+ * item i's tree lnL, identical to plk_likelihood_branch on that item alone. */
+plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
+                                       double* lnl);
+
 /* ---- device-resident branch optimisation ------------------------------- */
 /* A traversal step whose children carry edge ids instead of lengths: the
  * lengths are read from the device copy of the branch-length array, which

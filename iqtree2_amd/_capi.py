@@ -44,6 +44,11 @@ class PlkInstanceDesc(ctypes.Structure):
                 ("safe_numeric", c_int)]
 
 
+class PlkBatchItem(ctypes.Structure):
+    _fields_ = [("instance", c_int), ("ops", c_void_p), ("nops", c_int), ("node", PlkEnd),
+                ("dad", PlkEnd), ("length", c_double)]
+
+
 class PlkKernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", c_int), ("total_ms", c_double),
                 ("bytes", c_double), ("flops", c_double)]
@@ -89,6 +94,7 @@ _PROTOS = {
     "plk_optimize_branch": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd, c_double,
                                     c_double, c_double, c_int, _DP, _DP, P(c_int)]),
     "plk_rell": (c_int, [c_void_p, c_int, _F32, c_int, _DP]),
+    "plk_likelihood_branch_batch": (c_int, [c_void_p, P(PlkBatchItem), c_int, _DP]),
     "plk_optimize_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
                                    c_double, c_double, c_int, _I32, _DP]),
     "plk_read_partial": (c_int, [c_void_p, c_int, c_int, _DP, _I32]),

@@ -1,0 +1,216 @@
+// Batched likelihood over many instances (partitions of a PhyloSuperTree,
+// tree/phylosupertree.cpp:701-735, or independent requests) in one
+// persistent launch.
+//
+// Work list: every (instance, pattern tile) pair with real patterns, ordered
+// by the host largest-cost first (computePartitionOrder,
+// tree/phylosupertree.cpp:654-690). Workgroups take the next tile from an
+// atomic counter; a tile runs its instance's whole op list (transition tables
+// of all ops of all instances were built by one k_prep_batch launch) and the
+// root lnL of its instance's branch, and stores its lnL sum in its own slot.
+// k_batch_reduce then adds each instance's tile sums in tile order, so the
+// result does not depend on which workgroup ran which tile.
+#pragma once
+#include "plk_kernels.h"
+
+namespace plkdev {
+
+struct BatchItem {
+    InstDev I;
+    ModelDev m;
+    const plk_op* ops;     // this instance's ops inside the batch op array
+    const void* tables;    // [nops][2][table_size] storage precision
+    const double* root;    // val[C][n] then, for a tip dad, lut[SU+1][C][n]
+    int nops;
+    int table_size;
+    int tile_begin;        // first global tile of this instance
+    int ntiles;            // tiles with real patterns
+    plk_end node, dad;
+    int n, C, f32;
+};
+
+// One table per (item, op, child): k_prep_ops' arithmetic for a batch.
+struct PrepJob {
+    int item;
+    int op;
+};
+
+template <typename T>
+__device__ inline void prep_table(const BatchItem& it, int o, int k, double* ex) {
+    const int n = it.n, C = it.C, SU = it.I.SU;
+    const plk_child ch = it.ops[o].child[k];
+    for (int t = threadIdx.x; t < C * n; t += blockDim.x) {
+        const int c = t / n, i = t % n;
+        ex[t] = exp(it.m.eval[i] * (it.m.rates[c] * ch.length));
+    }
+    __syncthreads();
+    T* out = reinterpret_cast<T*>(const_cast<void*>(it.tables)) + (size_t)(2 * o + k) * it.table_size;
+    if (!ch.is_tip) {
+        for (int t = threadIdx.x; t < C * n * n; t += blockDim.x) {
+            const int i = t % n, x = (t / n) % n, c = t / (n * n);
+            out[t] = (T)(it.m.evec[x * n + i] * ex[c * n + i]);
+        }
+    } else {
+        for (int t = threadIdx.x; t < (SU + 1) * C * n; t += blockDim.x) {
+            const int x = t % n, c = (t / n) % C, s = t / (n * C);
+            double v;
+            if (s == SU) {
+                v = 1.0;
+            } else {
+                const double* ev = it.m.evec + x * n;
+                const double* tp = it.m.tip + s * n;
+                const double* e = ex + c * n;
+                v = ev[0] * e[0] * tp[0];
+                for (int i = 1; i < n; i++) v += ev[i] * e[i] * tp[i];
+            }
+            out[t] = (T)v;
+        }
+    }
+    __syncthreads();
+}
+
+// one workgroup per (item, op); root tables of every item by extra workgroups
+__global__ void __launch_bounds__(kThreads)
+k_prep_batch(const BatchItem* __restrict__ items, const PrepJob* __restrict__ jobs, int njobs,
+             int nitems, const double* __restrict__ lengths, double* __restrict__ roots) {
+    extern __shared__ double ex[];
+    if ((int)blockIdx.x < njobs) {
+        const PrepJob j = jobs[blockIdx.x];
+        const BatchItem& it = items[j.item];
+        for (int k = 0; k < 2; k++) {
+            if (it.f32) prep_table<float>(it, j.op, k, ex);
+            else prep_table<double>(it, j.op, k, ex);
+        }
+        return;
+    }
+    const int i = blockIdx.x - njobs;
+    if (i >= nitems) return;
+    const BatchItem& it = items[i];
+    const int n = it.n, C = it.C, nv = C * n, SU = it.I.SU;
+    double* out = const_cast<double*>(it.root);
+    const double len = lengths[i];
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+        const int c = t / n, q = t % n;
+        out[t] = exp(it.m.eval[q] * (it.m.rates[c] * len)) * it.m.props[c];
+    }
+    if (!it.dad.is_tip) return;
+    for (int t = threadIdx.x; t < (SU + 1) * nv; t += blockDim.x) {
+        const int q = t % n, c = (t / n) % C, s = t / nv;
+        out[nv + t] = exp(it.m.eval[q] * (it.m.rates[c] * len)) * it.m.props[c] * it.m.tip[s * n + q];
+    }
+}
+
+// all ops of one tile of one item, then its root lnL; returns the tile's
+// lnL sum (thread 0)
+template <typename T, int N, int C>
+__device__ inline double batch_tile(const BatchItem& it, int tile, unsigned char* smem) {
+    constexpr int PB = kThreads / C;
+    constexpr int block = N * C;
+    T* inv_sh = reinterpret_cast<T*>(smem);
+    T* tab_sh = inv_sh + N * N;
+    const InstDev& I = it.I;
+    const int c = threadIdx.x % C;
+    const int p = tile * PB + threadIdx.x / C;
+    const T* tables = reinterpret_cast<const T*>(it.tables);
+    __syncthreads();
+    for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)it.m.inv_evec[t];
+    for (int o = 0; o < it.nops; o++) {
+        const plk_op op = it.ops[o];
+        int left = 0, right = 1;
+        if (!op.child[0].is_tip && op.child[1].is_tip) { left = 1; right = 0; }
+        const plk_child L = op.child[left], R = op.child[right];
+        const int szL = L.is_tip ? (I.SU + 1) * block : block * N;
+        const int szR = R.is_tip ? (I.SU + 1) * block : block * N;
+        __syncthreads();
+        const T* gL = tables + (size_t)(2 * o + left) * it.table_size;
+        const T* gR = tables + (size_t)(2 * o + right) * it.table_size;
+        for (int t = threadIdx.x; t < szL; t += kThreads) tab_sh[t] = gL[t];
+        for (int t = threadIdx.x; t < szR; t += kThreads) tab_sh[it.table_size + t] = gR[t];
+        __syncthreads();
+        op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + it.table_size, L.is_tip, L.id, R.is_tip, R.id,
+                          op.dad_slot, p, c);
+    }
+    double* root_sh = reinterpret_cast<double*>(tab_sh);
+    const plk_end node = it.node, dad = it.dad;
+    __syncthreads();
+    const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
+    for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = it.root[t];
+    __syncthreads();
+    double nv[N], dv[N], adjust;
+    int minsc;
+    load_branch<T, N, C>(I, node, dad, nullptr, p, c, nv, dv, minsc, adjust);
+    double lhc;
+    if (dad.is_tip) {
+        const int s = I.states[(size_t)dad.id * I.Ppad + p];
+        const double* lut = root_sh + block + (s * C + c) * N;
+        lhc = lut[0] * nv[0];
+#pragma unroll
+        for (int i = 1; i < N; i++) lhc = fma(lut[i], nv[i], lhc);
+    } else {
+        const double* val = root_sh + c * N;
+        lhc = val[0] * dv[0] * nv[0];
+#pragma unroll
+        for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
+    }
+    const double lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
+    double contrib = 0.0;
+    if (c == 0) {
+        const double lh = fabs(lh0) + I.invar[p];
+        const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
+        if (p < I.P) {
+            I.ptn_lh[p] = lnp;
+            contrib = lnp * I.freq[p];
+        }
+    }
+    double v[1] = {contrib};
+    block_sum<1>(v, reinterpret_cast<double*>(smem));
+    return v[0];
+}
+
+// Instance classes a batch kernel is built for (bit mask)
+enum BatchClass { BC_DNA64 = 1, BC_PROT64 = 2, BC_DNA32 = 4, BC_PROT32 = 8, BC_ALL = 15 };
+
+__host__ __device__ inline int batch_class(int n, int f32) {
+    return n == 4 ? (f32 ? BC_DNA32 : BC_DNA64) : (f32 ? BC_PROT32 : BC_PROT64);
+}
+
+// Persistent workgroups over tiles [tile_begin, tile_end) of the list. MASK
+// names the instance classes the launch may meet; a tile's code path is
+// uniform across its workgroup.
+template <int MASK>
+__global__ void __launch_bounds__(kThreads)
+k_batch(const BatchItem* __restrict__ items, const int* __restrict__ tile_item, int tile_begin,
+        int tile_end, unsigned* __restrict__ counter, double* __restrict__ tile_sums) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int next_sh;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) next_sh = tile_begin + (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int g = next_sh;
+        if (g >= tile_end) break;
+        const int i = tile_item[g];
+        const BatchItem& it = items[i];
+        const int tile = g - it.tile_begin;
+        const int cls = batch_class(it.n, it.f32);
+        double s = 0.0;
+        if ((MASK & BC_DNA64) && cls == BC_DNA64) s = batch_tile<double, 4, 4>(it, tile, smem);
+        else if ((MASK & BC_PROT64) && cls == BC_PROT64) s = batch_tile<double, 20, 4>(it, tile, smem);
+        else if ((MASK & BC_DNA32) && cls == BC_DNA32) s = batch_tile<float, 4, 4>(it, tile, smem);
+        else if ((MASK & BC_PROT32) && cls == BC_PROT32) s = batch_tile<float, 20, 4>(it, tile, smem);
+        if (threadIdx.x == 0) tile_sums[g] = s;
+    }
+}
+
+// lnl[i] = sum of item i's tile sums in tile order (one thread per item)
+__global__ void __launch_bounds__(kThreads)
+k_batch_reduce(const BatchItem* __restrict__ items, int nitems, const double* __restrict__ tile_sums,
+               double* __restrict__ lnl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    double s = 0.0;
+    for (int t = 0; t < items[i].ntiles; t++) s += tile_sums[items[i].tile_begin + t];
+    lnl[i] = s;
+}
+
+}  // namespace plkdev

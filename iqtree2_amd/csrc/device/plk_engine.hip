@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -15,6 +16,7 @@
 #include "plk.h"
 #include "plk_kernels.h"
 #include "plk_sweep.h"
+#include "plk_batch.h"
 
 using namespace plkdev;
 
@@ -105,10 +107,10 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 }  // namespace
 
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
-                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_NKIND };
+                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
-                                          "reduce", "rell", "newton_sweep"};
+                                          "reduce", "rell", "newton_sweep", "batch", "prep_batch"};
 
 struct ProfRec {
     int kind;
@@ -130,6 +132,29 @@ struct plk_context {
     std::mutex mu;
     std::vector<Instance*> inst;
     double* h_out = nullptr;  // pinned host staging for reduced scalars
+    // batched evaluation buffers (grown on demand)
+    BatchItem* bt_items = nullptr;
+    int bt_items_cap = 0;
+    PrepJob* bt_jobs = nullptr;
+    int bt_jobs_cap = 0;
+    plk_op* bt_ops = nullptr;
+    int bt_ops_cap = 0;
+    unsigned char* bt_tables = nullptr;
+    int bt_tables_cap = 0;
+    double* bt_roots = nullptr;
+    int bt_roots_cap = 0;
+    int* bt_tile_item = nullptr;
+    int bt_tile_cap = 0;
+    double* bt_tile_sums = nullptr;
+    int bt_tile_sums_cap = 0;
+    double* bt_vals = nullptr;      // lengths in, lnl out
+    int bt_vals_cap = 0;
+    unsigned* bt_counter = nullptr;
+    int bt_counter_cap = 0;
+    int batch_blocks_per_cu = 0;    // 0: autotuned
+    // autotuner: key (class mix, log2 tiles) -> (split launches, blocks per CU)
+    std::map<long, std::pair<int, int>> batch_tune;
+    bool autotune = true;
 };
 
 namespace {
@@ -479,6 +504,8 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     HIP_TRY(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
     if (const char* e = std::getenv("PLK_SWEEP_BLOCKS_PER_CU")) ctx->sweep_blocks_per_cu = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PLK_BATCH_BLOCKS_PER_CU")) ctx->batch_blocks_per_cu = std::max(1, std::atoi(e));
+    if (std::getenv("PLK_NO_AUTOTUNE")) ctx->autotune = false;
     ctx->budget = (cfg && cfg->hbm_budget) ? cfg->hbm_budget : (size_t)(0.9 * (double)free_b);
     hipError_t e = hipHostMalloc((void**)&ctx->h_out, 64 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -496,6 +523,10 @@ plk_status plk_destroy(plk_context* ctx) {
         if (ctx->inst[i]) plk_instance_destroy(ctx, i);
     (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+    void* bt[] = {ctx->bt_items, ctx->bt_jobs, ctx->bt_ops, ctx->bt_tables, ctx->bt_roots,
+                  ctx->bt_tile_item, ctx->bt_tile_sums, ctx->bt_vals, ctx->bt_counter};
+    for (void* b : bt)
+        if (b) (void)hipFree(b);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     (void)hipHostFree(ctx->h_out);
     delete ctx;
@@ -908,6 +939,230 @@ plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid)
 }  // namespace
 
 extern "C" {
+
+plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
+                                       double* lnl) {
+    if (!ctx || nitems < 0 || (nitems > 0 && (!items || !lnl))) return fail(PLK_ERR_ARG, "invalid batch");
+    if (nitems == 0) return PLK_OK;
+    plk_status st;
+    bool fused = true;
+    std::vector<Instance*> ins(nitems);
+    for (int i = 0; i < nitems; i++) {
+        Instance* in = get_instance(ctx, items[i].instance);
+        if ((st = ready(in)) != PLK_OK) return st;
+        if ((st = check_ops(in, items[i].ops, items[i].nops)) != PLK_OK) return st;
+        if ((st = check_end(in, items[i].node, false)) != PLK_OK) return st;
+        if ((st = check_end(in, items[i].dad, true)) != PLK_OK) return st;
+        if (!(items[i].length >= 0.0) || !std::isfinite(items[i].length)) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
+        for (int j = 0; j < i; j++)
+            if (items[j].instance == items[i].instance) return fail(PLK_ERR_ARG, "instance %d appears twice in a batch", items[i].instance);
+        ins[i] = in;
+        if (in->d.ncat != 4 || (in->d.nstates != 4 && in->d.nstates != 20)) fused = false;
+    }
+    if (!fused) {
+        // shapes outside the fused kernel: one launch per item
+        for (int i = 0; i < nitems; i++)
+            if ((st = plk_likelihood_branch(ctx, items[i].instance, items[i].ops, items[i].nops,
+                                            items[i].node, items[i].dad, items[i].length, &lnl[i],
+                                            nullptr)) != PLK_OK && st != PLK_ERR_NONFINITE)
+                return st;
+        return PLK_OK;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    // largest partitions first (PhyloSuperTree::computePartitionOrder)
+    std::vector<int> order(nitems);
+    std::vector<double> cost(nitems);
+    for (int i = 0; i < nitems; i++) {
+        order[i] = i;
+        const double n = ins[i]->d.nstates;
+        cost[i] = (double)ins[i]->d.npatterns * (items[i].nops + 1) * n * n;
+    }
+    // grouped by instance class (so each class's tiles are contiguous for a
+    // per-class launch), largest partitions first inside a class
+    std::vector<int> cls(nitems);
+    for (int i = 0; i < nitems; i++)
+        cls[i] = batch_class(ins[i]->d.nstates, ins[i]->d.precision == PLK_PREC_F32);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (cls[a] != cls[b]) return cls[a] > cls[b];
+        return cost[a] > cost[b];
+    });
+    std::vector<BatchItem> hb(nitems);
+    std::vector<PrepJob> jobs;
+    std::vector<plk_op> ops;
+    std::vector<int> tile_item;
+    std::vector<double> lens(nitems);
+    size_t tab_bytes = 0, root_doubles = 0, smem = 0, smem_prep = 0;
+    std::vector<size_t> tab_off(nitems), root_off(nitems), op_off(nitems);
+    double bytes = 0.0;
+    for (int r = 0; r < nitems; r++) {
+        const int i = order[r];
+        Instance* in = ins[i];
+        op_off[r] = ops.size();
+        ops.insert(ops.end(), items[i].ops, items[i].ops + items[i].nops);
+        tab_off[r] = tab_bytes;
+        tab_bytes += round_up((size_t)items[i].nops * 2 * in->table_size * in->elem, 256);
+        root_off[r] = root_doubles;
+        root_doubles += (size_t)(in->d.state_unknown + 2) * in->block;
+        for (int o = 0; o < items[i].nops; o++) jobs.push_back(PrepJob{r, o});
+        const int PB = kThreads / in->d.ncat;
+        const int ntiles = (in->d.npatterns + PB - 1) / PB;
+        BatchItem& b = hb[r];
+        b.I = inst_dev(in);
+        b.m = model_dev(in);
+        b.nops = items[i].nops;
+        b.table_size = in->table_size;
+        b.tile_begin = (int)tile_item.size();
+        b.ntiles = ntiles;
+        b.node = items[i].node;
+        b.dad = items[i].dad;
+        b.n = in->d.nstates;
+        b.C = in->d.ncat;
+        b.f32 = in->d.precision == PLK_PREC_F32;
+        for (int t = 0; t < ntiles; t++) tile_item.push_back(r);
+        lens[r] = items[i].length;
+        const size_t s1 = round_up((size_t)b.n * b.n * in->elem, 16) +
+                          std::max(2 * (size_t)in->table_size * in->elem,
+                                   (size_t)(in->d.state_unknown + 2) * in->block * 8);
+        smem = std::max(smem, s1);
+        smem_prep = std::max(smem_prep, (size_t)in->block * 8);
+        double ob = 0.0, of = 0.0;
+        for (int o = 0; o < items[i].nops; o++) {
+            double bb, ff;
+            op_cost(in, items[i].ops[o], bb, ff);
+            ob += bb;
+        }
+        const int nint = 1 + (items[i].dad.is_tip ? 0 : 1);
+        ob += (double)in->block * in->elem * nint + (items[i].dad.is_tip ? 1 : 0) + in->scale_elem * nint + 24.0;
+        bytes += ob * in->d.npatterns;
+        (void)of;
+    }
+    const int total_tiles = (int)tile_item.size();
+    if ((st = grow(ctx, nullptr, &ctx->bt_items, ctx->bt_items_cap, nitems)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_jobs, ctx->bt_jobs_cap, std::max<int>(1, jobs.size()))) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_ops, ctx->bt_ops_cap, std::max<int>(1, ops.size()))) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_tables, ctx->bt_tables_cap, (int)std::max<size_t>(256, tab_bytes))) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_roots, ctx->bt_roots_cap, (int)root_doubles)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_tile_item, ctx->bt_tile_cap, total_tiles)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_tile_sums, ctx->bt_tile_sums_cap, total_tiles)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_vals, ctx->bt_vals_cap, 2 * nitems)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_counter, ctx->bt_counter_cap, 64)) != PLK_OK) return st;
+    for (int r = 0; r < nitems; r++) {
+        hb[r].ops = ctx->bt_ops + op_off[r];
+        hb[r].tables = ctx->bt_tables + tab_off[r];
+        hb[r].root = ctx->bt_roots + root_off[r];
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->bt_items, hb.data(), sizeof(BatchItem) * nitems, hipMemcpyHostToDevice, ctx->stream));
+    if (!jobs.empty())
+        HIP_TRY(hipMemcpyAsync(ctx->bt_jobs, jobs.data(), sizeof(PrepJob) * jobs.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (!ops.empty())
+        HIP_TRY(hipMemcpyAsync(ctx->bt_ops, ops.data(), sizeof(plk_op) * ops.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->bt_tile_item, tile_item.data(), sizeof(int) * total_tiles, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->bt_vals, lens.data(), sizeof(double) * nitems, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned), ctx->stream));
+    int pe = prof_begin(ctx);
+    hipLaunchKernelGGL(k_prep_batch, dim3((int)jobs.size() + nitems), dim3(kThreads), smem_prep, ctx->stream,
+                       ctx->bt_items, ctx->bt_jobs, (int)jobs.size(), nitems, ctx->bt_vals, ctx->bt_roots);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_PREP_BATCH, (double)tab_bytes, 0.0);
+    // class ranges of the tile list
+    int mix = 0;
+    std::vector<std::pair<int, int>> ranges;   // (class, end tile)
+    for (int r = 0; r < nitems; r++) {
+        const int c = batch_class(hb[r].n, hb[r].f32);
+        mix |= c;
+        const int end = hb[r].tile_begin + hb[r].ntiles;
+        if (ranges.empty() || ranges.back().first != c) ranges.push_back({c, end});
+        else ranges.back().second = end;
+    }
+    auto launch = [&](int split, int bpc) -> plk_status {
+        HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned) * 16, ctx->stream));
+        auto one = [&](int mask, int begin, int end, int k) -> plk_status {
+            const void* fn;
+            switch (mask) {
+            case BC_DNA64: fn = (const void*)k_batch<BC_DNA64>; break;
+            case BC_PROT64: fn = (const void*)k_batch<BC_PROT64>; break;
+            case BC_DNA32: fn = (const void*)k_batch<BC_DNA32>; break;
+            case BC_PROT32: fn = (const void*)k_batch<BC_PROT32>; break;
+            default: fn = (const void*)k_batch<BC_ALL>; break;
+            }
+            int nb = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, smem));
+            const int grid = std::max(1, std::min(end - begin, ctx->num_cu * std::max(1, std::min(nb, bpc))));
+            const BatchItem* a0 = ctx->bt_items;
+            const int* a1 = ctx->bt_tile_item;
+            unsigned* a4 = ctx->bt_counter + k;
+            double* a5 = ctx->bt_tile_sums;
+            void* args[] = {(void*)&a0, (void*)&a1, (void*)&begin, (void*)&end, (void*)&a4, (void*)&a5};
+            HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, smem, ctx->stream));
+            return PLK_OK;
+        };
+        if (!split || ranges.size() == 1) {
+            const int mask = ranges.size() == 1 ? ranges[0].first : BC_ALL;
+            return one(mask, 0, total_tiles, 0);
+        }
+        int begin = 0, k = 0;
+        for (auto& rg : ranges) {
+            plk_status s2 = one(rg.first, begin, rg.second, k++);
+            if (s2 != PLK_OK) return s2;
+            begin = rg.second;
+        }
+        return PLK_OK;
+    };
+    int lg = 0;
+    while ((1 << lg) < total_tiles) lg++;
+    const long key = (long)mix * 64 + lg;
+    int split = 1, bpc = ctx->batch_blocks_per_cu > 0 ? ctx->batch_blocks_per_cu : 2;
+    auto tuned = ctx->batch_tune.find(key);
+    if (ctx->batch_blocks_per_cu == 0 && tuned != ctx->batch_tune.end()) {
+        split = tuned->second.first;
+        bpc = tuned->second.second;
+    } else if (ctx->batch_blocks_per_cu == 0 && ctx->autotune) {
+        // time every candidate once on this batch (results are identical)
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        float best = 1e30f;
+        const int splits[2] = {0, 1}, bpcs[3] = {1, 2, 4};
+        for (int sp : splits) {
+            if (sp == 1 && ranges.size() == 1) continue;
+            for (int b : bpcs) {
+                HIP_TRY(hipEventRecord(e0, ctx->stream));
+                if ((st = launch(sp, b)) != PLK_OK) return st;
+                HIP_TRY(hipEventRecord(e1, ctx->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; split = sp; bpc = b; }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->batch_tune[key] = {split, bpc};
+    }
+    pe = prof_begin(ctx);
+    if ((st = launch(split, bpc)) != PLK_OK) return st;
+    prof_end(ctx, pe, K_BATCH, bytes, 0.0);
+    hipLaunchKernelGGL(k_batch_reduce, dim3((nitems + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       ctx->stream, ctx->bt_items, nitems, ctx->bt_tile_sums, ctx->bt_vals + nitems);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> out(nitems);
+    HIP_TRY(hipMemcpyAsync(out.data(), ctx->bt_vals + nitems, sizeof(double) * nitems, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    bool nonfinite = false;
+    for (int r = 0; r < nitems; r++) {
+        const int i = order[r];
+        ins[i]->ptn_lh_ready = true;
+        ins[i]->theta_ready = ins[i]->theta_ready;   // theta untouched
+        double v = out[r];
+        if (!std::isfinite(v)) {
+            if ((st = fixup_nonfinite(ctx, ins[i], &v, nullptr)) != PLK_OK) return st;
+            nonfinite = true;
+        }
+        lnl[i] = v;
+    }
+    if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch in a batch item; fix-up applied");
+    return PLK_OK;
+}
 
 plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops, int nops,
                               const plk_sweep_branch* branches, int nbranches, double* lengths,

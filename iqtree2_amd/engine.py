@@ -19,6 +19,7 @@ from ._capi import (OP_DTYPE, PREC_F32, PREC_F64, PlkConfig, PlkEnd, PlkError,
                     u8ptr)
 
 __all__ = ["Context", "Instance", "PhyloTree", "Workload", "Patterns", "end", "ops_array",
+           "likelihood_branch_batch",
            "PREC_F32", "PREC_F64", "PlkError"]
 
 
@@ -289,6 +290,26 @@ class Instance:
         check(lib().plk_read_partial(self.ctx.handle, self.id, int(slot), dptr(part),
                                      i32ptr(scale)), "plk_read_partial")
         return part, scale
+
+
+def likelihood_branch_batch(ctx, jobs):
+    """jobs: list of (instance, ops, node, dad, length); one fused launch
+    (plk_likelihood_branch_batch). Returns the lnL of every job."""
+    items = (_capi.PlkBatchItem * len(jobs))()
+    keep = []
+    for k, (inst, ops, node, dad, length) in enumerate(jobs):
+        arr = np.ascontiguousarray(ops, dtype=OP_DTYPE) if ops is not None and len(ops) else None
+        keep.append(arr)
+        items[k].instance = inst.id
+        items[k].ops = arr.ctypes.data_as(ctypes.c_void_p) if arr is not None else None
+        items[k].nops = 0 if arr is None else len(arr)
+        items[k].node = node
+        items[k].dad = dad
+        items[k].length = float(length)
+    out = np.zeros(len(jobs))
+    check(lib().plk_likelihood_branch_batch(ctx.handle, items, len(jobs), dptr(out)),
+          "plk_likelihood_branch_batch")
+    return out
 
 
 class PhyloTree:

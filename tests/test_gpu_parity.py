@@ -223,3 +223,51 @@ def test_device_newton_sweep_matches_host_newton(ctx, seq_type, precision, safe,
     # the tree lnL at the new lengths, recomputed from scratch
     dev.clear_all_partial()
     assert abs(dev.compute_likelihood() - ld) <= 1e-8 * abs(ld)
+
+
+def partition_set(n_parts, ntaxa=24, seed=7, f32_every=0):
+    """Partitions of mixed data type and length on one shared tree."""
+    base = ia.Workload(seq_type=ia.DNA, ntaxa=ntaxa, nsite=10, seed=seed)
+    edges = base.edges()
+    rng = np.random.default_rng(seed)
+    parts = []
+    for k in range(n_parts):
+        st = ia.PROTEIN if k % 3 == 2 else ia.DNA
+        nsite = int(rng.integers(60, 1500))
+        w = ia.Workload(seq_type=st, ntaxa=ntaxa, nsite=nsite, ncat=4, alpha=-1.0,
+                        p_inv=0.1 if st == ia.DNA and k % 2 else 0.0,
+                        gap_frac=0.02 if st == ia.DNA else 0.0, seed=1000 + k, edges=edges)
+        prec = ia.PREC_F32 if f32_every and k % f32_every == 0 else ia.PREC_F64
+        parts.append((w, prec))
+    return parts
+
+
+def test_partition_batch_equals_per_partition(ctx):
+    """plk_likelihood_branch_batch over mixed DNA/protein, FP64/FP32
+    partitions equals one plk_likelihood_branch per partition, is
+    reproducible, and every autotuned launch shape gives the same result."""
+    parts = partition_set(40, f32_every=5)
+    jobs, single = [], []
+    for k, (w, prec) in enumerate(parts):
+        planner = ia.PhyloTree(None, w)
+        branches = planner.branches()
+        a, b = branches[k % len(branches)]
+        ops, node, dad, length = planner.plan(a, b)
+        inst = ia.Instance.from_workload(ctx, w, precision=prec)
+        jobs.append((inst, ops, node, dad, length))
+    for inst, ops, node, dad, length in jobs:
+        single.append(inst.likelihood_branch(ops, node, dad, length))
+    first = ia.likelihood_branch_batch(ctx, jobs)      # includes autotuning trials
+    second = ia.likelihood_branch_batch(ctx, jobs)     # tuned shape
+    np.testing.assert_array_equal(first, second)
+    np.testing.assert_allclose(first, np.array(single), rtol=1e-12, atol=0)
+    # against the oracle for the FP64 partitions
+    for k, (w, prec) in enumerate(parts[:8]):
+        if prec != ia.PREC_F64:
+            continue
+        inst, ops, node, dad, length = jobs[k]
+        orc = OracleTree.from_workload(w)
+        lo = orc.likelihood_branch(ops, node, dad, length)
+        assert abs(first[k] - lo) <= LNL_RTOL * abs(lo)
+    for inst, *_ in jobs:
+        inst.close()

@@ -17,6 +17,7 @@
 #include "plk_kernels.h"
 #include "plk_sweep.h"
 #include "plk_batch.h"
+#include "plk_codon.h"
 
 using namespace plkdev;
 
@@ -107,10 +108,12 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 }  // namespace
 
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
-                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH, K_NKIND };
+                K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH,
+                K_TRAVERSE_MFMA, K_ROOT, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
-                                          "reduce", "rell", "newton_sweep", "batch", "prep_batch"};
+                                          "reduce", "rell", "newton_sweep", "batch", "prep_batch",
+                                          "traverse_mfma", "root"};
 
 struct ProfRec {
     int kind;
@@ -346,9 +349,59 @@ plk_status launch_traverse_p(plk_context* ctx, Instance* in, int nops, int mode,
     return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
 }
 
+// branch-end bytes per pattern of the root stage (lnL or theta)
+double root_bytes(const Instance* in, int mode, plk_end dad) {
+    const double s = (double)in->elem, blk = (double)in->block;
+    const double ncnt = in->d.safe_numeric ? in->d.ncat : 1;
+    const int nint = 1 + (dad.is_tip ? 0 : 1);
+    double b = blk * s * nint + (dad.is_tip ? 1 : 0) + in->scale_elem * ncnt * nint;
+    b += mode == MODE_BRANCH ? 24.0 : blk * s + 8.0;
+    return b;
+}
+
+template <typename T, int C>
+plk_status launch_root_stream(plk_context* ctx, Instance* in, int mode, plk_end node, plk_end dad) {
+    const int grid = in->Ppad / (kThreads / C);
+    const int pe = prof_begin(ctx);
+    hipLaunchKernelGGL((k_root_stream<T, kCodonN, C>), dim3(grid), dim3(kThreads), 0, ctx->stream,
+                       inst_dev(in), model_dev(in), mode, node, dad, in->root_tab, in->block_sums);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_ROOT, root_bytes(in, mode, dad) * in->d.npatterns,
+             (mode == MODE_BRANCH ? 3.0 : 1.0) * in->block * in->d.npatterns);
+    return PLK_OK;
+}
+
+// 61 states: the MFMA traversal kernel, then the streaming root stage
+template <typename T>
+plk_status launch_traverse_codon(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                                 plk_end dad) {
+    if (nops > 0) {
+        const int grid = in->Ppad / kCodonTile;
+        const int pe = prof_begin(ctx);
+        hipLaunchKernelGGL(k_traverse_codon<T>, dim3(grid), dim3(kThreads), 0, ctx->stream,
+                           inst_dev(in), model_dev(in), in->ops, nops,
+                           reinterpret_cast<const T*>(in->tables), in->table_size);
+        HIP_TRY(hipGetLastError());
+        prof_end(ctx, pe, K_TRAVERSE_MFMA, in->staged_bytes * in->d.npatterns,
+                 in->staged_flops * in->d.npatterns);
+    }
+    if (mode == MODE_PARTIALS) return PLK_OK;
+    switch (in->d.ncat) {
+    case 1: return launch_root_stream<T, 1>(ctx, in, mode, node, dad);
+    case 2: return launch_root_stream<T, 2>(ctx, in, mode, node, dad);
+    case 4: return launch_root_stream<T, 4>(ctx, in, mode, node, dad);
+    case 8: return launch_root_stream<T, 8>(ctx, in, mode, node, dad);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+}
+
 plk_status launch_traverse(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
                            plk_end dad) {
-    if (in->d.precision == PLK_PREC_F32) return launch_traverse_p<float>(ctx, in, nops, mode, node, dad);
+    const bool f32 = in->d.precision == PLK_PREC_F32;
+    if (in->d.nstates == kCodonN)
+        return f32 ? launch_traverse_codon<float>(ctx, in, nops, mode, node, dad)
+                   : launch_traverse_codon<double>(ctx, in, nops, mode, node, dad);
+    if (f32) return launch_traverse_p<float>(ctx, in, nops, mode, node, dad);
     return launch_traverse_p<double>(ctx, in, nops, mode, node, dad);
 }
 
@@ -393,6 +446,8 @@ plk_status launch_buffer(plk_context* ctx, Instance* in, double length, int mode
                        : launch_buffer_n<double, 4>(ctx, in, length, mode);
     case 20: return f32 ? launch_buffer_n<float, 20>(ctx, in, length, mode)
                         : launch_buffer_n<double, 20>(ctx, in, length, mode);
+    case kCodonN: return f32 ? launch_buffer_n<float, kCodonN>(ctx, in, length, mode)
+                             : launch_buffer_n<double, kCodonN>(ctx, in, length, mode);
     }
     return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
 }
@@ -558,8 +613,12 @@ plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* d, int
     if (d->ntaxa < 3 || d->npatterns <= 0 || d->nslots <= 0 || d->nstates < 2 || d->ncat < 1 ||
         d->state_unknown < d->nstates || d->state_unknown > 255)
         return fail(PLK_ERR_ARG, "invalid instance shape");
-    if (d->nstates != 4 && d->nstates != 20)
+    if (d->nstates != 4 && d->nstates != 20 && d->nstates != kCodonN)
         return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built into this library", d->nstates);
+    // IQ-TREE forces per-category scaling for every other state count
+    // (tree/phylotreesse.cpp:95-96); the 61-state kernel implements only that
+    if (d->nstates == kCodonN && !d->safe_numeric)
+        return fail(PLK_ERR_UNSUPPORTED, "61-state models need safe_numeric = 1");
     if (d->ncat != 1 && d->ncat != 2 && d->ncat != 4 && d->ncat != 8)
         return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built into this library", d->ncat);
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -938,36 +997,13 @@ plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid)
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
-                                       double* lnl) {
-    if (!ctx || nitems < 0 || (nitems > 0 && (!items || !lnl))) return fail(PLK_ERR_ARG, "invalid batch");
-    if (nitems == 0) return PLK_OK;
+// One persistent work-list launch over items whose shape k_batch is built
+// for (4 or 20 states, 4 categories); arguments already validated.
+plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, int nitems, Instance* const* ins,
+                       double* lnl) {
     plk_status st;
-    bool fused = true;
-    std::vector<Instance*> ins(nitems);
-    for (int i = 0; i < nitems; i++) {
-        Instance* in = get_instance(ctx, items[i].instance);
-        if ((st = ready(in)) != PLK_OK) return st;
-        if ((st = check_ops(in, items[i].ops, items[i].nops)) != PLK_OK) return st;
-        if ((st = check_end(in, items[i].node, false)) != PLK_OK) return st;
-        if ((st = check_end(in, items[i].dad, true)) != PLK_OK) return st;
-        if (!(items[i].length >= 0.0) || !std::isfinite(items[i].length)) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
-        for (int j = 0; j < i; j++)
-            if (items[j].instance == items[i].instance) return fail(PLK_ERR_ARG, "instance %d appears twice in a batch", items[i].instance);
-        ins[i] = in;
-        if (in->d.ncat != 4 || (in->d.nstates != 4 && in->d.nstates != 20)) fused = false;
-    }
-    if (!fused) {
-        // shapes outside the fused kernel: one launch per item
-        for (int i = 0; i < nitems; i++)
-            if ((st = plk_likelihood_branch(ctx, items[i].instance, items[i].ops, items[i].nops,
-                                            items[i].node, items[i].dad, items[i].length, &lnl[i],
-                                            nullptr)) != PLK_OK && st != PLK_ERR_NONFINITE)
-                return st;
-        return PLK_OK;
-    }
     std::lock_guard<std::mutex> lock(ctx->mu);
     // largest partitions first (PhyloSuperTree::computePartitionOrder)
     std::vector<int> order(nitems);
@@ -1152,7 +1188,6 @@ plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* i
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];
         ins[i]->ptn_lh_ready = true;
-        ins[i]->theta_ready = ins[i]->theta_ready;   // theta untouched
         double v = out[r];
         if (!std::isfinite(v)) {
             if ((st = fixup_nonfinite(ctx, ins[i], &v, nullptr)) != PLK_OK) return st;
@@ -1163,6 +1198,58 @@ plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* i
     if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch in a batch item; fix-up applied");
     return PLK_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
+                                       double* lnl) {
+    if (!ctx || nitems < 0 || (nitems > 0 && (!items || !lnl))) return fail(PLK_ERR_ARG, "invalid batch");
+    if (nitems == 0) return PLK_OK;
+    plk_status st;
+    std::vector<Instance*> ins(nitems);
+    for (int i = 0; i < nitems; i++) {
+        Instance* in = get_instance(ctx, items[i].instance);
+        if ((st = ready(in)) != PLK_OK) return st;
+        if ((st = check_ops(in, items[i].ops, items[i].nops)) != PLK_OK) return st;
+        if ((st = check_end(in, items[i].node, false)) != PLK_OK) return st;
+        if ((st = check_end(in, items[i].dad, true)) != PLK_OK) return st;
+        if (!(items[i].length >= 0.0) || !std::isfinite(items[i].length)) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
+        for (int j = 0; j < i; j++)
+            if (items[j].instance == items[i].instance) return fail(PLK_ERR_ARG, "instance %d appears twice in a batch", items[i].instance);
+        ins[i] = in;
+    }
+    // shapes the fused kernel is built for go into one launch; the rest
+    // (61 states, category counts other than 4) run one traversal each
+    std::vector<plk_batch_item> fused;
+    std::vector<Instance*> fused_ins;
+    std::vector<int> fused_at;
+    bool nonfinite = false;
+    for (int i = 0; i < nitems; i++) {
+        const Instance* in = ins[i];
+        if (in->d.ncat == 4 && (in->d.nstates == 4 || in->d.nstates == 20)) {
+            fused.push_back(items[i]);
+            fused_ins.push_back(ins[i]);
+            fused_at.push_back(i);
+            continue;
+        }
+        st = plk_likelihood_branch(ctx, items[i].instance, items[i].ops, items[i].nops, items[i].node,
+                                   items[i].dad, items[i].length, &lnl[i], nullptr);
+        if (st == PLK_ERR_NONFINITE) nonfinite = true;
+        else if (st != PLK_OK) return st;
+    }
+    if (!fused.empty()) {
+        std::vector<double> out(fused.size());
+        st = batch_fused(ctx, fused.data(), (int)fused.size(), fused_ins.data(), out.data());
+        if (st != PLK_OK && st != PLK_ERR_NONFINITE) return st;
+        if (st == PLK_ERR_NONFINITE) nonfinite = true;
+        for (size_t k = 0; k < fused.size(); k++) lnl[fused_at[k]] = out[k];
+    }
+    if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch in a batch item; fix-up applied");
+    return PLK_OK;
+}
+
 
 plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops, int nops,
                               const plk_sweep_branch* branches, int nbranches, double* lengths,

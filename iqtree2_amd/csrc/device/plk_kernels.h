@@ -468,15 +468,22 @@ k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ bl
         val[2 * N * C + t] = cof * v1;
     }
     __syncthreads();
+    // With an odd state count and an even block, the reference's derivative
+    // dot product (dotProductTriple, tree/phylokernelnew.h:668-716, called
+    // with N = block at 2489-2491) adds the last (category, state) term twice
+    // to lh, d1 and d2; the derivatives reproduce that for identical Newton
+    // steps, while the lnL returned beside them uses the plain sum.
+    constexpr bool kTwiceLast = (N & 1) && !((N * C) & 1);
     const int c = threadIdx.x % C;
     const int p = blockIdx.x * PB + threadIdx.x / C;
-    double lh = 0.0, d1 = 0.0, d2 = 0.0;
+    double lh = 0.0, d1 = 0.0, d2 = 0.0, lhq = 0.0;
     if (p < I.Ppad) {
         const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
         const double* v0 = val + c * N;
         const double* v1 = val + N * C + c * N;
         const double* v2 = val + 2 * N * C + c * N;
-#pragma unroll
+        constexpr int kUnroll = N <= 20 ? N : 4;   // 61 states: bounded register use
+#pragma unroll kUnroll
         for (int i = 0; i < N; i++) {
             const double t = (double)th[i];
             lh = fma(v0[i], t, lh);
@@ -485,11 +492,19 @@ k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ bl
                 d2 = fma(v2[i], t, d2);
             }
         }
+        lhq = lh;
+        if (kTwiceLast && mode == BUF_DERV && c == C - 1) {
+            const double t = (double)th[N - 1];
+            lhq = fma(v0[N - 1], t, lh);
+            d1 = fma(v1[N - 1], t, d1);
+            d2 = fma(v2[N - 1], t, d2);
+        }
     }
     lh = group_sum_ordered<C>(lh);
     if (mode == BUF_DERV) {
         d1 = group_sum_ordered<C>(d1);
         d2 = group_sum_ordered<C>(d2);
+        lhq = kTwiceLast ? group_sum_ordered<C>(lhq) : lh;
     }
     double acc[3] = {0.0, 0.0, 0.0};
     if (c == 0 && p < I.P) {
@@ -497,7 +512,7 @@ k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ bl
         const double f = I.freq[p];
         const double lnp = log(lh) + I.buffer_scale[p];
         if (mode == BUF_DERV) {
-            const double inv = 1.0 / lh;
+            const double inv = 1.0 / (kTwiceLast ? fabs(lhq) + I.invar[p] : lh);
             const double df_frac = d1 * inv, ddf_frac = d2 * inv;
             const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
             acc[0] = tmp1;

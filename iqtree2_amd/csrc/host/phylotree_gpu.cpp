@@ -10,8 +10,9 @@ PhyloTreeGPU::PhyloTreeGPU(plk_context* ctx, const Workload& w, int precision, i
     computed_.assign((size_t)tree_.nnodes * 3, 0);
     owner_.assign(tree_.nnodes, -1);
     size_.assign((size_t)tree_.nnodes * 3, 0);
-    if (!ctx_) return;
     const ModelParams& m = w.model;
+    nstates_ = m.nstates;
+    if (!ctx_) return;
     plk_instance_desc d{};
     d.ntaxa = tree_.ntaxa;
     d.nstates = m.nstates;
@@ -20,7 +21,11 @@ PhyloTreeGPU::PhyloTreeGPU(plk_context* ctx, const Workload& w, int precision, i
     d.nslots = num_slots();
     d.state_unknown = seq_state_unknown(m.type);
     d.precision = precision;
-    d.safe_numeric = safe_numeric;
+    // PhyloTree::setLikelihoodKernel (tree/phylotreesse.cpp:94-95): per-category
+    // scaling on request, from 2000 taxa (utils/tools.cpp:1262), and for every
+    // state count other than 4 and 20
+    d.safe_numeric = safe_numeric || tree_.ntaxa >= 2000 || (m.nstates != 4 && m.nstates != 20);
+    safe_ = d.safe_numeric;
     status_ = plk_instance_create(ctx_, &d, &inst_);
     if (status_ != PLK_OK) return;
     status_ = plk_set_patterns(ctx_, inst_, w.patterns.states.data(), w.freq.data(), w.invar.data());
@@ -275,7 +280,9 @@ int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, i
 int PhyloTreeGPU::optimize_all_branches(int iterations, double tolerance, int max_nr, double* lnl,
                                         int* n_derv) {
     if (!ctx_) return PLK_ERR_STATE;
-    if (!device_newton) return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
+    // the device sweep is built for 4 and 20 states; codon models take the host loop
+    if (!device_newton || (nstates_ != 4 && nstates_ != 20))
+        return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
     std::vector<int> nodes, nodes2;
     best_traversal(nodes, nodes2);
     double tree_lh;

@@ -59,6 +59,8 @@ private:
     plk_context* ctx_ = nullptr;
     int inst_ = -1;
     int status_ = 0;
+    int nstates_ = 0;
+    bool safe_ = false;
     std::vector<char> computed_;   // [nnodes*3]: partial of v seen from adj[v][k]
     std::vector<int> owner_;       // [nnodes]: orientation holding v's buffer
     std::vector<int> size_;        // [nnodes*3] leaves below v away from adj[v][k]; 0 = unknown

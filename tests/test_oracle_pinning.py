@@ -26,12 +26,14 @@ def oracle_branch(w, planner, orc, a, b):
     (ia.DNA, 5, 300, 0.2, 0.05),
     (ia.DNA, 6, 200, 0.0, 0.0),
     (ia.PROTEIN, 4, 150, 0.1, 0.0),
+    (ia.CODON, 4, 80, 0.0, 0.0),
 ])
 def test_brute_force_small_trees(seq_type, ntaxa, nsite, p_inv, gap):
     w = ia.Workload(seq_type=seq_type, ntaxa=ntaxa, nsite=nsite, ncat=4, mean_len=0.3,
                     alpha=0.7, p_inv=p_inv, gap_frac=gap, seed=11 + ntaxa)
     planner = ia.PhyloTree(None, w)
-    orc = OracleTree.from_workload(w)
+    # codon models run with per-category scaling, as IQ-TREE forces for them
+    orc = OracleTree.from_workload(w, safe_numeric=seq_type == ia.CODON)
     ref, ref_p = brute_force_lnl(w.ntaxa, w.edges(), w.get("states"), w.get("Q"), w.get("pi"),
                                  w.get("rates"), w.get("props"), w.get("invar"), w.get("freq"),
                                  indicators(seq_type, w.nstates, w.state_unknown))
@@ -44,12 +46,13 @@ def test_brute_force_small_trees(seq_type, ntaxa, nsite, p_inv, gap):
 @pytest.mark.parametrize("seq_type,ntaxa,nsite,mean_len,p_inv", [
     (ia.DNA, 40, 2000, 0.1, 0.2),
     (ia.PROTEIN, 24, 600, 0.2, 0.0),
+    (ia.CODON, 16, 300, 0.15, 0.1),
 ])
 def test_expm_pruning(seq_type, ntaxa, nsite, mean_len, p_inv):
     w = ia.Workload(seq_type=seq_type, ntaxa=ntaxa, nsite=nsite, ncat=4, mean_len=mean_len,
                     alpha=0.5, p_inv=p_inv, gap_frac=0.02 if seq_type == ia.DNA else 0.0, seed=7)
     planner = ia.PhyloTree(None, w)
-    orc = OracleTree.from_workload(w, nthreads=4)
+    orc = OracleTree.from_workload(w, safe_numeric=seq_type == ia.CODON, nthreads=4)
     pr = pruner_for(w)
     branches = planner.branches()
     for a, b in branches[:: max(1, len(branches) // 6)]:
@@ -94,6 +97,48 @@ def test_derivatives_analytic():
             assert abs(lnl_b - ref_lnl) <= 1e-10 * abs(ref_lnl)
             assert abs(df - ref_df) <= 1e-8 * max(1.0, abs(ref_df))
             assert abs(ddf - ref_ddf) <= 1e-8 * max(1.0, abs(ref_ddf))
+
+
+@pytest.mark.parametrize("ncat", [1, 3, 4])
+def test_codon_derivatives_and_odd_state_quirk(ncat):
+    """61 states: with an odd block (ncat odd) the oracle's derivatives are
+    the analytic ones. With an even block the reference's dotProductTriple
+    (tree/phylokernelnew.h:668-716 called with N = block) counts the last
+    (category, state) term of lh, d1 and d2 twice; the oracle must match the
+    analytic derivatives with exactly that term added, computed here from
+    real-basis partials and the last row of U^-1."""
+    w = ia.Workload(seq_type=ia.CODON, ntaxa=6, nsite=200, ncat=ncat, mean_len=0.1, alpha=0.5,
+                    p_inv=0.1, seed=5)
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w, safe_numeric=True)
+    pr = pruner_for(w)
+    ev, inv = w.get("eval"), w.get("inv_evec")
+    rates, props, freq, invar = w.get("rates"), w.get("props"), w.get("freq"), w.get("invar")
+    block_even = (w.nstates * ncat) % 2 == 0
+    for a, b in planner.branches()[::3]:
+        ops, node, dad, length = planner.plan(a, b)
+        orc.likelihood_buffer(ops, node, dad)
+        for t in (length, 2.5 * length + 1e-3):
+            df, ddf = orc.likelihood_derv(t)
+            A, sa = pr.side(a, b)
+            B, sb = pr.side(b, a)
+            Pt = pr.P(t)
+            rQ = rates[:, None, None] * w.get("Q")[None]
+            P1 = np.einsum("cxz,czy->cxy", rQ, Pt)
+            P2 = np.einsum("cxz,czy->cxy", rQ, P1)
+            pi = w.get("pi")
+            sc = np.exp(sa + sb)
+            s = [np.einsum("c,x,cpx,cxy,cpy->p", props, pi, A, M, B) * sc for M in (Pt, P1, P2)]
+            if block_even:
+                c = ncat - 1
+                cof = ev[-1] * rates[c]
+                x = props[c] * np.exp(cof * t) * (A[c] @ inv[-1]) * (B[c] @ inv[-1]) * sc
+                s = [s[0] + x, s[1] + cof * x, s[2] + cof * cof * x]
+            lh = np.abs(s[0]) + invar
+            ref_df = float(np.dot(freq, s[1] / lh))
+            ref_ddf = float(np.dot(freq, s[2] / lh - (s[1] / lh) ** 2))
+            assert abs(df - ref_df) <= 1e-8 * max(1.0, abs(ref_df)), (ncat, a, b, t, df, ref_df)
+            assert abs(ddf - ref_ddf) <= 1e-8 * max(1.0, abs(ref_ddf)), (ncat, a, b, t, ddf, ref_ddf)
 
 
 def test_newton_reaches_stationary_point():

@@ -161,11 +161,24 @@ typedef struct {
 
 /* PhyloSuperTree::computeLikelihood's partition loop
  * (tree/phylosupertree.cpp:701-735) as one persistent launch: every item's
- * traversal and branch lnL, instances of 4 and 20 states and either precision
- * together, pattern tiles handed out largest-partition first. lnl[i] gets
- * item i's tree lnL, identical to plk_likelihood_branch on that item alone. */
+ * traversal and branch lnL, instances of 4 and 20 states with 4 categories
+ * and either precision together, pattern tiles handed out largest-partition
+ * first; items of other shapes (61 states, other category counts) run one
+ * traversal each in the same call. lnl[i] gets item i's tree lnL, equal to
+ * plk_likelihood_branch on that item alone to rounding (1e-12 relative). */
 plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
                                        double* lnl);
+
+/* computeLikelihoodDerv (tree/phylotreesse.cpp:220-224 ->
+ * computeLikelihoodDervSIMD, tree/phylokernelnew.h:2233-2633) for n (instance,
+ * length) pairs in one launch per shape, each on its instance's current
+ * theta buffer; an instance may appear more than once. df/ddf/lnl[i] are
+ * bit-identical to plk_likelihood_derv(instances[i], lengths[i]). Used to
+ * advance the Newton iterations of many requests together. An item whose
+ * derivative is not finite gets df = ddf = 0 and lnl = NaN, the others are
+ * valid, and the call returns PLK_ERR_NONFINITE. */
+plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
+                                     int n, double* df, double* ddf, double* lnl);
 
 /* ---- device-resident branch optimisation ------------------------------- */
 /* A traversal step whose children carry edge ids instead of lengths: the

@@ -60,6 +60,35 @@ class PlkhWorkloadSpec(ctypes.Structure):
                 ("gap_frac", c_double), ("seed", ctypes.c_uint64)]
 
 
+class PlksConfig(ctypes.Structure):
+    _fields_ = [("max_inflight", c_int), ("hbm_budget", c_size_t)]
+
+
+class PlksRequest(ctypes.Structure):
+    _fields_ = [("desc", PlkInstanceDesc), ("states", c_void_p), ("freq", c_void_p),
+                ("invar", c_void_p), ("eval", c_void_p), ("evec", c_void_p),
+                ("inv_evec", c_void_p), ("tip_eigen", c_void_p), ("cat_rates", c_void_p),
+                ("cat_props", c_void_p), ("ops", c_void_p), ("nops", c_int), ("node", PlkEnd),
+                ("dad", PlkEnd), ("length", c_double), ("kind", c_int), ("min_len", c_double),
+                ("max_len", c_double), ("max_nr", c_int), ("weights", c_void_p),
+                ("nboot", c_int)]
+
+
+class PlksResult(ctypes.Structure):
+    _fields_ = [("status", c_int), ("lnl", c_double), ("length", c_double), ("n_derv", c_int),
+                ("nboot", c_int), ("submit_s", c_double), ("admit_s", c_double),
+                ("finish_s", c_double)]
+
+
+class PlksStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_long), ("traversal_batches", ctypes.c_long),
+                ("derv_batches", ctypes.c_long), ("completed", ctypes.c_long),
+                ("peak_inflight", c_int), ("peak_bytes", c_size_t)]
+
+
+PLKS_LNL, PLKS_NEWTON, PLKS_RELL = 0, 1, 2
+
+
 # numpy view of an op list with the memory layout of plk_op
 OP_DTYPE = np.dtype([("dad_slot", "<i4"), ("nchildren", "<i4"),
                      ("c0_tip", "<i4"), ("c0_id", "<i4"), ("c0_len", "<f8"),
@@ -95,11 +124,19 @@ _PROTOS = {
                                     c_double, c_double, c_int, _DP, _DP, P(c_int)]),
     "plk_rell": (c_int, [c_void_p, c_int, _F32, c_int, _DP]),
     "plk_likelihood_branch_batch": (c_int, [c_void_p, P(PlkBatchItem), c_int, _DP]),
+    "plk_likelihood_derv_batch": (c_int, [c_void_p, _I32, _DP, c_int, _DP, _DP, _DP]),
     "plk_optimize_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
                                    c_double, c_double, c_int, _I32, _DP]),
     "plk_read_partial": (c_int, [c_void_p, c_int, c_int, _DP, _I32]),
     "plk_profile_start": (c_int, [c_void_p]),
     "plk_profile_stop": (c_int, [c_void_p, P(PlkKernelStat), c_int, P(c_int)]),
+    # plk_serve.h
+    "plks_create": (c_int, [c_void_p, P(PlksConfig), P(c_void_p)]),
+    "plks_destroy": (c_int, [c_void_p]),
+    "plks_submit": (c_int, [c_void_p, P(PlksRequest), P(ctypes.c_uint64)]),
+    "plks_wait": (c_int, [c_void_p, ctypes.c_uint64, c_double, P(PlksResult), _DP, P(c_int)]),
+    "plks_get_stats": (c_int, [c_void_p, P(PlksStats)]),
+    "plks_clock": (c_double, [c_void_p]),
     # plk_host.h
     "plkh_num_states": (c_int, [c_int]),
     "plkh_state_unknown": (c_int, [c_int]),

@@ -213,4 +213,40 @@ k_batch_reduce(const BatchItem* __restrict__ items, int nitems, const double* __
     lnl[i] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Batched theta-buffer evaluations: one item = (instance, branch length), the
+// derivative and lnL sums of computeLikelihoodDervSIMD for each. Every item
+// is split into the same pattern tiles as a lone plk_likelihood_derv launch
+// and its tile sums are reduced with the same fixed-order tree, so an item's
+// result is bit-identical to the single-instance call whatever else shares
+// the batch. One launch per (precision, states, categories) shape present.
+struct DervItem {
+    InstDev I;
+    ModelDev m;
+    double length;
+    int tile_begin;   // first global tile
+    int ntiles;       // Ppad / (kThreads / C)
+};
+
+template <typename T, int N, int C>
+__global__ void __launch_bounds__(kThreads)
+k_derv_batch(const DervItem* __restrict__ items, const int* __restrict__ tile_item, int tile_begin,
+             double* __restrict__ tile_sums) {
+    __shared__ double val[3 * N * C];
+    __shared__ double red[3 * (kThreads / 64)];
+    const int g = tile_begin + blockIdx.x;
+    const DervItem& it = items[tile_item[g]];
+    buffer_block<T, N, C>(it.I, it.m, it.length, BUF_DERV, g - it.tile_begin, val, red,
+                          tile_sums + 3 * (size_t)g);
+}
+
+// out[3i..3i+2] = df, ddf, lnL of item i (one workgroup per item)
+__global__ void __launch_bounds__(kThreads)
+k_derv_reduce(const DervItem* __restrict__ items, const double* __restrict__ tile_sums,
+              double* __restrict__ out) {
+    __shared__ double sh[kThreads];
+    const DervItem& it = items[blockIdx.x];
+    reduce_block(tile_sums + 3 * (size_t)it.tile_begin, it.ntiles, 3, out + 3 * (size_t)blockIdx.x, sh);
+}
+
 }  // namespace plkdev

@@ -18,6 +18,7 @@
 #include "plk_sweep.h"
 #include "plk_batch.h"
 #include "plk_codon.h"
+#include "newton.h"
 
 using namespace plkdev;
 
@@ -154,6 +155,15 @@ struct plk_context {
     int bt_vals_cap = 0;
     unsigned* bt_counter = nullptr;
     int bt_counter_cap = 0;
+    // batched derivative buffers
+    DervItem* db_items = nullptr;
+    int db_items_cap = 0;
+    int* db_tile_item = nullptr;
+    int db_tile_cap = 0;
+    double* db_tile_sums = nullptr;
+    int db_tile_sums_cap = 0;
+    double* db_out = nullptr;
+    int db_out_cap = 0;
     int batch_blocks_per_cu = 0;    // 0: autotuned
     // autotuner: key (class mix, log2 tiles) -> (split launches, blocks per CU)
     std::map<long, std::pair<int, int>> batch_tune;
@@ -579,7 +589,8 @@ plk_status plk_destroy(plk_context* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
     void* bt[] = {ctx->bt_items, ctx->bt_jobs, ctx->bt_ops, ctx->bt_tables, ctx->bt_roots,
-                  ctx->bt_tile_item, ctx->bt_tile_sums, ctx->bt_vals, ctx->bt_counter};
+                  ctx->bt_tile_item, ctx->bt_tile_sums, ctx->bt_vals, ctx->bt_counter,
+                  ctx->db_items, ctx->db_tile_item, ctx->db_tile_sums, ctx->db_out};
     for (void* b : bt)
         if (b) (void)hipFree(b);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -869,53 +880,16 @@ plk_status plk_optimize_branch(plk_context* ctx, int id, const plk_op* ops, int 
     if (st != PLK_OK) return st;
     // Optimization::minimizeNewton (utils/optimization.cpp:422-499) on
     // f = -dlnL/dt (PhyloTree::computeFuncDerv, tree/phylotree.cpp:2529-2536)
-    int evals = 0;
-    auto fderv = [&](double x, double& f, double& dfx) -> plk_status {
+    plk::NewtonRun nr;
+    nr.start(length, min_len, max_len, min_len, max_nr);
+    while (!nr.done()) {
         double d1, d2;
-        plk_status s = plk_likelihood_derv(ctx, id, x, &d1, &d2, nullptr);
-        evals++;
-        f = -d1;
-        dfx = -d2;
-        return s;
-    };
-    const double x1 = min_len, x2 = max_len, xacc = min_len;
-    double f, df, dx, dxold, temp, xh, xl, rts, rts_old;
-    double result = 0.0;
-    bool done = false;
-    rts = length;
-    if (rts < x1) rts = x1;
-    if (rts > x2) rts = x2;
-    if ((st = fderv(rts, f, df)) != PLK_OK) return st;
-    if (df >= 0.0 && std::fabs(f) < xacc) {
-        result = rts;
-        done = true;
+        if ((st = plk_likelihood_derv(ctx, id, nr.x(), &d1, &d2, nullptr)) != PLK_OK) return st;
+        nr.feed(-d1, -d2);
     }
-    if (!done) {
-        if (f < 0.0) { xl = rts; xh = x2; } else { xh = rts; xl = x1; }
-        dx = dxold = std::fabs(xh - xl);
-        for (int j = 1; j <= max_nr && !done; j++) {
-            rts_old = rts;
-            if ((df <= 0.0) || (((rts - xh) * df - f) * ((rts - xl) * df - f) >= 0.0)) {
-                dxold = dx;
-                dx = 0.5 * (xh - xl);
-                rts = xl + dx;
-                if (xl == rts) { result = rts; done = true; break; }
-            } else {
-                dxold = dx;
-                dx = f / df;
-                temp = rts;
-                rts -= dx;
-                if (temp == rts) { result = rts; done = true; break; }
-            }
-            if (std::fabs(dx) < xacc || j == max_nr) { result = rts_old; done = true; break; }
-            if ((st = fderv(rts, f, df)) != PLK_OK) return st;
-            if (df > 0.0 && std::fabs(f) < xacc) { result = rts; done = true; break; }
-            if (f < 0.0) xl = rts;
-            else if (f > 0.0) xh = rts;
-        }
-        (void)dxold;
-        if (!done) return fail(PLK_ERR_NEWTON, "maximum number of iterations exceeded in minimizeNewton");
-    }
+    if (nr.failed()) return fail(PLK_ERR_NEWTON, "maximum number of iterations exceeded in minimizeNewton");
+    const int evals = nr.evals();
+    double result = nr.result();
     double optx = result;
     // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625)
     if (optx > max_len * 0.95) {
@@ -998,6 +972,30 @@ plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid)
 }  // namespace
 
 namespace {
+
+// k_derv_batch over tiles [begin, end) whose items all have this shape
+template <typename T, int N>
+plk_status launch_derv_batch_n(plk_context* ctx, int C, int begin, int end) {
+    const dim3 grid(end - begin), blk(kThreads);
+    switch (C) {
+    case 1: hipLaunchKernelGGL((k_derv_batch<T, N, 1>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
+    case 2: hipLaunchKernelGGL((k_derv_batch<T, N, 2>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
+    case 4: hipLaunchKernelGGL((k_derv_batch<T, N, 4>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
+    case 8: hipLaunchKernelGGL((k_derv_batch<T, N, 8>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
+    default: return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", C);
+    }
+    HIP_TRY(hipGetLastError());
+    return PLK_OK;
+}
+
+plk_status launch_derv_batch(plk_context* ctx, int f32, int n, int C, int begin, int end) {
+    switch (n) {
+    case 4: return f32 ? launch_derv_batch_n<float, 4>(ctx, C, begin, end) : launch_derv_batch_n<double, 4>(ctx, C, begin, end);
+    case 20: return f32 ? launch_derv_batch_n<float, 20>(ctx, C, begin, end) : launch_derv_batch_n<double, 20>(ctx, C, begin, end);
+    case kCodonN: return f32 ? launch_derv_batch_n<float, kCodonN>(ctx, C, begin, end) : launch_derv_batch_n<double, kCodonN>(ctx, C, begin, end);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", n);
+}
 
 // One persistent work-list launch over items whose shape k_batch is built
 // for (4 or 20 states, 4 categories); arguments already validated.
@@ -1247,6 +1245,83 @@ plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* i
         for (size_t k = 0; k < fused.size(); k++) lnl[fused_at[k]] = out[k];
     }
     if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch in a batch item; fix-up applied");
+    return PLK_OK;
+}
+
+plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
+                                     int n, double* df, double* ddf, double* lnl) {
+    if (!ctx || n < 0 || (n > 0 && (!instances || !lengths || !df || !ddf)))
+        return fail(PLK_ERR_ARG, "invalid derivative batch");
+    if (n == 0) return PLK_OK;
+    plk_status st;
+    std::vector<Instance*> ins(n);
+    for (int i = 0; i < n; i++) {
+        ins[i] = get_instance(ctx, instances[i]);
+        if ((st = ready(ins[i])) != PLK_OK) return st;
+        if (!ins[i]->theta_ready) return fail(PLK_ERR_STATE, "item %d: theta buffer not computed", i);
+        if (!std::isfinite(lengths[i])) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    // items grouped by shape, so each shape's tiles are one contiguous launch
+    auto shape = [&](int i) {
+        const Instance* in = ins[i];
+        return (in->d.precision == PLK_PREC_F32 ? 1 : 0) * 100000 + in->d.nstates * 100 + in->d.ncat;
+    };
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return shape(a) < shape(b); });
+    std::vector<DervItem> hd(n);
+    std::vector<int> tile_item;
+    double bytes = 0.0;
+    for (int r = 0; r < n; r++) {
+        const Instance* in = ins[order[r]];
+        DervItem& d = hd[r];
+        d.I = inst_dev(in);
+        d.m = model_dev(in);
+        d.length = lengths[order[r]];
+        d.tile_begin = (int)tile_item.size();
+        d.ntiles = in->Ppad / (kThreads / in->d.ncat);
+        for (int t = 0; t < d.ntiles; t++) tile_item.push_back(r);
+        bytes += ((double)in->block * in->elem + 24.0) * in->d.npatterns;
+    }
+    const int total = (int)tile_item.size();
+    if ((st = grow(ctx, nullptr, &ctx->db_items, ctx->db_items_cap, n)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_tile_item, ctx->db_tile_cap, total)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_tile_sums, ctx->db_tile_sums_cap, 3 * total)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_out, ctx->db_out_cap, 3 * n)) != PLK_OK) return st;
+    HIP_TRY(hipMemcpyAsync(ctx->db_items, hd.data(), sizeof(DervItem) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->db_tile_item, tile_item.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
+    const int pe = prof_begin(ctx);
+    for (int r = 0; r < n;) {
+        int e = r;
+        while (e < n && shape(order[e]) == shape(order[r])) e++;
+        const Instance* in = ins[order[r]];
+        st = launch_derv_batch(ctx, in->d.precision == PLK_PREC_F32, in->d.nstates, in->d.ncat,
+                               hd[r].tile_begin, hd[e - 1].tile_begin + hd[e - 1].ntiles);
+        if (st != PLK_OK) return st;
+        r = e;
+    }
+    prof_end(ctx, pe, K_DERV, bytes, 0.0);
+    hipLaunchKernelGGL(k_derv_reduce, dim3(n), dim3(kThreads), 0, ctx->stream, ctx->db_items,
+                       ctx->db_tile_sums, ctx->db_out);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> out(3 * (size_t)n);
+    HIP_TRY(hipMemcpyAsync(out.data(), ctx->db_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    bool nonfinite = false;
+    for (int r = 0; r < n; r++) {
+        const int i = order[r];
+        df[i] = out[3 * r];
+        ddf[i] = out[3 * r + 1];
+        if (lnl) lnl[i] = out[3 * r + 2];
+        if (!std::isfinite(df[i])) {
+            // as plk_likelihood_derv; lnl = NaN marks the item
+            df[i] = ddf[i] = 0.0;
+            if (lnl) lnl[i] = std::nan("");
+            nonfinite = true;
+        }
+    }
+    if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-derivative in a batch item");
     return PLK_OK;
 }
 

@@ -451,11 +451,11 @@ k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
 // 2312-2551, computeLikelihoodFromBufferSIMD 3270-3343). Per pattern:
 // lh = |sum_{c,i} val0 theta| + invar, d1 = sum val1 theta, d2 = sum val2 theta;
 // df += f d1/lh, ddf += f (d2/lh - (d1/lh)^2), lnL += f (log lh + scale).
+// One workgroup's share (patterns tile*PB ..) of the theta-buffer sums;
+// out[0..2] = df, ddf and lnL partial sums (thread 0). val: LDS [3][N*C].
 template <typename T, int N, int C>
-__global__ void __launch_bounds__(kThreads)
-k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ block_sums) {
-    __shared__ double val[3 * N * C];
-    __shared__ double red[3 * (kThreads / 64)];
+__device__ inline void buffer_block(const InstDev& I, const ModelDev& m, double length, int mode,
+                                    int tile, double* val, double* red, double* out) {
     constexpr int PB = kThreads / C;
     constexpr int block = N * C;
     for (int t = threadIdx.x; t < N * C; t += kThreads) {
@@ -475,7 +475,7 @@ k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ bl
     // steps, while the lnL returned beside them uses the plain sum.
     constexpr bool kTwiceLast = (N & 1) && !((N * C) & 1);
     const int c = threadIdx.x % C;
-    const int p = blockIdx.x * PB + threadIdx.x / C;
+    const int p = tile * PB + threadIdx.x / C;
     double lh = 0.0, d1 = 0.0, d2 = 0.0, lhq = 0.0;
     if (p < I.Ppad) {
         const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
@@ -525,17 +525,24 @@ k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ bl
     }
     block_sum<3>(acc, red);
     if (threadIdx.x == 0) {
-        block_sums[3 * blockIdx.x + 0] = acc[0];
-        block_sums[3 * blockIdx.x + 1] = acc[1];
-        block_sums[3 * blockIdx.x + 2] = acc[2];
+        out[0] = acc[0];
+        out[1] = acc[1];
+        out[2] = acc[2];
     }
+}
+
+template <typename T, int N, int C>
+__global__ void __launch_bounds__(kThreads)
+k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ block_sums) {
+    __shared__ double val[3 * N * C];
+    __shared__ double red[3 * (kThreads / 64)];
+    buffer_block<T, N, C>(I, m, length, mode, blockIdx.x, val, red, block_sums + 3 * blockIdx.x);
 }
 
 // Fixed-order final reduction of per-workgroup partial sums: thread t adds
 // entries t, t+256, ... in index order, then a fixed tree over threads.
-__global__ void __launch_bounds__(kThreads)
-k_reduce(const double* __restrict__ in, int nblocks, int nvals, double* __restrict__ out) {
-    __shared__ double sh[kThreads];
+__device__ inline void reduce_block(const double* __restrict__ in, int nblocks, int nvals,
+                                    double* __restrict__ out, double* sh) {
     for (int j = 0; j < nvals; j++) {
         double s = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += kThreads) s += in[(size_t)b * nvals + j];
@@ -548,6 +555,12 @@ k_reduce(const double* __restrict__ in, int nblocks, int nvals, double* __restri
         if (threadIdx.x == 0) out[j] = sh[0];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_reduce(const double* __restrict__ in, int nblocks, int nvals, double* __restrict__ out) {
+    __shared__ double sh[kThreads];
+    reduce_block(in, nblocks, nvals, out, sh);
 }
 
 // RELL: out[b] = sum_p w[b][p] * ptn_lh[p], one workgroup per replicate.

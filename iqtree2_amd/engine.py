@@ -312,6 +312,18 @@ def likelihood_branch_batch(ctx, jobs):
     return out
 
 
+def likelihood_derv_batch(ctx, pairs):
+    """pairs: list of (instance, length) on current theta buffers; one launch
+    per shape (plk_likelihood_derv_batch). Returns arrays df, ddf, lnl."""
+    n = len(pairs)
+    ids = np.array([inst.id for inst, _ in pairs], np.int32)
+    lens = np.array([float(x) for _, x in pairs], np.float64)
+    df, ddf, lnl = np.zeros(n), np.zeros(n), np.zeros(n)
+    check(lib().plk_likelihood_derv_batch(ctx.handle, i32ptr(ids), dptr(lens), n, dptr(df),
+                                          dptr(ddf), dptr(lnl)), "plk_likelihood_derv_batch")
+    return df, ddf, lnl
+
+
 class PhyloTree:
     """PhyloTree-shaped C++ driver (traversal planning + Newton branch optimisation)."""
 

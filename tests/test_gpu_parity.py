@@ -386,3 +386,26 @@ def test_batch_mixed_shapes(ctx):
     np.testing.assert_allclose(batched, np.array(single), rtol=1e-12, atol=0)
     for inst, *_ in jobs:
         inst.close()
+
+
+def test_derv_batch_bit_identical(ctx):
+    """plk_likelihood_derv_batch over mixed shapes (and an instance twice)
+    gives exactly plk_likelihood_derv's df, ddf and lnL per item."""
+    specs = [(ia.DNA, 4, ia.PREC_F64), (ia.PROTEIN, 4, ia.PREC_F64), (ia.CODON, 4, ia.PREC_F64),
+             (ia.DNA, 2, ia.PREC_F32), (ia.DNA, 4, ia.PREC_F64), (ia.CODON, 1, ia.PREC_F32)]
+    insts = []
+    for k, (st, ncat, prec) in enumerate(specs):
+        w = ia.Workload(seq_type=st, ntaxa=9 + k, nsite=300 + 97 * k, ncat=ncat, alpha=0.6,
+                        seed=70 + k)
+        planner = ia.PhyloTree(None, w)
+        ops, node, dad, length = planner.plan(*planner.branches()[k % 3])
+        inst = ia.Instance.from_workload(ctx, w, precision=prec, safe_numeric=st == ia.CODON)
+        inst.likelihood_buffer(ops, node, dad)
+        insts.append((inst, length))
+    pairs = [(inst, x * f) for inst, x in insts for f in (0.5, 2.0)] + [(insts[0][0], 0.3)]
+    df, ddf, lnl = ia.likelihood_derv_batch(ctx, pairs)
+    for k, (inst, x) in enumerate(pairs):
+        d1, d2, l1 = inst.likelihood_derv(x)
+        assert (df[k], ddf[k], lnl[k]) == (d1, d2, l1), (k, df[k], d1, ddf[k], d2, lnl[k], l1)
+    for inst, _ in insts:
+        inst.close()

@@ -22,7 +22,7 @@ def header_functions():
             continue
         text = open(os.path.join(ROOT, "include", fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"\b(plkh?_[a-z0-9_]+)\s*\(", text):
+        for m in re.finditer(r"\b(plk[hs]?_[a-z0-9_]+)\s*\(", text):
             names.add(m.group(1))
     return names
 

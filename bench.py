@@ -190,6 +190,129 @@ def cpu_partitions(ia, parts, plans, budget_s, threads):
                 sample=f"{n} of {len(parts)} partitions, full traversal + lnL each ({elapsed:.1f} s)")
 
 
+def make_requests(ia, nreq, seed=20261202 + 4):
+    """Config 5: per request N ~ log-uniform[8, 256] taxa and L ~
+    log-uniform[500, 50,000] sites on a seeded Exp(0.1) tree; DNA GTR+G4
+    (60%), protein +G4 (30%) or codon GY94 +G4 (10%, omega ~ Gamma(2, 0.25),
+    kappa ~ U(1, 4), equal codon frequencies); op full lnL, single-branch
+    Newton or one multinomial(L, freq/L) bootstrap reweighting, in equal
+    shares."""
+    import numpy as np
+    from iqtree2_amd.serving import Request
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(nreq):
+        ntaxa = int(round(np.exp(rng.uniform(np.log(8), np.log(256)))))
+        nsite = int(round(np.exp(rng.uniform(np.log(500), np.log(50000)))))
+        u = rng.random()
+        st = ia.DNA if u < 0.6 else (ia.PROTEIN if u < 0.9 else ia.CODON)
+        w = ia.Workload(seq_type=st, ntaxa=ntaxa, nsite=nsite, ncat=4, mean_len=0.1,
+                        alpha=-1.0, seed=seed * 1000 + k)
+        kind = ("lnl", "newton", "rell")[int(rng.integers(3))]
+        weights = None
+        if kind == "rell":
+            f = w.get("freq")
+            weights = rng.multinomial(int(f.sum()), f / f.sum())[None, :]
+        branch = int(rng.integers(2 * ntaxa - 3))
+        reqs.append(Request(w, kind, branch=branch, weights=weights))
+    return reqs
+
+
+def bench_serving(ia, ctx, nreq, seed=20261202 + 4):
+    """Capacity with every request queued at once, then a Poisson stream at
+    70% of it; latency = completion - submission on the server clock."""
+    import threading
+    import numpy as np
+    from iqtree2_amd.serving import Server
+    t0 = time.perf_counter()
+    reqs = make_requests(ia, nreq, seed)
+    kinds = {0: "lnl", 1: "newton", 2: "rell"}
+    log(f"serving: {nreq} requests, {sum(r.npatterns for r in reqs)} patterns, largest "
+        f"footprint {max(r.footprint() for r in reqs) / 2**30:.2f} GiB "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+
+    def evals(r, res):
+        # tree-level evaluations: traversal lnL, or theta + Newton steps + final lnL
+        if r.kind == 1:
+            return r.npatterns * (res["n_derv"] + 2)
+        return r.npatterns
+
+    # warm-up: a few requests through a server (kernel loads, autotuning)
+    with Server(ctx) as srv:
+        for r in reqs[:8]:
+            srv.wait(srv.submit(r))
+    # capacity: all queued at once
+    with Server(ctx) as srv:
+        c0 = srv.clock()
+        tickets = [srv.submit(r) for r in reqs]
+        closed = [srv.wait(t) for t in tickets]
+        cap_s = srv.clock() - c0
+        cap_stats = srv.stats()
+    capacity = nreq / cap_s
+    lam = 0.7 * capacity
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / lam, nreq)
+    arrivals = np.cumsum(gaps)
+    results = [None] * nreq
+    with Server(ctx) as srv:
+        tk = [None] * nreq
+        start = time.perf_counter()
+
+        def submitter():
+            for k, at in enumerate(arrivals):
+                delay = at - (time.perf_counter() - start)
+                if delay > 0:
+                    time.sleep(delay)
+                tk[k] = srv.submit(reqs[k])
+
+        th = threading.Thread(target=submitter)
+        s0 = srv.clock()
+        th.start()
+        for k in range(nreq):
+            while tk[k] is None:
+                time.sleep(0.0005)
+            results[k] = srv.wait(tk[k])
+        th.join()
+        span = srv.clock() - s0
+        stats = srv.stats()
+    lat = np.array([r["finish_s"] - r["submit_s"] for r in results]) * 1e3
+    ok = all(r["status"] == 0 for r in results)
+    same = all(a["lnl"] == b["lnl"] and a["length"] == b["length"] for a, b in zip(results, closed))
+    total_evals = sum(evals(r, res) for r, res in zip(reqs, results))
+    mix = {}
+    for r in reqs:
+        key = f"{ {4: 'dna', 20: 'protein', 61: 'codon'}[r.nstates]}_{kinds[r.kind]}"
+        mix[key] = mix.get(key, 0) + 1
+    by_model = {}
+    for r, l in zip(reqs, lat):
+        m = {4: "dna", 20: "protein", 61: "codon"}[r.nstates]
+        by_model.setdefault(m, []).append(l)
+    return {
+        "requests": nreq,
+        "arrival_rate_per_s": lam,
+        "capacity_req_per_s": capacity,
+        "capacity_site_lik_evals_per_s": sum(evals(r, res) for r, res in zip(reqs, closed)) / cap_s,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()),
+        "max_ms": float(lat.max()),
+        "p50_ms_by_model": {m: float(np.percentile(v, 50)) for m, v in by_model.items()},
+        "p99_ms_by_model": {m: float(np.percentile(v, 99)) for m, v in by_model.items()},
+        "throughput_req_per_s": nreq / span,
+        "site_lik_evals_per_s": total_evals / span,
+        "all_ok": ok,
+        "results_identical_to_closed_loop_run": same,
+        "mix": mix,
+        "max_taxa": max(r.ntaxa for r in reqs),
+        "max_patterns": max(r.npatterns for r in reqs),
+        "scheduler": {"iterations": stats["iterations"],
+                      "traversal_batches": stats["traversal_batches"],
+                      "derv_batches": stats["derv_batches"],
+                      "peak_inflight": stats["peak_inflight"],
+                      "peak_gib": stats["peak_bytes"] / 2**30,
+                      "closed_loop_peak_inflight": cap_stats["peak_inflight"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +325,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--partitions", type=int, default=500)
     ap.add_argument("--no-partitions", action="store_true")
+    ap.add_argument("--requests", type=int, default=160)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -255,6 +380,11 @@ def main():
                                  min(16, os.cpu_count() or 1))
         log(f"partitions: {json.dumps(parts)}")
 
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(ia, ctx, args.requests)
+        log(f"serving: {json.dumps(serving)}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -287,6 +417,7 @@ def main():
         "kernels": kernels,
         "fp32_storage": fp32,
         "partition_batch": parts,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     tree.close()

@@ -169,6 +169,19 @@ typedef struct {
 plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
                                        double* lnl);
 
+/* Per-item choice of plk_batch_evaluate */
+typedef enum {
+    PLK_BATCH_LNL = 0,   /* computeLikelihoodBranch: the item's tree lnL */
+    PLK_BATCH_THETA = 1  /* computeLikelihoodBuffer: theta of the item's branch */
+} plk_batch_mode;
+
+/* The same work list with a mode per item: traversal then branch lnL
+ * (lnl[i] as plk_likelihood_branch_batch) or traversal then the theta
+ * buffer (as plk_likelihood_buffer, ready for plk_likelihood_derv; lnl[i] =
+ * 0), all items in one launch. modes may be NULL (all PLK_BATCH_LNL). */
+plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, const int* modes,
+                              int nitems, double* lnl);
+
 /* computeLikelihoodDerv (tree/phylotreesse.cpp:220-224 ->
  * computeLikelihoodDervSIMD, tree/phylokernelnew.h:2233-2633) for n (instance,
  * length) pairs in one launch per shape, each on its instance's current

@@ -14,9 +14,10 @@
  * number of resident requests stays below max_inflight and their summed
  * plk_instance_footprint stays within hbm_budget, keeps each admitted
  * request's alignment and partials resident until it completes, and then
- * iterates: every request waiting for a traversal joins one fused
- * plk_likelihood_branch_batch launch; every branch optimisation in
- * progress advances by one minimizeNewton derivative evaluation through one
+ * iterates: every request waiting for a traversal (its tree lnL, or the
+ * theta buffer a branch optimisation starts from) joins one fused
+ * plk_batch_evaluate launch; every branch optimisation in progress advances
+ * by one minimizeNewton derivative evaluation through one
  * plk_likelihood_derv_batch launch. Requests that arrive meanwhile join at
  * the next iteration. A request's result does not depend on what else was
  * resident: the batched launches reduce every item in its own fixed order.
@@ -84,11 +85,15 @@ typedef struct {
 
 typedef struct {
     long iterations;         /* scheduler iterations that did device work */
-    long traversal_batches;  /* plk_likelihood_branch_batch launches */
+    long traversal_batches;  /* plk_batch_evaluate launches */
     long derv_batches;       /* plk_likelihood_derv_batch launches */
     long completed;
     int peak_inflight;
     size_t peak_bytes;
+    double admit_s;          /* scheduler wall time: instance set-up of admitted requests */
+    double traverse_s;       /* fused traversal launches (and RELL sums) */
+    double derv_s;           /* batched derivative launches */
+    double retire_s;         /* releasing completed requests */
 } plks_stats;
 
 plk_status plks_create(plk_context* ctx, const plks_config* cfg, plks_server** out);

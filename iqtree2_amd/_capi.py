@@ -83,7 +83,8 @@ class PlksResult(ctypes.Structure):
 class PlksStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_long), ("traversal_batches", ctypes.c_long),
                 ("derv_batches", ctypes.c_long), ("completed", ctypes.c_long),
-                ("peak_inflight", c_int), ("peak_bytes", c_size_t)]
+                ("peak_inflight", c_int), ("peak_bytes", c_size_t), ("admit_s", c_double),
+                ("traverse_s", c_double), ("derv_s", c_double), ("retire_s", c_double)]
 
 
 PLKS_LNL, PLKS_NEWTON, PLKS_RELL = 0, 1, 2
@@ -125,6 +126,7 @@ _PROTOS = {
     "plk_rell": (c_int, [c_void_p, c_int, _F32, c_int, _DP]),
     "plk_likelihood_branch_batch": (c_int, [c_void_p, P(PlkBatchItem), c_int, _DP]),
     "plk_likelihood_derv_batch": (c_int, [c_void_p, _I32, _DP, c_int, _DP, _DP, _DP]),
+    "plk_batch_evaluate": (c_int, [c_void_p, P(PlkBatchItem), _I32, c_int, _DP]),
     "plk_optimize_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
                                    c_double, c_double, c_int, _I32, _DP]),
     "plk_read_partial": (c_int, [c_void_p, c_int, c_int, _DP, _I32]),

@@ -27,7 +27,10 @@ struct BatchItem {
     int ntiles;            // tiles with real patterns
     plk_end node, dad;
     int n, C, f32;
+    int mode;              // BATCH_LNL: branch lnL; BATCH_THETA: theta buffer of the branch
 };
+
+enum { BATCH_LNL = 0, BATCH_THETA = 1 };
 
 // One table per (item, op, child): k_prep_ops' arithmetic for a batch.
 struct PrepJob {
@@ -133,6 +136,14 @@ __device__ inline double batch_tile(const BatchItem& it, int tile, unsigned char
     double* root_sh = reinterpret_cast<double*>(tab_sh);
     const plk_end node = it.node, dad = it.dad;
     __syncthreads();
+    if (it.mode == BATCH_THETA) {
+        // computeLikelihoodBufferSIMD's theta: tip dads read the tip vectors
+        if (dad.is_tip)
+            for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = it.m.tip[t];
+        __syncthreads();
+        theta_apply<T, N, C>(I, node, dad, root_sh, p, c);
+        return 0.0;
+    }
     const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
     for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = it.root[t];
     __syncthreads();

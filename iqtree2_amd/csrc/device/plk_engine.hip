@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -90,19 +91,33 @@ struct Instance {
     double staged_bytes = 0.0, staged_flops = 0.0;   // per-pattern cost of the staged ops
 };
 
+size_t alloc_class(size_t bytes);
+
+// device bytes of an instance, as allocated (size classes included), with
+// the op tables of a full traversal
 size_t footprint_of(const plk_instance_desc* d, int Ppad) {
     const size_t elem = d->precision == PLK_PREC_F32 ? 4 : 8;
     const size_t scale_elem = d->precision == PLK_PREC_F32 ? 4 : 2;
-    const size_t block = (size_t)d->nstates * d->ncat;
+    const size_t n = d->nstates, SU = d->state_unknown;
+    const size_t block = n * d->ncat;
     const size_t ncnt = d->safe_numeric ? d->ncat : 1;
+    const size_t table = std::max(block * n, (SU + 1) * block);
+    const size_t bufs[] = {
+        (size_t)d->ntaxa * Ppad,                                  // states
+        (size_t)Ppad * 8, (size_t)Ppad * 8,                       // freq, invar
+        (n + 2 * n * n + (SU + 1) * n + 2 * (size_t)d->ncat) * 8, // model
+        (size_t)d->nslots * Ppad * block * elem,                  // partials
+        (size_t)d->nslots * Ppad * ncnt * scale_elem,             // scale counters
+        (size_t)Ppad * block * elem,                              // theta
+        (size_t)Ppad * 8, (size_t)Ppad * 8,                       // buffer_scale, ptn_lh
+        (size_t)(Ppad / (kThreads / 8) + 1) * 3 * 8,              // block sums
+        64,                                                       // reduced scalars
+        (SU + 2) * block * 8,                                     // root tables
+        sizeof(plk_op) * (size_t)d->nslots,                       // ops of a full traversal
+        (size_t)d->nslots * 2 * table * elem,                     // and their tables
+    };
     size_t b = 0;
-    b += round_up((size_t)d->ntaxa * Ppad, 256);                 // states
-    b += 2 * (size_t)Ppad * 8;                                   // freq, invar
-    b += (size_t)d->nslots * Ppad * block * elem;                // partials
-    b += round_up((size_t)d->nslots * Ppad * ncnt * scale_elem, 256);
-    b += (size_t)Ppad * block * elem;                            // theta
-    b += 2 * (size_t)Ppad * 8;                                   // buffer_scale, ptn_lh
-    b += (size_t)(Ppad / 8 + 64) * 3 * 8;                        // block sums
+    for (size_t x : bufs) b += alloc_class(x);
     return b;
 }
 
@@ -136,6 +151,8 @@ struct plk_context {
     std::mutex mu;
     std::vector<Instance*> inst;
     double* h_out = nullptr;  // pinned host staging for reduced scalars
+    unsigned char* h_stage = nullptr;   // pinned host staging for uploads
+    size_t h_stage_cap = 0;
     // batched evaluation buffers (grown on demand)
     BatchItem* bt_items = nullptr;
     int bt_items_cap = 0;
@@ -165,34 +182,107 @@ struct plk_context {
     double* db_out = nullptr;
     int db_out_cap = 0;
     int batch_blocks_per_cu = 0;    // 0: autotuned
+    // caching device allocator: live blocks by address, free blocks by size
+    // class; `used` counts live bytes, `cached` the free blocks kept
+    std::unordered_map<void*, size_t> live;
+    std::multimap<size_t, void*> cache;
+    size_t cached = 0;
     // autotuner: key (class mix, log2 tiles) -> (split launches, blocks per CU)
     std::map<long, std::pair<int, int>> batch_tune;
+    std::map<long, int> batch_seen;   // occurrences of each key before tuning
     bool autotune = true;
 };
 
 namespace {
 
+// Allocation size classes: powers of two up to 64 KiB, then four steps per
+// octave, so a freed block serves any later request of its class (at most
+// 25% rounding above 64 KiB).
+size_t alloc_class(size_t bytes) {
+    bytes = round_up(bytes ? bytes : 8, 256);
+    size_t p = 256;
+    while (p < bytes) p <<= 1;
+    if (p <= 65536) return p;
+    const size_t step = p / 8;
+    size_t c = p / 2;
+    while (c < bytes) c += step;
+    return c;
+}
+
+void release_cached(plk_context* ctx, size_t need) {
+    // largest blocks first, until live + cached + need fits the budget
+    while (!ctx->cache.empty() && ctx->used + ctx->cached + need > ctx->budget) {
+        auto last = std::prev(ctx->cache.end());
+        (void)hipFree(last->second);
+        ctx->cached -= last->first;
+        ctx->cache.erase(last);
+    }
+}
+
+// Device allocation through the context's block cache. Every use of a
+// block is ordered on the context stream, so a block freed by one instance
+// can back the next one without a device synchronisation.
 template <typename P>
 plk_status dmalloc(plk_context* ctx, Instance* in, P** ptr, size_t bytes) {
-    bytes = round_up(bytes ? bytes : 8, 256);
-    if (ctx->used + bytes > ctx->budget)
+    const size_t cls = alloc_class(bytes);
+    if (ctx->used + cls > ctx->budget)
         return fail(PLK_ERR_NOMEM, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use",
-                    bytes, ctx->used, ctx->budget);
+                    cls, ctx->used, ctx->budget);
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) return fail(PLK_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    ctx->used += bytes;
-    if (in) in->bytes += bytes;
+    auto it = ctx->cache.find(cls);
+    if (it != ctx->cache.end()) {
+        p = it->second;
+        ctx->cache.erase(it);
+        ctx->cached -= cls;
+    } else {
+        release_cached(ctx, cls);
+        hipError_t e = hipMalloc(&p, cls);
+        if (e != hipSuccess && !ctx->cache.empty()) {
+            (void)hipGetLastError();
+            release_cached(ctx, ctx->budget);   // everything cached
+            e = hipMalloc(&p, cls);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(PLK_ERR_NOMEM, "hipMalloc(%zu): %s", cls, hipGetErrorString(e));
+        }
+    }
+    ctx->live[p] = cls;
+    ctx->used += cls;
+    if (in) in->bytes += cls;
     *ptr = reinterpret_cast<P*>(p);
     return PLK_OK;
 }
 
-void dfree(plk_context* ctx, Instance* in, void* p, size_t bytes) {
+void dfree(plk_context* ctx, Instance* in, void* p, size_t) {
     if (!p) return;
-    (void)hipFree(p);
-    bytes = round_up(bytes ? bytes : 8, 256);
-    ctx->used = ctx->used >= bytes ? ctx->used - bytes : 0;
-    if (in) in->bytes = in->bytes >= bytes ? in->bytes - bytes : 0;
+    auto it = ctx->live.find(p);
+    if (it == ctx->live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const size_t cls = it->second;
+    ctx->live.erase(it);
+    ctx->used = ctx->used >= cls ? ctx->used - cls : 0;
+    if (in) in->bytes = in->bytes >= cls ? in->bytes - cls : 0;
+    ctx->cache.insert({cls, p});
+    ctx->cached += cls;
+}
+
+Instance* get_instance(plk_context* ctx, int id);
+
+// returns the instance's blocks to the cache (caller holds ctx->mu)
+plk_status destroy_instance(plk_context* ctx, int id) {
+    Instance* in = get_instance(ctx, id);
+    if (!in) return fail(PLK_ERR_ARG, "unknown instance %d", id);
+    void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
+                    in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
+                    in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
+                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing};
+    for (void* b : bufs) dfree(ctx, nullptr, b, 0);
+    delete in;
+    ctx->inst[id] = nullptr;
+    return PLK_OK;
 }
 
 // event pair around a launch when profiling is on; returns the pair index or -1
@@ -584,17 +674,16 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
 
 plk_status plk_destroy(plk_context* ctx) {
     if (!ctx) return PLK_OK;
-    for (int i = 0; i < (int)ctx->inst.size(); i++)
-        if (ctx->inst[i]) plk_instance_destroy(ctx, i);
     (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < (int)ctx->inst.size(); i++)
+        if (ctx->inst[i]) destroy_instance(ctx, i);
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
-    void* bt[] = {ctx->bt_items, ctx->bt_jobs, ctx->bt_ops, ctx->bt_tables, ctx->bt_roots,
-                  ctx->bt_tile_item, ctx->bt_tile_sums, ctx->bt_vals, ctx->bt_counter,
-                  ctx->db_items, ctx->db_tile_item, ctx->db_tile_sums, ctx->db_out};
-    for (void* b : bt)
-        if (b) (void)hipFree(b);
+    // every block still live (batch buffers) and every cached one
+    for (auto& kv : ctx->live) (void)hipFree(kv.first);
+    for (auto& kv : ctx->cache) (void)hipFree(kv.second);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     (void)hipHostFree(ctx->h_out);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     delete ctx;
     return PLK_OK;
 }
@@ -675,11 +764,15 @@ plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* d, int
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             st = fail(PLK_ERR_DEVICE, "initialising instance buffers failed");
     }
-    ctx->inst.push_back(in);
-    *id = (int)ctx->inst.size() - 1;
+    // ids of destroyed instances are reused
+    int slot = 0;
+    while (slot < (int)ctx->inst.size() && ctx->inst[slot]) slot++;
+    if (slot == (int)ctx->inst.size()) ctx->inst.push_back(nullptr);
+    ctx->inst[slot] = in;
+    *id = slot;
     if (st != PLK_OK) {
         std::string msg = g_last_error;
-        plk_instance_destroy(ctx, *id);
+        destroy_instance(ctx, *id);
         *id = -1;
         g_last_error = msg;
         return st;
@@ -688,19 +781,9 @@ plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* d, int
 }
 
 plk_status plk_instance_destroy(plk_context* ctx, int id) {
-    Instance* in = get_instance(ctx, id);
-    if (!in) return fail(PLK_ERR_ARG, "unknown instance %d", id);
-    (void)hipStreamSynchronize(ctx->stream);
-    void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
-                    in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
-                    in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
-                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
-    ctx->used = ctx->used >= in->bytes ? ctx->used - in->bytes : 0;
-    delete in;
-    ctx->inst[id] = nullptr;
-    return PLK_OK;
+    if (!ctx) return fail(PLK_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return destroy_instance(ctx, id);
 }
 
 plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, const double* freq,
@@ -711,10 +794,33 @@ plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, con
     for (size_t i = 0; i < (size_t)in->d.ntaxa * P; i++)
         if (states[i] > in->d.state_unknown) return fail(PLK_ERR_ARG, "tip state %d > STATE_UNKNOWN", states[i]);
     std::lock_guard<std::mutex> lock(ctx->mu);
-    HIP_TRY(hipMemcpy2DAsync(in->states, in->Ppad, states, P, P, in->d.ntaxa, hipMemcpyHostToDevice,
-                             ctx->stream));
-    HIP_TRY(hipMemcpyAsync(in->freq, freq, sizeof(double) * P, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(in->invar, invar, sizeof(double) * P, hipMemcpyHostToDevice, ctx->stream));
+    // padded rows, weights and +I mass packed into pinned memory: three
+    // contiguous copies instead of one staged copy per taxon
+    const size_t Ppad = in->Ppad, nst = (size_t)in->d.ntaxa * Ppad;
+    const size_t need = nst + 2 * Ppad * sizeof(double) + 16;
+    if (need > ctx->h_stage_cap) {
+        const size_t cap = std::max(need, 2 * ctx->h_stage_cap);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->h_stage_cap = 0;
+        HIP_TRY(hipHostMalloc((void**)&ctx->h_stage, cap, hipHostMallocDefault));
+        ctx->h_stage_cap = cap;
+    }
+    unsigned char* hs = ctx->h_stage;
+    for (int t = 0; t < in->d.ntaxa; t++) {
+        std::memcpy(hs + (size_t)t * Ppad, states + (size_t)t * P, P);
+        std::memset(hs + (size_t)t * Ppad + P, in->d.state_unknown, Ppad - P);
+    }
+    double* hf = reinterpret_cast<double*>(hs + round_up(nst, 16));
+    double* hv = hf + Ppad;
+    std::memcpy(hf, freq, sizeof(double) * P);
+    std::memset(hf + P, 0, sizeof(double) * (Ppad - P));
+    std::memcpy(hv, invar, sizeof(double) * P);
+    std::memset(hv + P, 0, sizeof(double) * (Ppad - P));
+    HIP_TRY(hipMemcpyAsync(in->states, hs, nst, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->freq, hf, sizeof(double) * Ppad, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->invar, hv, sizeof(double) * Ppad, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     in->patterns_set = true;
     in->theta_ready = false;
@@ -999,8 +1105,8 @@ plk_status launch_derv_batch(plk_context* ctx, int f32, int n, int C, int begin,
 
 // One persistent work-list launch over items whose shape k_batch is built
 // for (4 or 20 states, 4 categories); arguments already validated.
-plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, int nitems, Instance* const* ins,
-                       double* lnl) {
+plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int* modes, int nitems,
+                       Instance* const* ins, double* lnl) {
     plk_status st;
     std::lock_guard<std::mutex> lock(ctx->mu);
     // largest partitions first (PhyloSuperTree::computePartitionOrder)
@@ -1052,6 +1158,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, int nitems
         b.n = in->d.nstates;
         b.C = in->d.ncat;
         b.f32 = in->d.precision == PLK_PREC_F32;
+        b.mode = modes ? modes[i] : BATCH_LNL;
         for (int t = 0; t < ntiles; t++) tile_item.push_back(r);
         lens[r] = items[i].length;
         const size_t s1 = round_up((size_t)b.n * b.n * in->elem, 16) +
@@ -1150,8 +1257,12 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, int nitems
     if (ctx->batch_blocks_per_cu == 0 && tuned != ctx->batch_tune.end()) {
         split = tuned->second.first;
         bpc = tuned->second.second;
-    } else if (ctx->batch_blocks_per_cu == 0 && ctx->autotune) {
-        // time every candidate once on this batch (results are identical)
+    } else if (ctx->batch_blocks_per_cu == 0 && ctx->autotune && total_tiles >= 4 * ctx->num_cu &&
+               ++ctx->batch_seen[key] >= 2) {
+        // time every candidate once on this batch (results are identical);
+        // only a shape that recurs is tuned (a partitioned tree is evaluated
+        // many times; a one-off batch of requests is not worth the trials),
+        // and batches of fewer tiles than a few per CU take the default
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -1185,6 +1296,11 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, int nitems
     bool nonfinite = false;
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];
+        if (hb[r].mode == BATCH_THETA) {
+            ins[i]->theta_ready = true;
+            lnl[i] = 0.0;
+            continue;
+        }
         ins[i]->ptn_lh_ready = true;
         double v = out[r];
         if (!std::isfinite(v)) {
@@ -1203,7 +1319,16 @@ extern "C" {
 
 plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
                                        double* lnl) {
+    return plk_batch_evaluate(ctx, items, nullptr, nitems, lnl);
+}
+
+plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, const int* modes,
+                              int nitems, double* lnl) {
     if (!ctx || nitems < 0 || (nitems > 0 && (!items || !lnl))) return fail(PLK_ERR_ARG, "invalid batch");
+    if (modes)
+        for (int i = 0; i < nitems; i++)
+            if (modes[i] != PLK_BATCH_LNL && modes[i] != PLK_BATCH_THETA)
+                return fail(PLK_ERR_ARG, "item %d: invalid mode %d", i, modes[i]);
     if (nitems == 0) return PLK_OK;
     plk_status st;
     std::vector<Instance*> ins(nitems);
@@ -1222,7 +1347,7 @@ plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* i
     // (61 states, category counts other than 4) run one traversal each
     std::vector<plk_batch_item> fused;
     std::vector<Instance*> fused_ins;
-    std::vector<int> fused_at;
+    std::vector<int> fused_at, fused_mode;
     bool nonfinite = false;
     for (int i = 0; i < nitems; i++) {
         const Instance* in = ins[i];
@@ -1230,16 +1355,24 @@ plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* i
             fused.push_back(items[i]);
             fused_ins.push_back(ins[i]);
             fused_at.push_back(i);
+            fused_mode.push_back(modes ? modes[i] : PLK_BATCH_LNL);
             continue;
         }
-        st = plk_likelihood_branch(ctx, items[i].instance, items[i].ops, items[i].nops, items[i].node,
-                                   items[i].dad, items[i].length, &lnl[i], nullptr);
+        if (modes && modes[i] == PLK_BATCH_THETA) {
+            lnl[i] = 0.0;
+            st = plk_likelihood_buffer(ctx, items[i].instance, items[i].ops, items[i].nops,
+                                       items[i].node, items[i].dad);
+        } else {
+            st = plk_likelihood_branch(ctx, items[i].instance, items[i].ops, items[i].nops,
+                                       items[i].node, items[i].dad, items[i].length, &lnl[i], nullptr);
+        }
         if (st == PLK_ERR_NONFINITE) nonfinite = true;
         else if (st != PLK_OK) return st;
     }
     if (!fused.empty()) {
         std::vector<double> out(fused.size());
-        st = batch_fused(ctx, fused.data(), (int)fused.size(), fused_ins.data(), out.data());
+        st = batch_fused(ctx, fused.data(), fused_mode.data(), (int)fused.size(), fused_ins.data(),
+                         out.data());
         if (st != PLK_OK && st != PLK_ERR_NONFINITE) return st;
         if (st == PLK_ERR_NONFINITE) nonfinite = true;
         for (size_t k = 0; k < fused.size(); k++) lnl[fused_at[k]] = out[k];

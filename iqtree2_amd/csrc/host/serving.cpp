@@ -133,51 +133,52 @@ void finish_rell(plks_server* s, Req* q) {
 void step(plks_server* s) {
     plk_context* ctx = s->ctx;
     bool did = false;
-    // every request waiting for its tree lnL: one fused traversal launch
+    // every request that needs a traversal (tree lnL, or theta for a branch
+    // optimisation just admitted): one fused launch
     std::vector<Req*> tv;
     for (Req* q : s->inflight)
-        if (q->phase == Req::TRAVERSE) tv.push_back(q);
+        if (q->phase == Req::TRAVERSE || q->phase == Req::THETA) tv.push_back(q);
     if (!tv.empty()) {
         did = true;
         std::vector<plk_batch_item> items(tv.size());
+        std::vector<int> modes(tv.size());
         std::vector<double> lnl(tv.size(), 0.0);
         for (size_t k = 0; k < tv.size(); k++) {
             const plks_request& r = tv[k]->r;
             items[k] = plk_batch_item{tv[k]->inst, r.ops, r.nops, r.node, r.dad, r.length};
+            modes[k] = tv[k]->phase == Req::THETA ? PLK_BATCH_THETA : PLK_BATCH_LNL;
         }
-        const int st = plk_likelihood_branch_batch(ctx, items.data(), (int)items.size(), lnl.data());
+        const double tb = s->now();
+        std::vector<int> status(tv.size(), PLK_OK);
+        const int st = plk_batch_evaluate(ctx, items.data(), modes.data(), (int)items.size(), lnl.data());
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->stats.traversal_batches++;
         }
-        if (st == PLK_OK || st == PLK_ERR_NONFINITE) {
-            for (size_t k = 0; k < tv.size(); k++) {
-                tv[k]->res.lnl = lnl[k];
-                tv[k]->res.status = PLK_OK;
-            }
-        } else {
+        if (st != PLK_OK && st != PLK_ERR_NONFINITE) {
             // a request the batch rejected: find it by giving each its own
             // batch (the same per-item arithmetic as in a shared one)
             for (size_t k = 0; k < tv.size(); k++) {
-                const int sk = plk_likelihood_branch_batch(ctx, &items[k], 1, &tv[k]->res.lnl);
-                tv[k]->res.status = sk == PLK_ERR_NONFINITE ? PLK_OK : sk;
+                const int sk = plk_batch_evaluate(ctx, &items[k], &modes[k], 1, &lnl[k]);
+                status[k] = sk == PLK_ERR_NONFINITE ? PLK_OK : sk;
             }
         }
-        for (Req* q : tv) finish_rell(s, q);
-    }
-    // branch optimisations just admitted: theta for their branch
-    for (Req* q : s->inflight) {
-        if (q->phase != Req::THETA) continue;
-        did = true;
-        const plks_request& r = q->r;
-        const int st = plk_likelihood_buffer(ctx, q->inst, r.ops, r.nops, r.node, r.dad);
-        if (st != PLK_OK) {
-            q->res.status = st;
-            q->phase = Req::DONE;
-            continue;
+        for (size_t k = 0; k < tv.size(); k++) {
+            Req* q = tv[k];
+            q->res.status = status[k];
+            if (status[k] != PLK_OK) {
+                q->phase = Req::DONE;
+            } else if (q->phase == Req::THETA) {
+                const plks_request& r = q->r;
+                q->nr.start(r.length, r.min_len, r.max_len, r.min_len, r.max_nr);
+                q->phase = Req::NEWTON;
+            } else {
+                q->res.lnl = lnl[k];
+                finish_rell(s, q);
+            }
         }
-        q->nr.start(r.length, r.min_len, r.max_len, r.min_len, r.max_nr);
-        q->phase = Req::NEWTON;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.traverse_s += s->now() - tb;
     }
     // one derivative evaluation for every optimisation in progress, and the
     // final lnL evaluations of those whose Newton iteration has ended
@@ -206,6 +207,7 @@ void step(plks_server* s) {
     if (!dv.empty()) {
         did = true;
         const size_t n = dv.size();
+        const double tb = s->now();
         std::vector<double> df(n), ddf(n), lnl(n);
         const int st = plk_likelihood_derv_batch(ctx, ids.data(), xs.data(), (int)n, df.data(), ddf.data(),
                                                  lnl.data());
@@ -261,6 +263,8 @@ void step(plks_server* s) {
                 q->phase = Req::DONE;
             }
         }
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.derv_s += s->now() - tb;
     }
     if (did) {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -288,9 +292,16 @@ void run(plks_server* s) {
             s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->inflight.empty(); });
             if (s->stop && s->queue.empty() && s->inflight.empty()) return;
         }
+        const double t0 = s->now();
         admit(s);
+        const double t1 = s->now();
         if (!s->inflight.empty()) step(s);
+        const double t2 = s->now();
         retire(s);
+        const double t3 = s->now();
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.admit_s += t1 - t0;
+        s->stats.retire_s += t3 - t2;
     }
 }
 

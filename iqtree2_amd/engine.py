@@ -312,6 +312,25 @@ def likelihood_branch_batch(ctx, jobs):
     return out
 
 
+def batch_evaluate(ctx, jobs, modes):
+    """plk_batch_evaluate: jobs as likelihood_branch_batch, modes[i] 0 for the
+    branch lnL, 1 for the theta buffer of the branch (lnl 0)."""
+    items = (_capi.PlkBatchItem * len(jobs))()
+    keep = []
+    for k, (inst, ops, node, dad, length) in enumerate(jobs):
+        arr = np.ascontiguousarray(ops, dtype=OP_DTYPE) if ops is not None and len(ops) else None
+        keep.append(arr)
+        items[k].instance = inst.id
+        items[k].ops = arr.ctypes.data if arr is not None else None
+        items[k].nops = len(arr) if arr is not None else 0
+        items[k].node, items[k].dad, items[k].length = node, dad, float(length)
+    m = np.ascontiguousarray(modes, np.int32)
+    out = np.zeros(len(jobs))
+    check(lib().plk_batch_evaluate(ctx.handle, items, i32ptr(m), len(jobs), dptr(out)),
+          "plk_batch_evaluate")
+    return out
+
+
 def likelihood_derv_batch(ctx, pairs):
     """pairs: list of (instance, length) on current theta buffers; one launch
     per shape (plk_likelihood_derv_batch). Returns arrays df, ddf, lnl."""

@@ -348,9 +348,11 @@ def test_partition_batch_equals_per_partition(ctx):
         jobs.append((inst, ops, node, dad, length))
     for inst, ops, node, dad, length in jobs:
         single.append(inst.likelihood_branch(ops, node, dad, length))
-    first = ia.likelihood_branch_batch(ctx, jobs)      # includes autotuning trials
-    second = ia.likelihood_branch_batch(ctx, jobs)     # tuned shape
+    first = ia.likelihood_branch_batch(ctx, jobs)      # default launch shape
+    second = ia.likelihood_branch_batch(ctx, jobs)     # recurring shape: autotuning trials
+    third = ia.likelihood_branch_batch(ctx, jobs)      # tuned shape
     np.testing.assert_array_equal(first, second)
+    np.testing.assert_array_equal(first, third)
     np.testing.assert_allclose(first, np.array(single), rtol=1e-12, atol=0)
     # against the oracle for the FP64 partitions
     for k, (w, prec) in enumerate(parts[:8]):
@@ -409,3 +411,35 @@ def test_derv_batch_bit_identical(ctx):
         assert (df[k], ddf[k], lnl[k]) == (d1, d2, l1), (k, df[k], d1, ddf[k], d2, lnl[k], l1)
     for inst, _ in insts:
         inst.close()
+
+
+def test_batch_theta_mode_bit_identical(ctx):
+    """plk_batch_evaluate with theta items next to lnL items: the theta it
+    leaves gives exactly the derivatives of plk_likelihood_buffer's theta,
+    and the lnL items equal plk_likelihood_branch_batch."""
+    specs = [(ia.DNA, ia.PREC_F64), (ia.PROTEIN, ia.PREC_F64), (ia.DNA, ia.PREC_F32),
+             (ia.CODON, ia.PREC_F64), (ia.DNA, ia.PREC_F64), (ia.PROTEIN, ia.PREC_F32)]
+    jobs, refs, modes = [], [], []
+    for k, (st, prec) in enumerate(specs):
+        w = ia.Workload(seq_type=st, ntaxa=10 + 2 * k, nsite=400 + 150 * k, ncat=4, alpha=0.6,
+                        seed=300 + k)
+        planner = ia.PhyloTree(None, w)
+        ops, node, dad, length = planner.plan(*planner.branches()[k % 4])
+        safe = st == ia.CODON
+        a = ia.Instance.from_workload(ctx, w, precision=prec, safe_numeric=safe)
+        b = ia.Instance.from_workload(ctx, w, precision=prec, safe_numeric=safe)
+        jobs.append((a, ops, node, dad, length))
+        refs.append((b, ops, node, dad, length))
+        modes.append(k % 2)
+    out = ia.batch_evaluate(ctx, jobs, modes)
+    lnl_ref = ia.likelihood_branch_batch(ctx, [r for r, m in zip(refs, modes) if m == 0])
+    np.testing.assert_array_equal(out[np.array(modes) == 0], lnl_ref)
+    for (a, *_), (b, ops, node, dad, length), m in zip(jobs, refs, modes):
+        if m != 1:
+            continue
+        b.likelihood_buffer(ops, node, dad)
+        for x in (0.5 * length, length, 3 * length + 0.01):
+            assert a.likelihood_derv(x) == b.likelihood_derv(x)
+    for (a, *_), (b, *_) in zip(jobs, refs):
+        a.close()
+        b.close()

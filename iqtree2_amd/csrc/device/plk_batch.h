@@ -11,7 +11,7 @@
 // k_batch_reduce then adds each instance's tile sums in tile order, so the
 // result does not depend on which workgroup ran which tile.
 #pragma once
-#include "plk_kernels.h"
+#include "plk_mfma.h"
 
 namespace plkdev {
 
@@ -116,6 +116,18 @@ __device__ inline double batch_tile(const BatchItem& it, int tile, unsigned char
     const int p = tile * PB + threadIdx.x / C;
     const T* tables = reinterpret_cast<const T*>(it.tables);
     __syncthreads();
+    if constexpr (N == 20) {
+        // protein: the matrix-core tile of the lone traversal kernel, four
+        // 16-pattern sub-tiles per batch tile (PB = 64 with four categories)
+        static_assert(kThreads / C == 4 * kMfmaTile, "batch tile must be four MFMA tiles");
+        using L = MfmaSmall<T, N, C>;
+        T* inv_m = reinterpret_cast<T*>(smem);
+        T* tab_m = inv_m + L::kInv;
+        T* red_m = tab_m + 2 * L::kTab;
+        stage_inv<T>(it.m.inv_evec, inv_m, N, L::S);
+        mfma_ops_tile<T, N, C, 4>(I, it.ops, it.nops, tables, it.table_size, inv_m, tab_m, red_m,
+                                  tile * PB);
+    } else {
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)it.m.inv_evec[t];
     for (int o = 0; o < it.nops; o++) {
         const plk_op op = it.ops[o];
@@ -132,6 +144,7 @@ __device__ inline double batch_tile(const BatchItem& it, int tile, unsigned char
         __syncthreads();
         op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + it.table_size, L.is_tip, L.id, R.is_tip, R.id,
                           op.dad_slot, p, c);
+    }
     }
     double* root_sh = reinterpret_cast<double*>(tab_sh);
     const plk_end node = it.node, dad = it.dad;

@@ -183,11 +183,12 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
     }
 }
 
-// Branch ends for large state counts, one thread per (pattern, category)
-// with the C lanes of a pattern adjacent, streaming the N-element vectors.
-// MODE_BRANCH: pattern lnL with per-category scale alignment
-// (tree/phylokernelnew.h:2753-3104, SAFE_NUMERIC) and the workgroup sum;
-// MODE_THETA: theta and buffer_scale (tree/phylokernelnew.h:2091-2230).
+// Branch ends for the matrix-core kernels, one thread per (pattern,
+// category) with the C lanes of a pattern adjacent, streaming the
+// N-element vectors. MODE_BRANCH: pattern lnL (tree/phylokernelnew.h:
+// 2753-3104; safe mode aligns the per-category counters to the least-scaled
+// category) and the workgroup sum; MODE_THETA: theta and buffer_scale
+// (tree/phylokernelnew.h:2091-2230).
 template <typename T, int N, int C>
 __global__ void __launch_bounds__(kThreads)
 k_root_stream(InstDev I, ModelDev m, int mode, plk_end node, plk_end dad,
@@ -200,7 +201,8 @@ k_root_stream(InstDev I, ModelDev m, int mode, plk_end node, plk_end dad,
     const int p = blockIdx.x * PB + threadIdx.x / C;
     const T* partial = reinterpret_cast<const T*>(I.partial);
     const ST* scale = reinterpret_cast<const ST*>(I.scale);
-    const size_t sidx = (size_t)p * C + c;
+    const bool safe = I.safe != 0;
+    const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
     const T* nv = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
     int sc = scale[(size_t)node.id * I.scale_stride + sidx];
     const T* dv = nullptr;
@@ -211,8 +213,13 @@ k_root_stream(InstDev I, ModelDev m, int mode, plk_end node, plk_end dad,
         dv = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
         sc += scale[(size_t)dad.id * I.scale_stride + sidx];
     }
-    const int minsc = group_min_int<C>(sc);
-    const double adjust = Prec<T>::safe_adjust(sc - minsc);
+    // safe mode: per-category counters aligned to the least-scaled category
+    int minsc = sc;
+    double adjust = 1.0;
+    if (safe) {
+        minsc = group_min_int<C>(sc);
+        adjust = Prec<T>::safe_adjust(sc - minsc);
+    }
     if (mode == MODE_THETA) {
         T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
         if (dad.is_tip) {
@@ -234,7 +241,7 @@ k_root_stream(InstDev I, ModelDev m, int mode, plk_end node, plk_end dad,
         lhc = val[0] * (double)dv[0] * (double)nv[0];
         for (int i = 1; i < N; i++) lhc = fma(val[i] * (double)dv[i], (double)nv[i], lhc);
     }
-    const double lh0 = group_sum_sumvec<C>(lhc * adjust);
+    const double lh0 = safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
     double contrib = 0.0;
     if (c == 0) {
         const double lh = fabs(lh0) + I.invar[p];

@@ -19,6 +19,7 @@
 #include "plk_sweep.h"
 #include "plk_batch.h"
 #include "plk_codon.h"
+#include "plk_mfma.h"
 #include "newton.h"
 
 using namespace plkdev;
@@ -459,11 +460,11 @@ double root_bytes(const Instance* in, int mode, plk_end dad) {
     return b;
 }
 
-template <typename T, int C>
+template <typename T, int N, int C>
 plk_status launch_root_stream(plk_context* ctx, Instance* in, int mode, plk_end node, plk_end dad) {
     const int grid = in->Ppad / (kThreads / C);
     const int pe = prof_begin(ctx);
-    hipLaunchKernelGGL((k_root_stream<T, kCodonN, C>), dim3(grid), dim3(kThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((k_root_stream<T, N, C>), dim3(grid), dim3(kThreads), 0, ctx->stream,
                        inst_dev(in), model_dev(in), mode, node, dad, in->root_tab, in->block_sums);
     HIP_TRY(hipGetLastError());
     prof_end(ctx, pe, K_ROOT, root_bytes(in, mode, dad) * in->d.npatterns,
@@ -487,10 +488,41 @@ plk_status launch_traverse_codon(plk_context* ctx, Instance* in, int nops, int m
     }
     if (mode == MODE_PARTIALS) return PLK_OK;
     switch (in->d.ncat) {
-    case 1: return launch_root_stream<T, 1>(ctx, in, mode, node, dad);
-    case 2: return launch_root_stream<T, 2>(ctx, in, mode, node, dad);
-    case 4: return launch_root_stream<T, 4>(ctx, in, mode, node, dad);
-    case 8: return launch_root_stream<T, 8>(ctx, in, mode, node, dad);
+    case 1: return launch_root_stream<T, kCodonN, 1>(ctx, in, mode, node, dad);
+    case 2: return launch_root_stream<T, kCodonN, 2>(ctx, in, mode, node, dad);
+    case 4: return launch_root_stream<T, kCodonN, 4>(ctx, in, mode, node, dad);
+    case 8: return launch_root_stream<T, kCodonN, 8>(ctx, in, mode, node, dad);
+    }
+    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+}
+
+// 20 states: the MFMA traversal kernel, then the streaming root stage
+template <typename T, int C>
+plk_status launch_traverse_mfma_c(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                                  plk_end dad) {
+    constexpr int N = 20;
+    if (nops > 0) {
+        const int grid = in->Ppad / kMfmaTile;
+        const int pe = prof_begin(ctx);
+        hipLaunchKernelGGL((k_traverse_mfma<T, N, C>), dim3(grid), dim3(kThreads), 0, ctx->stream,
+                           inst_dev(in), model_dev(in), in->ops, nops,
+                           reinterpret_cast<const T*>(in->tables), in->table_size);
+        HIP_TRY(hipGetLastError());
+        prof_end(ctx, pe, K_TRAVERSE_MFMA, in->staged_bytes * in->d.npatterns,
+                 in->staged_flops * in->d.npatterns);
+    }
+    if (mode == MODE_PARTIALS) return PLK_OK;
+    return launch_root_stream<T, N, C>(ctx, in, mode, node, dad);
+}
+
+template <typename T>
+plk_status launch_traverse_protein(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
+                                   plk_end dad) {
+    switch (in->d.ncat) {
+    case 1: return launch_traverse_mfma_c<T, 1>(ctx, in, nops, mode, node, dad);
+    case 2: return launch_traverse_mfma_c<T, 2>(ctx, in, nops, mode, node, dad);
+    case 4: return launch_traverse_mfma_c<T, 4>(ctx, in, nops, mode, node, dad);
+    case 8: return launch_traverse_mfma_c<T, 8>(ctx, in, nops, mode, node, dad);
     }
     return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
 }
@@ -501,6 +533,9 @@ plk_status launch_traverse(plk_context* ctx, Instance* in, int nops, int mode, p
     if (in->d.nstates == kCodonN)
         return f32 ? launch_traverse_codon<float>(ctx, in, nops, mode, node, dad)
                    : launch_traverse_codon<double>(ctx, in, nops, mode, node, dad);
+    if (in->d.nstates == 20)
+        return f32 ? launch_traverse_protein<float>(ctx, in, nops, mode, node, dad)
+                   : launch_traverse_protein<double>(ctx, in, nops, mode, node, dad);
     if (f32) return launch_traverse_p<float>(ctx, in, nops, mode, node, dad);
     return launch_traverse_p<double>(ctx, in, nops, mode, node, dad);
 }

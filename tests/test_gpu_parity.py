@@ -73,6 +73,47 @@ def test_protein_g4(ctx):
     all_branch_parity(w, *setup(ctx, w), stride=5)
 
 
+@pytest.mark.parametrize("safe", [False, True])
+def test_protein_scaling_counts_bit_exact(ctx, safe):
+    """Protein partials on the matrix cores cross 2^-256 (150 taxa, long
+    branches): counters bit-exact in normal (per pattern) and safe (per
+    category) mode."""
+    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=150, nsite=400, ncat=4, mean_len=0.5, alpha=1.0,
+                    seed=17)
+    planner, inst, orc = setup(ctx, w, safe=safe)
+    a, b = planner.branches()[0]
+    ops, node, dad, length = planner.plan(a, b)
+    lg = inst.likelihood_branch(ops, node, dad, length)
+    lo = orc.likelihood_branch(ops, node, dad, length)
+    assert abs(lg - lo) <= LNL_RTOL * abs(lo)
+    assert max(orc.read_partial(int(op["dad_slot"]))[1].max() for op in ops) >= 1
+    compare_partials(inst, orc, ops, tol=1e-9)
+    inst.likelihood_buffer(None, node, dad)
+    orc.likelihood_buffer(None, node, dad)
+    dg, ddg, _ = inst.likelihood_derv(length)
+    do, ddo = orc.likelihood_derv(length)
+    assert abs(dg - do) <= DERV_RTOL * max(1.0, abs(do))
+    assert abs(ddg - ddo) <= DERV_RTOL * max(1.0, abs(ddo))
+
+
+@pytest.mark.parametrize("ncat", [1, 2, 8])
+def test_protein_category_counts(ctx, ncat):
+    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=14, nsite=700, ncat=ncat, mean_len=0.2, alpha=0.8,
+                    seed=40 + ncat)
+    all_branch_parity(w, *setup(ctx, w), stride=5)
+
+
+def test_protein_fp32_storage(ctx):
+    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=40, nsite=3000, ncat=4, mean_len=0.2, alpha=0.8,
+                    seed=20261208)
+    planner, inst32, orc = setup(ctx, w, precision=ia.PREC_F32)
+    for a, b in planner.branches()[::9]:
+        ops, node, dad, length = planner.plan(a, b)
+        lg = inst32.likelihood_branch(ops, node, dad, length)
+        lo = orc.likelihood_branch(ops, node, dad, length)
+        assert abs(lg - lo) <= 1e-6 * abs(lo), (a, b, lg, lo, abs(lg - lo))
+
+
 def test_codon_gy_g4(ctx):
     """61-state GY94+I+G4 on the matrix-core path (per-category scaling)."""
     w = ia.Workload(seq_type=ia.CODON, ntaxa=14, nsite=800, ncat=4, mean_len=0.1, alpha=0.5,

@@ -356,9 +356,17 @@ def main():
         f"lnL {lnls[-1][0]:.6f} -> {lnls[-1][1]:.6f}")
     trav_s, trav_lnl = time_traversal(ctx, tree, 5)
 
-    # dominant kernel and its roofline
+    # dominant kernel and its roofline; HBM traffic per launch from the
+    # committed rocprofv3 PMC summary of the same kernel on this workload
     dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
     achieved = dom["bytes"] / (dom["total_ms"] * 1e-3)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath) and (args.taxa, args.sites) == (200, 100000):
+        with open(tpath) as f:
+            entry = json.load(f).get(dom_name)
+        if entry:
+            traffic, traffic_src = entry["traffic_bytes"], entry["source"]
     kernels = {k: dict(launches=v["launches"], ms=round(v["total_ms"], 3),
                        avg_us=round(1e3 * v["total_ms"] / v["launches"], 2),
                        gbps=round(v["bytes"] / max(v["total_ms"], 1e-9) * 1e-6, 1))
@@ -418,7 +426,8 @@ def main():
                    "traversal_site_lik_evals_per_s": P / trav_s},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None, "avg_launch_us": 1e3 * dom["total_ms"] / dom["launches"],
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "avg_launch_us": 1e3 * dom["total_ms"] / dom["launches"],
                      "bytes_per_launch": dom["bytes"] / dom["launches"]},
         "kernels": kernels,
         "fp32_storage": fp32,

@@ -1073,14 +1073,17 @@ plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& gri
     const int PB = kThreads / C;
     const int ntiles = in->Ppad / PB;
     int grid = ctx->num_cu * std::min(nb, ctx->sweep_blocks_per_cu);
-    grid = std::max(1, std::min(grid, ntiles));
-    plk_status st = grow(ctx, in, &in->sw_part, in->sw_part_cap, 2 * grid);
+    grid = std::max(1, std::min(grid, std::min(ntiles, 2 * kThreads)));   // the exchange polls 2 per thread
+    // the same number of tiles for every workgroup: each exchange waits for
+    // the busiest one, so extra workgroups with a tile less only add slots
+    plk_status st = grow(ctx, in, &in->sw_part, in->sw_part_cap, 6 * grid);   // three slot buffers
     if (st != PLK_OK) return st;
-    const int nflags = (grid + 1) * kFlagStride;
+    HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(in->sw_part), kSlotEmptyWord,
+                              12 * (size_t)grid, ctx->stream));
+    const int nflags = 2 * kFlagStride;
     if ((st = grow(ctx, in, &in->sw_flags, in->sw_flags_cap, nflags)) != PLK_OK) return st;
     HIP_TRY(hipMemsetAsync(in->sw_flags, 0, sizeof(unsigned) * nflags, ctx->stream));
     a.part = in->sw_part;
-    a.arrive = in->sw_flags + kFlagStride;
     a.release = in->sw_flags;
     a.result = in->sw_out + 2;
     void* args[] = {&a};

@@ -6,18 +6,24 @@
 //
 // Every workgroup owns a fixed set of pattern tiles for the whole launch, so
 // partial updates and theta only read values the same thread wrote; the only
-// grid-wide exchange is one reduction per likelihood evaluation:
-//   - each workgroup stores its partial sums and then its own arrival flag
-//     (one cache line each, no shared counter);
-//   - workgroup 0 polls every flag in parallel, adds the partial sums in a
-//     fixed order, stores the result and then a release flag;
-//   - every workgroup polls the release flag and reads the result.
-// Everything exchanged goes through device-scope atomic loads and stores, so
-// it is coherent across the XCDs' L2 caches without writing the caches back;
-// a store of data is acknowledged (s_waitcnt) before the flag that publishes
-// it is stored, and a reader issues its data load only after it has seen the
-// flag. Flags carry an epoch that grows by one per reduction. All workgroups
-// therefore hold bit-identical (df, ddf) and take identical Newton decisions.
+// grid-wide exchange is one all-reduce per likelihood evaluation, with no
+// flags: the partial sums themselves are polled.
+//   - exchange e uses slot buffer e % 3; a slot holds a sentinel bit pattern
+//     (a signalling NaN that no arithmetic produces) until its owner stores
+//     its partial sum there;
+//   - every workgroup polls the slots of all workgroups together until none
+//     holds the sentinel and adds them in one fixed order, so all workgroups
+//     hold the same bits and take identical Newton decisions;
+//   - right after exchange e a workgroup restores the sentinel in its own
+//     slots of buffer (e+2) % 3, last used by exchange e-1: every workgroup
+//     has read e-1, since it published e only after reading e-1. Before its
+//     exchange-(e+1) store a workgroup waits for its own earlier stores to be
+//     acknowledged (s_waitcnt; by then long done), so anyone who sees its e+1
+//     value also sees the restored sentinel of buffer (e+2) % 3.
+// Values go through device-scope atomic loads and stores, coherent across the
+// XCDs' L2 caches without writing caches back. (Measured alternatives that
+// were slower: a master workgroup reducing and broadcasting; arrival flags
+// next to the values, one round trip more per exchange.)
 // Every spin is bounded: a workgroup that waits too long raises a shared
 // abort flag, every spinning workgroup checks it, and all of them leave the
 // kernel, which then reports an error.
@@ -27,7 +33,9 @@
 namespace plkdev {
 
 constexpr long kSpinLimit = 1L << 24;   // ~0.5 s of polling before giving up
-constexpr int kFlagStride = 32;         // one 128-byte line per arrival flag
+constexpr int kFlagStride = 32;         // one 128-byte line per flag
+constexpr unsigned long long kSlotEmpty = 0x7FF4DEAD7FF4DEADull;   // signalling NaN
+constexpr unsigned kSlotEmptyWord = 0x7FF4DEADu;                   // its 32-bit halves
 
 struct SweepArgs {
     InstDev I;
@@ -40,10 +48,10 @@ struct SweepArgs {
     double min_len, max_len;
     int max_nr;
     int table_size;
-    double* part;         // [gridDim.x][2] per-workgroup partial sums
-    unsigned* arrive;     // [gridDim.x * kFlagStride] arrival epochs (zeroed per launch)
-    unsigned* release;    // [kFlagStride] release epoch, [1] abort flag
-    double* result;       // [2] reduced sums
+    double* part;         // [3][gridDim.x][2] partial-sum slots (kSlotEmpty per launch)
+    unsigned* arrive;     // unused
+    unsigned* release;    // [1] abort flag (zeroed per launch)
+    double* result;       // unused
     int* nderv;           // [nbranches]
     double* out;          // [0] lnL from buffer after the sweep, [1] status
     long long* timing;    // nullable: wall-clock ticks of workgroup 0 per phase
@@ -70,10 +78,29 @@ __device__ inline double load_value(const double* p) {
 // every store this thread issued has been acknowledged
 __device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0); }
 
-// bounded wait until *flag == epoch; false on timeout or abort
-__device__ inline bool wait_epoch(const unsigned* flag, unsigned epoch, unsigned* abort_flag) {
+__device__ inline unsigned long long load_bits(const double* p) {
+    return __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_bits(double* p, unsigned long long v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait until the n (<= 4) slots hold values, all polled together;
+// false on timeout or abort.
+__device__ inline bool wait_slots(const double* const* slot, int n, double* out, unsigned* abort_flag) {
     for (long it = 0; it < kSpinLimit; it++) {
-        if (load_flag(flag) == epoch) return true;
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k < n) {
+                const unsigned long long b = load_bits(slot[k]);
+                out[k] = __longlong_as_double((long long)b);
+                all = all && b != kSlotEmpty;
+            }
+        }
+        if (all) return true;
         if ((it & 31) == 31 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return false;
         __builtin_amdgcn_s_sleep(1);
@@ -82,59 +109,69 @@ __device__ inline bool wait_epoch(const unsigned* flag, unsigned epoch, unsigned
     return false;
 }
 
+// Fixed-order sum over the workgroup's 256 partial sums; res[0..1] in LDS.
+// The xor butterfly gives every lane the same bits (each level adds the same
+// two operands, in either order), the four wave sums are added in wave order.
+__device__ inline void tree_sum2(double s0, double s1, double* red_sh, double* res) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        s0 += __shfl_xor(s0, d);
+        s1 += __shfl_xor(s1, d);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red_sh[2 * wave] = s0;
+        red_sh[2 * wave + 1] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r0 = red_sh[0], r1 = red_sh[1];
+        for (int w = 1; w < kThreads / 64; w++) {
+            r0 += red_sh[2 * w];
+            r1 += red_sh[2 * w + 1];
+        }
+        res[0] = r0;
+        res[1] = r1;
+    }
+    __syncthreads();
+}
+
 // Grid-wide fixed-order sum of two per-workgroup values (valid in thread 0
-// on entry); every workgroup gets the same res[0..1]. Returns false when the
-// exchange was aborted.
+// on entry); every workgroup computes the same res[0..1]. Returns false when
+// the exchange was aborted.
 __device__ inline bool grid_sum2(const SweepArgs& a, double v0, double v1, unsigned epoch,
                                  double* red_sh, double* res) {
     __shared__ int ok_sh;
+    const int G = gridDim.x;
+    double* buf = a.part + (size_t)(epoch % 3) * 2 * G;
     if (threadIdx.x == 0) {
-        store_value(&a.part[2 * blockIdx.x + 0], v0);
-        store_value(&a.part[2 * blockIdx.x + 1], v1);
-        stores_done();
-        store_flag(&a.arrive[blockIdx.x * kFlagStride], epoch);
+        stores_done();   // the sentinel restored after the previous exchange is visible
+        store_value(&buf[2 * blockIdx.x + 0], v0);
+        store_value(&buf[2 * blockIdx.x + 1], v1);
         ok_sh = 1;
     }
     __syncthreads();
     unsigned* abort_flag = &a.release[1];
-    if (blockIdx.x == 0) {
-        bool ok = true;
-        double s0 = 0.0, s1 = 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += kThreads) {
-            if (!wait_epoch(&a.arrive[b * kFlagStride], epoch, abort_flag)) { ok = false; break; }
-            s0 += load_value(&a.part[2 * b + 0]);
-            s1 += load_value(&a.part[2 * b + 1]);
-        }
-        if (!ok) ok_sh = 0;
-        __syncthreads();
-        if (ok_sh) {
-            red_sh[threadIdx.x] = s0;
-            red_sh[kThreads + threadIdx.x] = s1;
-            __syncthreads();
-            for (int w = kThreads / 2; w >= 1; w >>= 1) {
-                if ((int)threadIdx.x < w) {
-                    red_sh[threadIdx.x] += red_sh[threadIdx.x + w];
-                    red_sh[kThreads + threadIdx.x] += red_sh[kThreads + threadIdx.x + w];
-                }
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) {
-                store_value(&a.result[0], red_sh[0]);
-                store_value(&a.result[1], red_sh[kThreads]);
-                stores_done();
-                store_flag(&a.release[0], epoch);
-            }
-        }
+    // this thread's workgroups b = tid, tid + 256 (gridDim.x <= 512)
+    const int b0 = threadIdx.x, b1 = threadIdx.x + kThreads;
+    const double* slot[4] = {&buf[2 * b0], &buf[2 * b0 + 1], &buf[2 * b1], &buf[2 * b1 + 1]};
+    const int n = b1 < G ? 4 : (b0 < G ? 2 : 0);
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool ok = wait_slots(slot, n, x, abort_flag);
+    double s0 = 0.0, s1 = 0.0;
+    if (ok) {
+        if (n >= 2) { s0 += x[0]; s1 += x[1]; }
+        if (n == 4) { s0 += x[2]; s1 += x[3]; }
     }
-    if (threadIdx.x == 0 && ok_sh) {
-        if (wait_epoch(&a.release[0], epoch, abort_flag)) {
-            res[0] = load_value(&a.result[0]);
-            res[1] = load_value(&a.result[1]);
-        } else {
-            ok_sh = 0;
-        }
+    if (!ok) ok_sh = 0;
+    tree_sum2(s0, s1, red_sh, res);
+    if (threadIdx.x == 0) {
+        // every workgroup has read exchange e-1: re-arm this workgroup's
+        // slots of its buffer for exchange e+2
+        double* again = a.part + (size_t)((epoch + 2) % 3) * 2 * G;
+        store_bits(&again[2 * blockIdx.x + 0], kSlotEmpty);
+        store_bits(&again[2 * blockIdx.x + 1], kSlotEmpty);
     }
-    __syncthreads();
     return ok_sh != 0;
 }
 
@@ -267,6 +304,75 @@ __device__ inline bool sweep_eval(const SweepArgs& a, int mode, double x, double
     return ok;
 }
 
+// theta of branch (node, dad) and, from the same registers, the first
+// Newton evaluation (df, ddf) at x: the values summed are the stored theta
+// values, so the result equals a separate theta pass followed by sweep_eval
+template <typename T, int N, int C>
+__device__ inline bool theta_first_eval(const SweepArgs& a, plk_end node, plk_end dad, double x,
+                                        unsigned& epoch, const double* tip_sh, double* val_sh,
+                                        double* red_sh, double* res) {
+    const long long t0 = wall_clock64();
+    constexpr int PB = kThreads / C;
+    constexpr int block = N * C;
+    const InstDev& I = a.I;
+    for (int t = threadIdx.x; t < N * C; t += kThreads) {
+        const int cc = t / N, i = t % N;
+        const double cof = a.m.eval[i] * a.m.rates[cc];
+        const double v0 = exp(cof * x) * a.m.props[cc];
+        const double v1 = cof * v0;
+        val_sh[t] = v0;
+        val_sh[N * C + t] = v1;
+        val_sh[2 * N * C + t] = cof * v1;
+    }
+    __syncthreads();
+    const int c = threadIdx.x % C;
+    const int ntiles = I.Ppad / PB;
+    double acc[2] = {0.0, 0.0};
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int p = tile * PB + threadIdx.x / C;
+        double nv[N], dv[N], adjust;
+        int minsc;
+        load_branch<T, N, C>(I, node, dad, tip_sh, p, c, nv, dv, minsc, adjust);
+        T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
+        const double* v0 = val_sh + c * N;
+        const double* v1 = val_sh + N * C + c * N;
+        const double* v2 = val_sh + 2 * N * C + c * N;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const T tv = (T)(nv[i] * dv[i] * adjust);
+            th[i] = tv;
+            const double t = (double)tv;
+            s0 = fma(v0[i], t, s0);
+            s1 = fma(v1[i], t, s1);
+            s2 = fma(v2[i], t, s2);
+        }
+        if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
+        s0 = group_sum_ordered<C>(s0);
+        s1 = group_sum_ordered<C>(s1);
+        s2 = group_sum_ordered<C>(s2);
+        if (c == 0 && p < I.P) {
+            const double f = I.freq[p];
+            const double lh = fabs(s0) + I.invar[p];
+            const double inv = 1.0 / lh;
+            const double df_frac = s1 * inv, ddf_frac = s2 * inv;
+            const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
+            acc[0] += tmp1;
+            acc[1] += tmp2 - tmp1 * df_frac;
+        }
+    }
+    block_sum<2>(acc, red_sh);
+    epoch++;
+    const long long t1 = wall_clock64();
+    const bool ok = grid_sum2(a, acc[0], acc[1], epoch, red_sh, res);
+    if (a.timing && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long t2 = wall_clock64();
+        a.timing[PH_THETA] += t1 - t0;
+        a.timing[PH_EXCHANGE] += t2 - t1;
+    }
+    return ok;
+}
+
 template <typename T, int N, int C>
 __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -312,18 +418,10 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
             }
             __syncthreads();
         }
-        if (timed) {
-            const long long now = wall_clock64();
-            a.timing[PH_OPS] += now - tp;
-            tp = now;
-        }
-        for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-            theta_apply<T, N, C>(I, B.node, B.dad, tip_sh, tile * PB + threadIdx.x / C, c);
-        if (timed) a.timing[PH_THETA] += wall_clock64() - tp;
-
+        if (timed) a.timing[PH_OPS] += wall_clock64() - tp;
         // minimizeNewton(min_len, len, max_len, xacc = min_len, max_nr) on
         // f = -dlnL/dt (PhyloTree::computeFuncDerv); every thread holds the
-        // same scalars
+        // same scalars. The first evaluation comes with theta.
         const double len0 = lens[B.edge];
         int nd = 0;
         double f, df, dx, dxold, xl, xh, rts, rts_old, result = 0.0;
@@ -331,7 +429,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         rts = len0;
         if (rts < x1) rts = x1;
         if (rts > x2) rts = x2;
-        if (!sweep_eval<T, N, C>(a, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+        if (!theta_first_eval<T, N, C>(a, B.node, B.dad, rts, epoch, tip_sh, val_sh, red_sh, ctl)) {
             status = PLK_ERR_DEVICE;
             break;
         }

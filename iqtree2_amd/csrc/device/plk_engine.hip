@@ -615,14 +615,16 @@ plk_status stage_ops(plk_context* ctx, Instance* in, const plk_op* ops, int nops
         in->staged_bytes += b;
         in->staged_flops += f;
     }
-    const size_t smem = (size_t)in->d.ncat * in->d.nstates * sizeof(double);
+    const size_t n = in->d.nstates, SU = in->d.state_unknown;
+    const size_t smem = (n + n * n + SU * n) * sizeof(double);
+    const dim3 grid(2 * nops, in->d.ncat);
     const int pe = prof_begin(ctx);
     if (in->d.precision == PLK_PREC_F32) {
-        hipLaunchKernelGGL(k_prep_ops<float>, dim3(2 * nops), dim3(kThreads), smem, ctx->stream,
+        hipLaunchKernelGGL(k_prep_ops<float>, grid, dim3(kThreads), smem, ctx->stream,
                            in->ops, nops, model_dev(in), in->d.nstates, in->d.ncat,
                            in->d.state_unknown, in->table_size, reinterpret_cast<float*>(in->tables));
     } else {
-        hipLaunchKernelGGL(k_prep_ops<double>, dim3(2 * nops), dim3(kThreads), smem, ctx->stream,
+        hipLaunchKernelGGL(k_prep_ops<double>, grid, dim3(kThreads), smem, ctx->stream,
                            in->ops, nops, model_dev(in), in->d.nstates, in->d.ncat,
                            in->d.state_unknown, in->table_size, reinterpret_cast<double*>(in->tables));
     }

@@ -84,43 +84,47 @@ enum BufferMode { BUF_DERV = 0, BUF_LNL = 1 };
 // eigen-basis transition rows echild[c][x][i] = evec[x][i] exp(eval[i] r_c t)
 // for an internal child, or the tip table leaf[s][c][x] =
 // sum_i echild[c][x][i] tip[s][i] (leaf[SU] = 1) for a tip child.
-// One workgroup per (op, child); tables are written in storage precision.
+// One workgroup per (op, child, category); tables are written in storage
+// precision.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_prep_ops(const plk_op* __restrict__ ops, int nops, ModelDev m, int n, int C, int SU,
            int table_size, T* __restrict__ tables) {
-    extern __shared__ double ex[];   // [C][n]
-    const int o = blockIdx.x >> 1, k = blockIdx.x & 1;
+    // one block per (op, child, category); LDS: ex[n], then for a tip child
+    // U[n][n] and the tip vectors [SU][n]
+    extern __shared__ double sh[];
+    const int o = blockIdx.x >> 1, k = blockIdx.x & 1, c = blockIdx.y;
     if (o >= nops) return;
     const plk_child ch = ops[o].child[k];
-    for (int t = threadIdx.x; t < C * n; t += blockDim.x) {
-        const int c = t / n, i = t % n;
-        ex[t] = exp(m.eval[i] * (m.rates[c] * ch.length));
-    }
-    __syncthreads();
+    double* ex = sh;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ex[i] = exp(m.eval[i] * (m.rates[c] * ch.length));
     T* out = tables + (size_t)(2 * o + k) * table_size;
     if (!ch.is_tip) {
-        const int total = C * n * n;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            const int i = t % n, x = (t / n) % n, c = t / (n * n);
-            out[t] = (T)(m.evec[x * n + i] * ex[c * n + i]);
+        __syncthreads();
+        T* oc = out + (size_t)c * n * n;
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            const int i = t % n, x = t / n;
+            oc[t] = (T)(m.evec[x * n + i] * ex[i]);
         }
-    } else {
-        const int total = (SU + 1) * C * n;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            const int x = t % n, c = (t / n) % C, s = t / (n * C);
-            double v;
-            if (s == SU) {
-                v = 1.0;
-            } else {
-                const double* ev = m.evec + x * n;
-                const double* tp = m.tip + s * n;
-                const double* e = ex + c * n;
-                v = ev[0] * e[0] * tp[0];
-                for (int i = 1; i < n; i++) v += ev[i] * e[i] * tp[i];
-            }
-            out[t] = (T)v;
+        return;
+    }
+    double* ev_sh = ex + n;
+    double* tp_sh = ev_sh + n * n;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) ev_sh[t] = m.evec[t];
+    for (int t = threadIdx.x; t < SU * n; t += blockDim.x) tp_sh[t] = m.tip[t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < (SU + 1) * n; t += blockDim.x) {
+        const int x = t % n, s = t / n;
+        double v;
+        if (s == SU) {
+            v = 1.0;
+        } else {
+            const double* ev = ev_sh + x * n;
+            const double* tp = tp_sh + s * n;
+            v = ev[0] * ex[0] * tp[0];
+            for (int i = 1; i < n; i++) v += ev[i] * ex[i] * tp[i];
         }
+        out[(size_t)(s * C + c) * n + x] = (T)v;
     }
 }
 

@@ -10,8 +10,9 @@
 // followed by the per-(pattern, category) rescale when max_i |dad| < 2^-256
 // (FP32 storage: the exact power-of-two rescale of the float form).
 //
-// Work split: a workgroup of four waves owns 64 patterns, each wave 16 of
-// them as the N dimension of 16x16x4 MFMA tiles; the 61 states pad to 64 =
+// Work split: a workgroup of four waves owns 64 patterns of one rate
+// category (safe mode keeps a scale counter per category, so categories are
+// independent), each wave 16 patterns as the N dimension of 16x16x4 MFMA tiles; the 61 states pad to 64 =
 // four M tiles and sixteen K steps. E_child[c] and U^-1 sit in LDS (row
 // stride 65 elements); child vectors are read straight from HBM into the B
 // operand. In the C/D layout of the phase-A product a lane holds one
@@ -83,7 +84,8 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
                           tables + (size_t)(2 * o + right) * table_size};
         const bool tiptip = ch[0].is_tip && ch[1].is_tip;
         T* dst = partial + (size_t)op.dad_slot * I.slot_stride;
-        for (int c = 0; c < C; c++) {
+        {
+            const int c = blockIdx.y;   // per-category scaling makes categories independent
             acc_t prod[4];   // tmp[x = 16*xb + row(g, r)][pattern pl] in prod[xb][r]
             for (int k = 0; k < 2; k++) {
                 acc_t a[4];

@@ -477,9 +477,9 @@ template <typename T>
 plk_status launch_traverse_codon(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
                                  plk_end dad) {
     if (nops > 0) {
-        const int grid = in->Ppad / kCodonTile;
+        const dim3 grid(in->Ppad / kCodonTile, in->d.ncat);
         const int pe = prof_begin(ctx);
-        hipLaunchKernelGGL(k_traverse_codon<T>, dim3(grid), dim3(kThreads), 0, ctx->stream,
+        hipLaunchKernelGGL(k_traverse_codon<T>, grid, dim3(kThreads), 0, ctx->stream,
                            inst_dev(in), model_dev(in), in->ops, nops,
                            reinterpret_cast<const T*>(in->tables), in->table_size);
         HIP_TRY(hipGetLastError());

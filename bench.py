@@ -191,6 +191,41 @@ def cpu_partitions(ia, parts, plans, budget_s, threads):
                 sample=f"{n} of {len(parts)} partitions, full traversal + lnL each ({elapsed:.1f} s)")
 
 
+def bench_mfma(ia, ctx, reps=3):
+    """Config 3 (protein LG-like+G4, 100 taxa x 20,000 sites, Exp(0.2)
+    lengths, alpha 0.8) and a codon GY94+G4 case (64 taxa x 5,000 codons):
+    full post-order traversals with the tree lnL on the matrix-core kernels,
+    FP64 and FP32 storage, with |dlnL| of FP32 against FP64."""
+    cases = [("protein_100x20k", ia.Workload(seq_type=ia.PROTEIN, ntaxa=100, nsite=20000, ncat=4,
+                                               mean_len=0.2, alpha=0.8, seed=20261202 + 2)),
+             ("codon_64x5k", ia.Workload(seq_type=ia.CODON, ntaxa=64, nsite=5000, ncat=4,
+                                           mean_len=0.1, alpha=0.5, seed=77))]
+    out = {}
+    for name, w in cases:
+        res = {"patterns": w.npatterns}
+        lnl = {}
+        for prec, pn in ((ia.PREC_F64, "fp64"), (ia.PREC_F32, "fp32")):
+            tree = ia.PhyloTree(ctx, w, precision=prec)
+            tree.compute_likelihood()
+            ctx.profile_start()
+            t = time.perf_counter()
+            for _ in range(reps):
+                tree.clear_all_partial()
+                lnl[pn] = tree.compute_likelihood()
+            ctx.synchronize()
+            dt = (time.perf_counter() - t) / reps
+            k = ctx.profile_stop()["traverse_mfma"]
+            res[pn] = {"traversal_ms": 1e3 * dt,
+                       "site_lik_evals_per_s": w.npatterns / dt,
+                       "kernel_ms": k["total_ms"] / k["launches"],
+                       "tflops": k["flops"] / k["total_ms"] * 1e-9,
+                       "gbps": k["bytes"] / k["total_ms"] * 1e-6}
+            tree.close()
+        res["rel_dlnl_fp32"] = abs(lnl["fp32"] - lnl["fp64"]) / abs(lnl["fp64"])
+        out[name] = res
+    return out
+
+
 def make_requests(ia, nreq, seed=20261202 + 4):
     """Config 5: per request N ~ log-uniform[8, 256] taxa and L ~
     log-uniform[500, 50,000] sites on a seeded Exp(0.1) tree; DNA GTR+G4
@@ -333,6 +368,9 @@ def main():
     ap.add_argument("--no-partitions", action="store_true")
     ap.add_argument("--requests", type=int, default=160)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true")
+    ap.add_argument("--mfma-only", action="store_true",
+                    help="only the protein/codon traversal leg (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -340,6 +378,11 @@ def main():
         sys.exit(2)
 
     import iqtree2_amd as ia
+    if args.mfma_only:
+        ctx = ia.Context(0)
+        print(json.dumps({"mfma_traversals": bench_mfma(ia, ctx)}), flush=True)
+        ctx.close()
+        return
     t0 = time.perf_counter()
     w = make_workload(ia, args)
     log(f"workload: {w.ntaxa} taxa, {w.nsite} sites, {w.npatterns} patterns "
@@ -394,6 +437,11 @@ def main():
                                  min(16, os.cpu_count() or 1))
         log(f"partitions: {json.dumps(parts)}")
 
+    mfma = None
+    if not args.no_mfma:
+        mfma = bench_mfma(ia, ctx)
+        log(f"mfma traversals: {json.dumps(mfma)}")
+
     serving = None
     if not args.no_serving:
         serving = bench_serving(ia, ctx, args.requests)
@@ -432,6 +480,7 @@ def main():
         "kernels": kernels,
         "fp32_storage": fp32,
         "partition_batch": parts,
+        "mfma_traversals": mfma,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -5,12 +5,12 @@ C ABI, include/plk.h); this package is its Python host mirror.
 """
 from . import _capi
 from ._capi import PREC_F32, PREC_F64, PlkError, lib
-from .engine import (Context, Instance, Patterns, PhyloTree, Workload, end,
+from .engine import (BatchPlan, Context, Instance, Patterns, PhyloTree, Workload, end,
                      batch_evaluate, likelihood_branch_batch, likelihood_derv_batch,
                      ops_array)
 
 DNA, PROTEIN, CODON = _capi.DNA, _capi.PROTEIN, _capi.CODON
 
-__all__ = ["Context", "Instance", "Patterns", "PhyloTree", "Workload", "end", "ops_array",
+__all__ = ["BatchPlan", "Context", "Instance", "Patterns", "PhyloTree", "Workload", "end", "ops_array",
            "likelihood_branch_batch", "likelihood_derv_batch", "batch_evaluate",
            "PREC_F32", "PREC_F64", "PlkError", "lib", "DNA", "PROTEIN", "CODON"]

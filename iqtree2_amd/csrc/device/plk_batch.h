@@ -39,40 +39,38 @@ struct PrepJob {
 };
 
 template <typename T>
-__device__ inline void prep_table(const BatchItem& it, int o, int k, double* ex) {
+__device__ inline void prep_table(const BatchItem& it, int o, int k, int c, double* ex) {
     const int n = it.n, C = it.C, SU = it.I.SU;
     const plk_child ch = it.ops[o].child[k];
-    for (int t = threadIdx.x; t < C * n; t += blockDim.x) {
-        const int c = t / n, i = t % n;
-        ex[t] = exp(it.m.eval[i] * (it.m.rates[c] * ch.length));
-    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ex[i] = exp(it.m.eval[i] * (it.m.rates[c] * ch.length));
     __syncthreads();
     T* out = reinterpret_cast<T*>(const_cast<void*>(it.tables)) + (size_t)(2 * o + k) * it.table_size;
     if (!ch.is_tip) {
-        for (int t = threadIdx.x; t < C * n * n; t += blockDim.x) {
-            const int i = t % n, x = (t / n) % n, c = t / (n * n);
-            out[t] = (T)(it.m.evec[x * n + i] * ex[c * n + i]);
+        T* oc = out + (size_t)c * n * n;
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            const int i = t % n, x = t / n;
+            oc[t] = (T)(it.m.evec[x * n + i] * ex[i]);
         }
     } else {
-        for (int t = threadIdx.x; t < (SU + 1) * C * n; t += blockDim.x) {
-            const int x = t % n, c = (t / n) % C, s = t / (n * C);
+        for (int t = threadIdx.x; t < (SU + 1) * n; t += blockDim.x) {
+            const int x = t % n, s = t / n;
             double v;
             if (s == SU) {
                 v = 1.0;
             } else {
                 const double* ev = it.m.evec + x * n;
                 const double* tp = it.m.tip + s * n;
-                const double* e = ex + c * n;
-                v = ev[0] * e[0] * tp[0];
-                for (int i = 1; i < n; i++) v += ev[i] * e[i] * tp[i];
+                v = ev[0] * ex[0] * tp[0];
+                for (int i = 1; i < n; i++) v += ev[i] * ex[i] * tp[i];
             }
-            out[t] = (T)v;
+            out[(size_t)(s * C + c) * n + x] = (T)v;
         }
     }
     __syncthreads();
 }
 
-// one workgroup per (item, op); root tables of every item by extra workgroups
+// workgroups (item, op, category row); root tables of every item by extra
+// workgroups of row 0
 __global__ void __launch_bounds__(kThreads)
 k_prep_batch(const BatchItem* __restrict__ items, const PrepJob* __restrict__ jobs, int njobs,
              int nitems, const double* __restrict__ lengths, double* __restrict__ roots) {
@@ -80,14 +78,19 @@ k_prep_batch(const BatchItem* __restrict__ items, const PrepJob* __restrict__ jo
     if ((int)blockIdx.x < njobs) {
         const PrepJob j = jobs[blockIdx.x];
         const BatchItem& it = items[j.item];
-        for (int k = 0; k < 2; k++) {
-            if (it.f32) prep_table<float>(it, j.op, k, ex);
-            else prep_table<double>(it, j.op, k, ex);
-        }
+        // 4 states: one workgroup builds every category's (small) tables;
+        // larger state spaces split the categories over the grid's rows
+        const int c0 = it.n == 4 ? (blockIdx.y == 0 ? 0 : it.C) : (int)blockIdx.y;
+        const int dc = it.n == 4 ? 1 : (int)gridDim.y;
+        for (int c = c0; c < it.C; c += dc)
+            for (int k = 0; k < 2; k++) {
+                if (it.f32) prep_table<float>(it, j.op, k, c, ex);
+                else prep_table<double>(it, j.op, k, c, ex);
+            }
         return;
     }
     const int i = blockIdx.x - njobs;
-    if (i >= nitems) return;
+    if (i >= nitems || blockIdx.y != 0) return;
     const BatchItem& it = items[i];
     const int n = it.n, C = it.C, nv = C * n, SU = it.I.SU;
     double* out = const_cast<double*>(it.root);
@@ -103,23 +106,87 @@ k_prep_batch(const BatchItem* __restrict__ items, const PrepJob* __restrict__ jo
     }
 }
 
+// Branch ends streamed one state at a time (20 states keep no per-thread
+// state arrays), with the arithmetic of load_branch + the root / theta code
+// of k_traverse and k_root_stream. MODE_BRANCH returns the pattern's
+// category sum (every lane) and the scale minimum; MODE_THETA writes theta
+// and buffer_scale (tip dads read tip_eig).
+template <typename T, int N, int C>
+__device__ inline double branch_end_stream(const InstDev& I, plk_end node, plk_end dad, int mode,
+                                           const double* root_sh, int p, int c,
+                                           const double* __restrict__ tip_eig, int* minsc_out = nullptr) {
+    using ST = typename Prec<T>::scale_t;
+    constexpr int block = N * C;
+    const T* partial = reinterpret_cast<const T*>(I.partial);
+    const ST* scale = reinterpret_cast<const ST*>(I.scale);
+    const bool safe = I.safe != 0;
+    const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
+    const T* nv = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
+    int sc = scale[(size_t)node.id * I.scale_stride + sidx];
+    const T* dv = nullptr;
+    int s = 0;
+    if (dad.is_tip) {
+        s = I.states[(size_t)dad.id * I.Ppad + p];
+    } else {
+        dv = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
+        sc += scale[(size_t)dad.id * I.scale_stride + sidx];
+    }
+    int minsc = sc;
+    double adjust = 1.0;
+    if (safe) {
+        minsc = group_min_int<C>(sc);
+        adjust = Prec<T>::safe_adjust(sc - minsc);
+    }
+    if (mode == MODE_THETA) {
+        T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
+        if (dad.is_tip) {
+            const double* tp = tip_eig + s * N;
+#pragma unroll 4
+            for (int i = 0; i < N; i++) th[i] = (T)((double)nv[i] * tp[i] * adjust);
+        } else {
+#pragma unroll 4
+            for (int i = 0; i < N; i++) th[i] = (T)((double)nv[i] * (double)dv[i] * adjust);
+        }
+        if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
+        return 0.0;
+    }
+    double lhc;
+    if (dad.is_tip) {
+        const double* lut = root_sh + block + (s * C + c) * N;
+        lhc = lut[0] * (double)nv[0];
+#pragma unroll 1
+        for (int i = 1; i < N; i++) lhc = fma(lut[i], (double)nv[i], lhc);
+    } else {
+        const double* val = root_sh + c * N;
+        lhc = val[0] * (double)dv[0] * (double)nv[0];
+#pragma unroll 1
+        for (int i = 1; i < N; i++) lhc = fma(val[i] * (double)dv[i], (double)nv[i], lhc);
+    }
+    if (minsc_out) *minsc_out = minsc;
+    return safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
+}
+
+// patterns per work-list tile: one pattern per C threads (64 with four
+// categories; for 20 states four 16-pattern matrix-core tiles)
+__host__ __device__ constexpr int batch_tile_patterns(int, int C) { return kThreads / C; }
+
 // all ops of one tile of one item, then its root lnL; returns the tile's
 // lnL sum (thread 0)
 template <typename T, int N, int C>
-__device__ inline double batch_tile(const BatchItem& it, int tile, unsigned char* smem) {
-    constexpr int PB = kThreads / C;
+__device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsigned char* smem) {
+    constexpr int PB = batch_tile_patterns(N, C);
     constexpr int block = N * C;
     T* inv_sh = reinterpret_cast<T*>(smem);
     T* tab_sh = inv_sh + N * N;
     const InstDev& I = it.I;
     const int c = threadIdx.x % C;
-    const int p = tile * PB + threadIdx.x / C;
+    const int p = tile * PB + (int)threadIdx.x / C;
     const T* tables = reinterpret_cast<const T*>(it.tables);
     __syncthreads();
     if constexpr (N == 20) {
         // protein: the matrix-core tile of the lone traversal kernel, four
-        // 16-pattern sub-tiles per batch tile (PB = 64 with four categories)
-        static_assert(kThreads / C == 4 * kMfmaTile, "batch tile must be four MFMA tiles");
+        // 16-pattern sub-tiles per batch tile
+        static_assert(PB == 4 * kMfmaTile, "batch tile must be four MFMA tiles");
         using L = MfmaSmall<T, N, C>;
         T* inv_m = reinterpret_cast<T*>(smem);
         T* tab_m = inv_m + L::kInv;
@@ -154,29 +221,36 @@ __device__ inline double batch_tile(const BatchItem& it, int tile, unsigned char
         if (dad.is_tip)
             for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = it.m.tip[t];
         __syncthreads();
-        theta_apply<T, N, C>(I, node, dad, root_sh, p, c);
+        if constexpr (N == 20) branch_end_stream<T, N, C>(I, node, dad, MODE_THETA, root_sh, p, c, it.m.tip);
+        else theta_apply<T, N, C>(I, node, dad, root_sh, p, c);
         return 0.0;
     }
     const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
     for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = it.root[t];
     __syncthreads();
-    double nv[N], dv[N], adjust;
+    double lh0;
     int minsc;
-    load_branch<T, N, C>(I, node, dad, nullptr, p, c, nv, dv, minsc, adjust);
-    double lhc;
-    if (dad.is_tip) {
-        const int s = I.states[(size_t)dad.id * I.Ppad + p];
-        const double* lut = root_sh + block + (s * C + c) * N;
-        lhc = lut[0] * nv[0];
-#pragma unroll
-        for (int i = 1; i < N; i++) lhc = fma(lut[i], nv[i], lhc);
+    if constexpr (N == 20) {
+        // streamed (no per-thread state arrays), same arithmetic as below
+        lh0 = branch_end_stream<T, N, C>(I, node, dad, MODE_BRANCH, root_sh, p, c, it.m.tip, &minsc);
     } else {
-        const double* val = root_sh + c * N;
-        lhc = val[0] * dv[0] * nv[0];
+        double nv[N], dv[N], adjust;
+        load_branch<T, N, C>(I, node, dad, nullptr, p, c, nv, dv, minsc, adjust);
+        double lhc;
+        if (dad.is_tip) {
+            const int s = I.states[(size_t)dad.id * I.Ppad + p];
+            const double* lut = root_sh + block + (s * C + c) * N;
+            lhc = lut[0] * nv[0];
 #pragma unroll
-        for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
+            for (int i = 1; i < N; i++) lhc = fma(lut[i], nv[i], lhc);
+        } else {
+            const double* val = root_sh + c * N;
+            lhc = val[0] * dv[0] * nv[0];
+#pragma unroll
+            for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
+        }
+        lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
     }
-    const double lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
     double contrib = 0.0;
     if (c == 0) {
         const double lh = fabs(lh0) + I.invar[p];
@@ -211,9 +285,11 @@ k_batch(const BatchItem* __restrict__ items, const int* __restrict__ tile_item, 
         __syncthreads();
         if (threadIdx.x == 0) next_sh = tile_begin + (int)atomicAdd(counter, 1u);
         __syncthreads();
-        const int g = next_sh;
+        // wave-uniform by construction; readfirstlane lets the item's fields
+        // live in scalar registers
+        const int g = __builtin_amdgcn_readfirstlane(next_sh);
         if (g >= tile_end) break;
-        const int i = tile_item[g];
+        const int i = __builtin_amdgcn_readfirstlane(tile_item[g]);
         const BatchItem& it = items[i];
         const int tile = g - it.tile_begin;
         const int cls = batch_class(it.n, it.f32);

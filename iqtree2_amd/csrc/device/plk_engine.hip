@@ -1184,7 +1184,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         root_off[r] = root_doubles;
         root_doubles += (size_t)(in->d.state_unknown + 2) * in->block;
         for (int o = 0; o < items[i].nops; o++) jobs.push_back(PrepJob{r, o});
-        const int PB = kThreads / in->d.ncat;
+        const int PB = batch_tile_patterns(in->d.nstates, in->d.ncat);
         const int ntiles = (in->d.npatterns + PB - 1) / PB;
         BatchItem& b = hb[r];
         b.I = inst_dev(in);
@@ -1241,7 +1241,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     HIP_TRY(hipMemcpyAsync(ctx->bt_vals, lens.data(), sizeof(double) * nitems, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned), ctx->stream));
     int pe = prof_begin(ctx);
-    hipLaunchKernelGGL(k_prep_batch, dim3((int)jobs.size() + nitems), dim3(kThreads), smem_prep, ctx->stream,
+    hipLaunchKernelGGL(k_prep_batch, dim3((int)jobs.size() + nitems, 4), dim3(kThreads), smem_prep, ctx->stream,
                        ctx->bt_items, ctx->bt_jobs, (int)jobs.size(), nitems, ctx->bt_vals, ctx->bt_roots);
     HIP_TRY(hipGetLastError());
     prof_end(ctx, pe, K_PREP_BATCH, (double)tab_bytes, 0.0);
@@ -1307,7 +1307,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         float best = 1e30f;
-        const int splits[2] = {0, 1}, bpcs[3] = {1, 2, 4};
+        const int splits[2] = {0, 1}, bpcs[4] = {1, 2, 4, 8};
         for (int sp : splits) {
             if (sp == 1 && ranges.size() == 1) continue;
             for (int b : bpcs) {

@@ -83,6 +83,7 @@ __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict_
             }
         }
         __syncthreads();
+#pragma unroll 1
         for (int sub = 0; sub < SUB; sub++) {
         const int p0 = p_first + 16 * sub;
         const int pb = p0 + col;   // this lane's pattern as a B operand / A row

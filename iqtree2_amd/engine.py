@@ -292,43 +292,45 @@ class Instance:
         return part, scale
 
 
+class BatchPlan:
+    """A batch of (instance, ops, node, dad, length) jobs marshalled once for
+    repeated plk_batch_evaluate calls (the items reference the op arrays,
+    which the plan keeps alive). modes: None (all branch lnL) or per-job
+    0 (branch lnL) / 1 (theta buffer)."""
+
+    def __init__(self, ctx, jobs, modes=None):
+        self.ctx = ctx
+        self.n = len(jobs)
+        self.items = (_capi.PlkBatchItem * self.n)()
+        self._keep = []
+        for k, (inst, ops, node, dad, length) in enumerate(jobs):
+            arr = np.ascontiguousarray(ops, dtype=OP_DTYPE) if ops is not None and len(ops) else None
+            self._keep.append(arr)
+            it = self.items[k]
+            it.instance = inst.id
+            it.ops = arr.ctypes.data if arr is not None else None
+            it.nops = 0 if arr is None else len(arr)
+            it.node, it.dad, it.length = node, dad, float(length)
+        self.modes = None if modes is None else np.ascontiguousarray(modes, np.int32)
+        self.out = np.zeros(self.n)
+
+    def run(self):
+        m = i32ptr(self.modes) if self.modes is not None else None
+        check(lib().plk_batch_evaluate(self.ctx.handle, self.items, m, self.n, dptr(self.out)),
+              "plk_batch_evaluate")
+        return self.out.copy()
+
+
 def likelihood_branch_batch(ctx, jobs):
     """jobs: list of (instance, ops, node, dad, length); one fused launch
-    (plk_likelihood_branch_batch). Returns the lnL of every job."""
-    items = (_capi.PlkBatchItem * len(jobs))()
-    keep = []
-    for k, (inst, ops, node, dad, length) in enumerate(jobs):
-        arr = np.ascontiguousarray(ops, dtype=OP_DTYPE) if ops is not None and len(ops) else None
-        keep.append(arr)
-        items[k].instance = inst.id
-        items[k].ops = arr.ctypes.data_as(ctypes.c_void_p) if arr is not None else None
-        items[k].nops = 0 if arr is None else len(arr)
-        items[k].node = node
-        items[k].dad = dad
-        items[k].length = float(length)
-    out = np.zeros(len(jobs))
-    check(lib().plk_likelihood_branch_batch(ctx.handle, items, len(jobs), dptr(out)),
-          "plk_likelihood_branch_batch")
-    return out
+    (plk_likelihood_branch_batch semantics). Returns the lnL of every job."""
+    return BatchPlan(ctx, jobs).run()
 
 
 def batch_evaluate(ctx, jobs, modes):
     """plk_batch_evaluate: jobs as likelihood_branch_batch, modes[i] 0 for the
     branch lnL, 1 for the theta buffer of the branch (lnl 0)."""
-    items = (_capi.PlkBatchItem * len(jobs))()
-    keep = []
-    for k, (inst, ops, node, dad, length) in enumerate(jobs):
-        arr = np.ascontiguousarray(ops, dtype=OP_DTYPE) if ops is not None and len(ops) else None
-        keep.append(arr)
-        items[k].instance = inst.id
-        items[k].ops = arr.ctypes.data if arr is not None else None
-        items[k].nops = len(arr) if arr is not None else 0
-        items[k].node, items[k].dad, items[k].length = node, dad, float(length)
-    m = np.ascontiguousarray(modes, np.int32)
-    out = np.zeros(len(jobs))
-    check(lib().plk_batch_evaluate(ctx.handle, items, i32ptr(m), len(jobs), dptr(out)),
-          "plk_batch_evaluate")
-    return out
+    return BatchPlan(ctx, jobs, modes).run()
 
 
 def likelihood_derv_batch(ctx, pairs):

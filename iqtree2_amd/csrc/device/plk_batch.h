@@ -32,67 +32,74 @@ struct BatchItem {
 
 enum { BATCH_LNL = 0, BATCH_THETA = 1 };
 
-// One table per (item, op, child): k_prep_ops' arithmetic for a batch.
-struct PrepJob {
-    int item;
-    int op;
-};
-
+// Transition tables of a batch with k_prep_ops' arithmetic, and the root
+// tables of k_prep_root. Workgroup (item, row) builds the tables of ops row,
+// row + rows, ... of its item with the item's eigenvectors and tip vectors
+// staged in LDS once; row 0 also writes the item's root tables and workgroup
+// (0, 0) clears the work-list counters. LDS: evec n*n | tip SU*n | ex 2*C*n.
 template <typename T>
-__device__ inline void prep_table(const BatchItem& it, int o, int k, int c, double* ex) {
-    const int n = it.n, C = it.C, SU = it.I.SU;
-    const plk_child ch = it.ops[o].child[k];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) ex[i] = exp(it.m.eval[i] * (it.m.rates[c] * ch.length));
-    __syncthreads();
-    T* out = reinterpret_cast<T*>(const_cast<void*>(it.tables)) + (size_t)(2 * o + k) * it.table_size;
+__device__ inline void prep_child_table(const BatchItem& it, const plk_child ch, const double* ev_sh,
+                                        const double* tp_sh, const double* ex, T* out) {
+    const int n = it.n, C = it.C, SU = it.I.SU, nn = n * n;
     if (!ch.is_tip) {
-        T* oc = out + (size_t)c * n * n;
-        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            const int i = t % n, x = t / n;
-            oc[t] = (T)(it.m.evec[x * n + i] * ex[i]);
+        for (int t = threadIdx.x; t < C * nn; t += kThreads) {
+            const int c = t / nn, r = t - c * nn, x = r / n, i = r - x * n;
+            out[t] = (T)(ev_sh[x * n + i] * ex[c * n + i]);
         }
-    } else {
-        for (int t = threadIdx.x; t < (SU + 1) * n; t += blockDim.x) {
-            const int x = t % n, s = t / n;
-            double v;
-            if (s == SU) {
-                v = 1.0;
-            } else {
-                const double* ev = it.m.evec + x * n;
-                const double* tp = it.m.tip + s * n;
-                v = ev[0] * ex[0] * tp[0];
-                for (int i = 1; i < n; i++) v += ev[i] * ex[i] * tp[i];
-            }
-            out[(size_t)(s * C + c) * n + x] = (T)v;
-        }
-    }
-    __syncthreads();
-}
-
-// workgroups (item, op, category row); root tables of every item by extra
-// workgroups of row 0
-__global__ void __launch_bounds__(kThreads)
-k_prep_batch(const BatchItem* __restrict__ items, const PrepJob* __restrict__ jobs, int njobs,
-             int nitems, const double* __restrict__ lengths, double* __restrict__ roots) {
-    extern __shared__ double ex[];
-    if ((int)blockIdx.x < njobs) {
-        const PrepJob j = jobs[blockIdx.x];
-        const BatchItem& it = items[j.item];
-        // 4 states: one workgroup builds every category's (small) tables;
-        // larger state spaces split the categories over the grid's rows
-        const int c0 = it.n == 4 ? (blockIdx.y == 0 ? 0 : it.C) : (int)blockIdx.y;
-        const int dc = it.n == 4 ? 1 : (int)gridDim.y;
-        for (int c = c0; c < it.C; c += dc)
-            for (int k = 0; k < 2; k++) {
-                if (it.f32) prep_table<float>(it, j.op, k, c, ex);
-                else prep_table<double>(it, j.op, k, c, ex);
-            }
         return;
     }
-    const int i = blockIdx.x - njobs;
-    if (i >= nitems || blockIdx.y != 0) return;
+    const int nv = C * n;
+    for (int t = threadIdx.x; t < (SU + 1) * nv; t += kThreads) {
+        const int s = t / nv, c = (t / n) % C, x = t % n;
+        double v;
+        if (s == SU) {
+            v = 1.0;
+        } else {
+            const double* ev = ev_sh + x * n;
+            const double* e = ex + c * n;
+            const double* tp = tp_sh + s * n;
+            v = ev[0] * e[0] * tp[0];
+            for (int i = 1; i < n; i++) v += ev[i] * e[i] * tp[i];
+        }
+        out[t] = (T)v;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_prep_batch(const BatchItem* __restrict__ items, const double* __restrict__ lengths,
+             double* __restrict__ roots, unsigned* __restrict__ counters, int ncounters) {
+    extern __shared__ double sh[];
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int t = threadIdx.x; t < ncounters; t += kThreads) counters[t] = 0u;
+    const int i = blockIdx.x;
     const BatchItem& it = items[i];
-    const int n = it.n, C = it.C, nv = C * n, SU = it.I.SU;
+    const int n = it.n, C = it.C, SU = it.I.SU, nn = n * n, nv = C * n;
+    double* ev_sh = sh;
+    double* tp_sh = ev_sh + nn;
+    double* ex = tp_sh + SU * n;
+    if ((int)blockIdx.y < it.nops) {
+        for (int t = threadIdx.x; t < nn; t += kThreads) ev_sh[t] = it.m.evec[t];
+        for (int t = threadIdx.x; t < SU * n; t += kThreads) tp_sh[t] = it.m.tip[t];
+    }
+    for (int o = blockIdx.y; o < it.nops; o += gridDim.y) {
+        const plk_op op = it.ops[o];
+        __syncthreads();   // staging visible, the previous op's ex consumed
+        for (int t = threadIdx.x; t < 2 * nv; t += kThreads) {
+            const int k = t / nv, c = (t / n) % C, q = t % n;
+            ex[t] = exp(it.m.eval[q] * (it.m.rates[c] * op.child[k].length));
+        }
+        __syncthreads();
+        for (int k = 0; k < 2; k++) {
+            const size_t off = (size_t)(2 * o + k) * it.table_size;
+            if (it.f32)
+                prep_child_table<float>(it, op.child[k], ev_sh, tp_sh, ex + k * nv,
+                                        reinterpret_cast<float*>(const_cast<void*>(it.tables)) + off);
+            else
+                prep_child_table<double>(it, op.child[k], ev_sh, tp_sh, ex + k * nv,
+                                         reinterpret_cast<double*>(const_cast<void*>(it.tables)) + off);
+        }
+    }
+    if (blockIdx.y != 0) return;
     double* out = const_cast<double*>(it.root);
     const double len = lengths[i];
     for (int t = threadIdx.x; t < nv; t += blockDim.x) {
@@ -166,6 +173,24 @@ __device__ inline double branch_end_stream(const InstDev& I, plk_end node, plk_e
     return safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
 }
 
+// 16-pattern MFMA sub-tiles of a protein batch tile
+constexpr int kBatchSub = 4;
+
+// Dynamic LDS bytes batch_tile uses for one item shape: U^-1, the two child
+// tables of a step and the root tables (which reuse the table area), or for
+// 20 states the MFMA layout of mfma_ops_tile (U^-1 and tables with row
+// stride N + 1, kBatchSub x C x 16 maxima slots).
+__host__ __device__ inline size_t batch_smem_bytes(int n, int C, int SU, int table_size, size_t elem) {
+    const size_t root = (size_t)(SU + 2) * n * C * 8;
+    if (n == 20) {
+        const size_t mf = ((size_t)n * (n + 1) * (1 + 2 * C) + (size_t)kBatchSub * C * 16) * elem;
+        const size_t rt = (size_t)n * n * elem + root;
+        return mf > rt ? mf : rt;
+    }
+    const size_t tab = 2 * (size_t)table_size * elem;
+    return (size_t)n * n * elem + (tab > root ? tab : root);
+}
+
 // patterns per work-list tile: one pattern per C threads (64 with four
 // categories; for 20 states four 16-pattern matrix-core tiles)
 __host__ __device__ constexpr int batch_tile_patterns(int, int C) { return kThreads / C; }
@@ -186,13 +211,13 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
     if constexpr (N == 20) {
         // protein: the matrix-core tile of the lone traversal kernel, four
         // 16-pattern sub-tiles per batch tile
-        static_assert(PB == 4 * kMfmaTile, "batch tile must be four MFMA tiles");
+        static_assert(PB == kBatchSub * kMfmaTile, "batch tile must be kBatchSub MFMA tiles");
         using L = MfmaSmall<T, N, C>;
         T* inv_m = reinterpret_cast<T*>(smem);
         T* tab_m = inv_m + L::kInv;
         T* red_m = tab_m + 2 * L::kTab;
         stage_inv<T>(it.m.inv_evec, inv_m, N, L::S);
-        mfma_ops_tile<T, N, C, 4>(I, it.ops, it.nops, tables, it.table_size, inv_m, tab_m, red_m,
+        mfma_ops_tile<T, N, C, kBatchSub>(I, it.ops, it.nops, tables, it.table_size, inv_m, tab_m, red_m,
                                   tile * PB);
     } else {
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)it.m.inv_evec[t];

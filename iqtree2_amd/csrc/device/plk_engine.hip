@@ -154,22 +154,17 @@ struct plk_context {
     double* h_out = nullptr;  // pinned host staging for reduced scalars
     unsigned char* h_stage = nullptr;   // pinned host staging for uploads
     size_t h_stage_cap = 0;
+    bool h_stage_inflight = false;      // an upload from h_stage may still run
     // batched evaluation buffers (grown on demand)
-    BatchItem* bt_items = nullptr;
-    int bt_items_cap = 0;
-    PrepJob* bt_jobs = nullptr;
-    int bt_jobs_cap = 0;
-    plk_op* bt_ops = nullptr;
-    int bt_ops_cap = 0;
+    unsigned char* bt_upload = nullptr;   // items | ops | tile list | root lengths
+    int bt_upload_cap = 0;
     unsigned char* bt_tables = nullptr;
     int bt_tables_cap = 0;
     double* bt_roots = nullptr;
     int bt_roots_cap = 0;
-    int* bt_tile_item = nullptr;
-    int bt_tile_cap = 0;
     double* bt_tile_sums = nullptr;
     int bt_tile_sums_cap = 0;
-    double* bt_vals = nullptr;      // lengths in, lnl out
+    double* bt_vals = nullptr;      // lnl out
     int bt_vals_cap = 0;
     unsigned* bt_counter = nullptr;
     int bt_counter_cap = 0;
@@ -668,6 +663,31 @@ plk_status fixup_nonfinite(plk_context* ctx, Instance* in, double* lnl, double* 
 
 }  // namespace
 
+namespace {
+
+// Pinned host staging of at least need bytes for an upload on ctx->stream.
+// An upload that may still read the buffer (a call that returned early) is
+// waited for before the buffer is handed out again.
+plk_status host_stage(plk_context* ctx, size_t need, unsigned char** out) {
+    if (ctx->h_stage_inflight) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->h_stage_inflight = false;
+    }
+    if (need > ctx->h_stage_cap) {
+        const size_t cap = std::max(need, 2 * ctx->h_stage_cap);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->h_stage_cap = 0;
+        HIP_TRY(hipHostMalloc((void**)&ctx->h_stage, cap, hipHostMallocDefault));
+        ctx->h_stage_cap = cap;
+    }
+    *out = ctx->h_stage;
+    return PLK_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* plk_last_error(void) { return g_last_error.c_str(); }
@@ -835,16 +855,9 @@ plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, con
     // contiguous copies instead of one staged copy per taxon
     const size_t Ppad = in->Ppad, nst = (size_t)in->d.ntaxa * Ppad;
     const size_t need = nst + 2 * Ppad * sizeof(double) + 16;
-    if (need > ctx->h_stage_cap) {
-        const size_t cap = std::max(need, 2 * ctx->h_stage_cap);
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
-        ctx->h_stage = nullptr;
-        ctx->h_stage_cap = 0;
-        HIP_TRY(hipHostMalloc((void**)&ctx->h_stage, cap, hipHostMallocDefault));
-        ctx->h_stage_cap = cap;
-    }
-    unsigned char* hs = ctx->h_stage;
+    unsigned char* hs = nullptr;
+    plk_status st = host_stage(ctx, need, &hs);
+    if (st != PLK_OK) return st;
     for (int t = 0; t < in->d.ntaxa; t++) {
         std::memcpy(hs + (size_t)t * Ppad, states + (size_t)t * P, P);
         std::memset(hs + (size_t)t * Ppad + P, in->d.state_unknown, Ppad - P);
@@ -855,10 +868,12 @@ plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, con
     std::memset(hf + P, 0, sizeof(double) * (Ppad - P));
     std::memcpy(hv, invar, sizeof(double) * P);
     std::memset(hv + P, 0, sizeof(double) * (Ppad - P));
+    ctx->h_stage_inflight = true;
     HIP_TRY(hipMemcpyAsync(in->states, hs, nst, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(in->freq, hf, sizeof(double) * Ppad, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(in->invar, hv, sizeof(double) * Ppad, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->h_stage_inflight = false;
     in->patterns_set = true;
     in->theta_ready = false;
     return PLK_OK;
@@ -1166,47 +1181,40 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         if (cls[a] != cls[b]) return cls[a] > cls[b];
         return cost[a] > cost[b];
     });
+    // one pinned upload: items | ops | tile list | root lengths
     std::vector<BatchItem> hb(nitems);
-    std::vector<PrepJob> jobs;
-    std::vector<plk_op> ops;
-    std::vector<int> tile_item;
-    std::vector<double> lens(nitems);
-    size_t tab_bytes = 0, root_doubles = 0, smem = 0, smem_prep = 0;
-    std::vector<size_t> tab_off(nitems), root_off(nitems), op_off(nitems);
+    std::vector<size_t> op_off(nitems);
+    size_t tab_bytes = 0, root_doubles = 0, smem = 0, smem_prep = 0, nops_total = 0;
+    int total_tiles = 0, max_nops = 0;
     double bytes = 0.0;
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];
-        Instance* in = ins[i];
-        op_off[r] = ops.size();
-        ops.insert(ops.end(), items[i].ops, items[i].ops + items[i].nops);
-        tab_off[r] = tab_bytes;
-        tab_bytes += round_up((size_t)items[i].nops * 2 * in->table_size * in->elem, 256);
-        root_off[r] = root_doubles;
-        root_doubles += (size_t)(in->d.state_unknown + 2) * in->block;
-        for (int o = 0; o < items[i].nops; o++) jobs.push_back(PrepJob{r, o});
+        const Instance* in = ins[i];
+        op_off[r] = nops_total;
+        nops_total += items[i].nops;
+        max_nops = std::max(max_nops, items[i].nops);
         const int PB = batch_tile_patterns(in->d.nstates, in->d.ncat);
-        const int ntiles = (in->d.npatterns + PB - 1) / PB;
         BatchItem& b = hb[r];
         b.I = inst_dev(in);
         b.m = model_dev(in);
         b.nops = items[i].nops;
         b.table_size = in->table_size;
-        b.tile_begin = (int)tile_item.size();
-        b.ntiles = ntiles;
+        b.tile_begin = total_tiles;
+        b.ntiles = (in->d.npatterns + PB - 1) / PB;
         b.node = items[i].node;
         b.dad = items[i].dad;
         b.n = in->d.nstates;
         b.C = in->d.ncat;
         b.f32 = in->d.precision == PLK_PREC_F32;
         b.mode = modes ? modes[i] : BATCH_LNL;
-        for (int t = 0; t < ntiles; t++) tile_item.push_back(r);
-        lens[r] = items[i].length;
-        const size_t s1 = round_up((size_t)b.n * b.n * in->elem, 16) +
-                          std::max(2 * (size_t)in->table_size * in->elem,
-                                   (size_t)(in->d.state_unknown + 2) * in->block * 8);
-        smem = std::max(smem, s1);
-        smem_prep = std::max(smem_prep, (size_t)in->block * 8);
-        double ob = 0.0, of = 0.0;
+        total_tiles += b.ntiles;
+        b.tables = reinterpret_cast<const void*>(tab_bytes);   // offsets until the buffers exist
+        tab_bytes += round_up((size_t)items[i].nops * 2 * in->table_size * in->elem, 256);
+        b.root = reinterpret_cast<const double*>(root_doubles);
+        root_doubles += (size_t)(in->d.state_unknown + 2) * in->block;
+        smem = std::max(smem, batch_smem_bytes(b.n, b.C, in->d.state_unknown, in->table_size, in->elem));
+        smem_prep = std::max(smem_prep, (size_t)(b.n * b.n + in->d.state_unknown * b.n + 2 * in->block) * 8);
+        double ob = 0.0;
         for (int o = 0; o < items[i].nops; o++) {
             double bb, ff;
             op_cost(in, items[i].ops[o], bb, ff);
@@ -1215,34 +1223,45 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         const int nint = 1 + (items[i].dad.is_tip ? 0 : 1);
         ob += (double)in->block * in->elem * nint + (items[i].dad.is_tip ? 1 : 0) + in->scale_elem * nint + 24.0;
         bytes += ob * in->d.npatterns;
-        (void)of;
     }
-    const int total_tiles = (int)tile_item.size();
-    if ((st = grow(ctx, nullptr, &ctx->bt_items, ctx->bt_items_cap, nitems)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_jobs, ctx->bt_jobs_cap, std::max<int>(1, jobs.size()))) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_ops, ctx->bt_ops_cap, std::max<int>(1, ops.size()))) != PLK_OK) return st;
+    const size_t off_ops = round_up(sizeof(BatchItem) * nitems, 256);
+    const size_t off_tiles = off_ops + round_up(sizeof(plk_op) * std::max<size_t>(1, nops_total), 256);
+    const size_t off_lens = off_tiles + round_up(sizeof(int) * total_tiles, 256);
+    const size_t up_bytes = off_lens + sizeof(double) * nitems;
+    if ((st = grow(ctx, nullptr, &ctx->bt_upload, ctx->bt_upload_cap, (int)up_bytes)) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->bt_tables, ctx->bt_tables_cap, (int)std::max<size_t>(256, tab_bytes))) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->bt_roots, ctx->bt_roots_cap, (int)root_doubles)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_tile_item, ctx->bt_tile_cap, total_tiles)) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->bt_tile_sums, ctx->bt_tile_sums_cap, total_tiles)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_vals, ctx->bt_vals_cap, 2 * nitems)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_vals, ctx->bt_vals_cap, nitems)) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->bt_counter, ctx->bt_counter_cap, 64)) != PLK_OK) return st;
+    unsigned char* hs = nullptr;
+    if ((st = host_stage(ctx, up_bytes, &hs)) != PLK_OK) return st;
+    const BatchItem* d_items = reinterpret_cast<const BatchItem*>(ctx->bt_upload);
+    const plk_op* d_ops = reinterpret_cast<const plk_op*>(ctx->bt_upload + off_ops);
+    const int* d_tiles = reinterpret_cast<const int*>(ctx->bt_upload + off_tiles);
+    const double* d_lens = reinterpret_cast<const double*>(ctx->bt_upload + off_lens);
+    plk_op* h_ops = reinterpret_cast<plk_op*>(hs + off_ops);
+    int* h_tiles = reinterpret_cast<int*>(hs + off_tiles);
+    double* h_lens = reinterpret_cast<double*>(hs + off_lens);
     for (int r = 0; r < nitems; r++) {
-        hb[r].ops = ctx->bt_ops + op_off[r];
-        hb[r].tables = ctx->bt_tables + tab_off[r];
-        hb[r].root = ctx->bt_roots + root_off[r];
+        const int i = order[r];
+        BatchItem& b = hb[r];
+        b.ops = d_ops + op_off[r];
+        b.tables = ctx->bt_tables + reinterpret_cast<size_t>(b.tables);
+        b.root = ctx->bt_roots + reinterpret_cast<size_t>(b.root);
+        if (items[i].nops) std::memcpy(h_ops + op_off[r], items[i].ops, sizeof(plk_op) * items[i].nops);
+        for (int t = 0; t < b.ntiles; t++) h_tiles[b.tile_begin + t] = r;
+        h_lens[r] = items[i].length;
     }
-    HIP_TRY(hipMemcpyAsync(ctx->bt_items, hb.data(), sizeof(BatchItem) * nitems, hipMemcpyHostToDevice, ctx->stream));
-    if (!jobs.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->bt_jobs, jobs.data(), sizeof(PrepJob) * jobs.size(), hipMemcpyHostToDevice, ctx->stream));
-    if (!ops.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->bt_ops, ops.data(), sizeof(plk_op) * ops.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->bt_tile_item, tile_item.data(), sizeof(int) * total_tiles, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->bt_vals, lens.data(), sizeof(double) * nitems, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned), ctx->stream));
+    std::memcpy(hs, hb.data(), sizeof(BatchItem) * nitems);
+    HIP_TRY(hipMemcpyAsync(ctx->bt_upload, hs, up_bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->h_stage_inflight = true;
+    // tables: rows of ops per item so that the launch has a few thousand
+    // workgroups (a single item still gets one per op)
+    const int rows = std::max(1, std::min(max_nops, std::max(4, (4096 + nitems - 1) / nitems)));
     int pe = prof_begin(ctx);
-    hipLaunchKernelGGL(k_prep_batch, dim3((int)jobs.size() + nitems, 4), dim3(kThreads), smem_prep, ctx->stream,
-                       ctx->bt_items, ctx->bt_jobs, (int)jobs.size(), nitems, ctx->bt_vals, ctx->bt_roots);
+    hipLaunchKernelGGL(k_prep_batch, dim3(nitems, rows), dim3(kThreads), smem_prep, ctx->stream,
+                       d_items, d_lens, ctx->bt_roots, ctx->bt_counter, 16);
     HIP_TRY(hipGetLastError());
     prof_end(ctx, pe, K_PREP_BATCH, (double)tab_bytes, 0.0);
     // class ranges of the tile list
@@ -1255,8 +1274,10 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         if (ranges.empty() || ranges.back().first != c) ranges.push_back({c, end});
         else ranges.back().second = end;
     }
+    bool counters_clear = true;   // k_prep_batch cleared them
     auto launch = [&](int split, int bpc) -> plk_status {
-        HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned) * 16, ctx->stream));
+        if (!counters_clear) HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned) * 16, ctx->stream));
+        counters_clear = false;
         auto one = [&](int mask, int begin, int end, int k) -> plk_status {
             const void* fn;
             switch (mask) {
@@ -1269,8 +1290,8 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
             int nb = 0;
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, smem));
             const int grid = std::max(1, std::min(end - begin, ctx->num_cu * std::max(1, std::min(nb, bpc))));
-            const BatchItem* a0 = ctx->bt_items;
-            const int* a1 = ctx->bt_tile_item;
+            const BatchItem* a0 = d_items;
+            const int* a1 = d_tiles;
             unsigned* a4 = ctx->bt_counter + k;
             double* a5 = ctx->bt_tile_sums;
             void* args[] = {(void*)&a0, (void*)&a1, (void*)&begin, (void*)&end, (void*)&a4, (void*)&a5};
@@ -1328,11 +1349,12 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     if ((st = launch(split, bpc)) != PLK_OK) return st;
     prof_end(ctx, pe, K_BATCH, bytes, 0.0);
     hipLaunchKernelGGL(k_batch_reduce, dim3((nitems + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                       ctx->stream, ctx->bt_items, nitems, ctx->bt_tile_sums, ctx->bt_vals + nitems);
+                       ctx->stream, d_items, nitems, ctx->bt_tile_sums, ctx->bt_vals);
     HIP_TRY(hipGetLastError());
     std::vector<double> out(nitems);
-    HIP_TRY(hipMemcpyAsync(out.data(), ctx->bt_vals + nitems, sizeof(double) * nitems, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out.data(), ctx->bt_vals, sizeof(double) * nitems, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->h_stage_inflight = false;
     bool nonfinite = false;
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];
@@ -1379,9 +1401,14 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
         if ((st = check_end(in, items[i].node, false)) != PLK_OK) return st;
         if ((st = check_end(in, items[i].dad, true)) != PLK_OK) return st;
         if (!(items[i].length >= 0.0) || !std::isfinite(items[i].length)) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
-        for (int j = 0; j < i; j++)
-            if (items[j].instance == items[i].instance) return fail(PLK_ERR_ARG, "instance %d appears twice in a batch", items[i].instance);
         ins[i] = in;
+    }
+    {
+        std::vector<int> ids(nitems);
+        for (int i = 0; i < nitems; i++) ids[i] = items[i].instance;
+        std::sort(ids.begin(), ids.end());
+        for (int i = 1; i < nitems; i++)
+            if (ids[i] == ids[i - 1]) return fail(PLK_ERR_ARG, "instance %d appears twice in a batch", ids[i]);
     }
     // shapes the fused kernel is built for go into one launch; the rest
     // (61 states, category counts other than 4) run one traversal each

@@ -13,8 +13,8 @@
 // (state tile, register) pairs, so tmp stays in registers; phase B's output
 // has patterns on rows and 16 consecutive states across lanes, so every
 // store writes contiguous runs of a pattern's block. The rescale decision is
-// per pattern over the whole block (normal: the waves exchange their maxima
-// through LDS) or per category (safe), as in the reference.
+// per pattern over the whole block (normal: the maxima of every wave's
+// categories meet in LDS) or per category (safe), as in the reference.
 #pragma once
 #include "plk_codon.h"
 
@@ -41,15 +41,19 @@ __device__ inline void stage_inv(const double* __restrict__ inv, T* inv_sh, int 
 // times as many workgroups as one wave per 16 patterns would.
 constexpr int kMfmaTile = 16;
 
-// All steps of an op list for the patterns p0 .. p0 + 16*SUB - 1, SUB
-// 16-pattern tiles one after another under each step's staged tables. U^-1
-// is staged in inv_sh (row stride N + 1); tab_sh receives the two child
-// tables of each step (all categories); red_sh holds 2 x 4 x 16 per-wave
-// maxima.
+// All steps of an op list for the patterns p0 .. p0 + 16*SUB - 1. Under each
+// step's staged tables every wave computes its categories of the SUB
+// 16-pattern sub-tiles one after another and stores them unscaled, with the
+// per-(pattern, category) maxima in LDS; after one barrier the patterns
+// that need rescaling (rare: underflow below 2^-256 or the FP32 threshold)
+// are read back, scaled and stored again, and the scale counters written.
+// The stored values equal scaling the computed ones directly. U^-1 is staged
+// in inv_sh (row stride N + 1); tab_sh receives the two child tables of each
+// step (all categories); mx_sh holds SUB x C x 16 maxima.
 template <typename T, int N, int C, int SUB>
 __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict__ ops, int nops,
                                      const T* __restrict__ tables, int table_size, const T* inv_sh,
-                                     T* tab_sh, T* red_sh, int p_first) {
+                                     T* tab_sh, T* mx_sh, int p_first) {
     using ST = typename Prec<T>::scale_t;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
@@ -72,158 +76,159 @@ __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict_
         const T* gt[2] = {tables + (size_t)(2 * o + left) * table_size,
                           tables + (size_t)(2 * o + right) * table_size};
         const bool tiptip = ch[0].is_tip && ch[1].is_tip;
+        T* dst = partial + (size_t)op.dad_slot * I.slot_stride;
         // E tables of the internal children, every category, row x, column k
+        // (the barrier also orders the previous step's maxima reads before
+        // this step's writes)
         __syncthreads();
         for (int k = 0; k < 2; k++) {
             if (ch[k].is_tip) continue;
-            T* dst = tab_sh + k * L::kTab;
+            T* tdst = tab_sh + k * L::kTab;
             for (int t = threadIdx.x; t < C * N * N; t += kThreads) {
                 const int c = t / (N * N), r = t - c * N * N;
-                dst[c * N * S + (r / N) * S + r % N] = gt[k][t];
+                tdst[c * N * S + (r / N) * S + r % N] = gt[k][t];
             }
         }
         __syncthreads();
-#pragma unroll 1
-        for (int sub = 0; sub < SUB; sub++) {
-        const int p0 = p_first + 16 * sub;
-        const int pb = p0 + col;   // this lane's pattern as a B operand / A row
-        T* red = red_sh + (sub & 1) * 64;   // alternating, so no barrier is needed before refilling
-        acc_t d[CPW][NT];   // d[j][ib][r] = dad[p0 + row(g, r)][category wave + CW j][16 ib + col]
-        T wmax[4] = {(T)0, (T)0, (T)0, (T)0};   // per register-row pattern, over this wave's categories
         if (active) {
+#pragma unroll 1
+            for (int sub = 0; sub < SUB; sub++) {
+                const int p0 = p_first + 16 * sub;
+                const int pb = p0 + col;   // this lane's pattern as a B operand
 #pragma unroll
-            for (int j = 0; j < CPW; j++) {
-                const int c = wave + CW * j;
-                if (c >= C) break;
-                acc_t prod[NT];   // tmp[x = 16 xb + row(g, r)][pattern col]
+                for (int j = 0; j < CPW; j++) {
+                    const int c = wave + CW * j;
+                    if (c >= C) break;
+                    acc_t prod[NT];   // tmp[x = 16 xb + row(g, r)][pattern col]
 #pragma unroll
-                for (int k = 0; k < 2; k++) {
-                    acc_t a[NT];
-                    if (ch[k].is_tip) {
-                        const int s = I.states[(size_t)ch[k].id * I.Ppad + pb];
-                        const T* lut = gt[k] + (size_t)(s * C + c) * N;
+                    for (int k = 0; k < 2; k++) {
+                        acc_t a[NT];
+                        if (ch[k].is_tip) {
+                            const int s = I.states[(size_t)ch[k].id * I.Ppad + pb];
+                            const T* lut = gt[k] + (size_t)(s * C + c) * N;
 #pragma unroll
-                        for (int xb = 0; xb < NT; xb++)
+                            for (int xb = 0; xb < NT; xb++)
 #pragma unroll
-                            for (int r = 0; r < 4; r++) {
-                                const int x = 16 * xb + M::row(g, r);
-                                a[xb][r] = x < N ? lut[x] : (T)0;
+                                for (int r = 0; r < 4; r++) {
+                                    const int x = 16 * xb + M::row(g, r);
+                                    a[xb][r] = x < N ? lut[x] : (T)0;
+                                }
+                        } else {
+                            const T* E = tab_sh + k * L::kTab + c * N * S;
+                            const T* cv = partial + (size_t)ch[k].id * I.slot_stride + (size_t)pb * block + c * N;
+                            T bv[KS];
+#pragma unroll
+                            for (int st = 0; st < KS; st++) {
+                                const int kk = 4 * st + g;
+                                bv[st] = kk < N ? cv[kk] : (T)0;
                             }
-                    } else {
-                        const T* E = tab_sh + k * L::kTab + c * N * S;
-                        const T* cv = partial + (size_t)ch[k].id * I.slot_stride + (size_t)pb * block + c * N;
-                        T bv[KS];
 #pragma unroll
-                        for (int st = 0; st < KS; st++) {
-                            const int kk = 4 * st + g;
-                            bv[st] = kk < N ? cv[kk] : (T)0;
+                            for (int xb = 0; xb < NT; xb++) a[xb] = acc_t{0, 0, 0, 0};
+#pragma unroll
+                            for (int st = 0; st < KS; st++) {
+                                const int kk = 4 * st + g;
+                                const bool kin = kk < N;
+#pragma unroll
+                                for (int xb = 0; xb < NT; xb++) {
+                                    const int x = 16 * xb + col;
+                                    const T av = (kin && x < N) ? E[x * S + kk] : (T)0;
+                                    a[xb] = M::mma(av, bv[st], a[xb]);
+                                }
+                            }
                         }
+                        if (k == 0) {
 #pragma unroll
-                        for (int xb = 0; xb < NT; xb++) a[xb] = acc_t{0, 0, 0, 0};
+                            for (int xb = 0; xb < NT; xb++) prod[xb] = a[xb];
+                        } else {
 #pragma unroll
-                        for (int st = 0; st < KS; st++) {
-                            const int kk = 4 * st + g;
-                            const bool kin = kk < N;
-#pragma unroll
-                            for (int xb = 0; xb < NT; xb++) {
-                                const int x = 16 * xb + col;
-                                const T av = (kin && x < N) ? E[x * S + kk] : (T)0;
-                                a[xb] = M::mma(av, bv[st], a[xb]);
-                            }
+                            for (int xb = 0; xb < NT; xb++) prod[xb] = prod[xb] * a[xb];
                         }
                     }
-                    if (k == 0) {
+                    // phase B, patterns on M: A[p = col][k = g] is prod[xb][r] for
+                    // state x = 16 xb + row(g, r); B[k][i = col] = U^-1[i][x];
+                    // d[ib][r] = dad[p0 + row(g, r)][c][16 ib + col]
+                    acc_t d[NT];
 #pragma unroll
-                        for (int xb = 0; xb < NT; xb++) prod[xb] = a[xb];
-                    } else {
+                    for (int ib = 0; ib < NT; ib++) d[ib] = acc_t{0, 0, 0, 0};
 #pragma unroll
-                        for (int xb = 0; xb < NT; xb++) prod[xb] = prod[xb] * a[xb];
-                    }
-                }
-                // phase B, patterns on M: A[p = col][k = g] is prod[xb][r] for
-                // state x = 16 xb + row(g, r); B[k][i = col] = U^-1[i][x]
+                    for (int xb = 0; xb < NT; xb++)
 #pragma unroll
-                for (int ib = 0; ib < NT; ib++) d[j][ib] = acc_t{0, 0, 0, 0};
+                        for (int r = 0; r < 4; r++) {
+                            if (16 * xb + M::row(0, r) >= N) continue;   // a step with no state inside
+                            const int x = 16 * xb + M::row(g, r);
+                            const T av = prod[xb][r];
 #pragma unroll
-                for (int xb = 0; xb < NT; xb++)
+                            for (int ib = 0; ib < NT; ib++) {
+                                const int i = 16 * ib + col;
+                                const T bv = (i < N && x < N) ? inv_sh[i * S + x] : (T)0;
+                                d[ib] = M::mma(av, bv, d[ib]);
+                            }
+                        }
+                    // unscaled store and the category maxima
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        if (16 * xb + M::row(0, r) >= N) continue;   // a step with no state inside
-                        const int x = 16 * xb + M::row(g, r);
-                        const T a = prod[xb][r];
+                        T* row = dst + (size_t)(p0 + M::row(g, r)) * block + c * N;
+                        T m = (T)0;
 #pragma unroll
                         for (int ib = 0; ib < NT; ib++) {
                             const int i = 16 * ib + col;
-                            const T bv = (i < N && x < N) ? inv_sh[i * S + x] : (T)0;
-                            d[j][ib] = M::mma(a, bv, d[j][ib]);
+                            if (i < N) {
+                                row[i] = d[ib][r];
+                                m = fmax(m, fabs(d[ib][r]));
+                            }
+                        }
+#pragma unroll
+                        for (int dl = 1; dl < 16; dl <<= 1) m = fmax(m, __shfl_xor(m, dl));
+                        if (col == 0) mx_sh[(sub * C + c) * 16 + M::row(g, r)] = m;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        // rescale (rare) and scale counters; lane (g, col) handles the patterns
+        // row(g, r) of each sub-tile, columns col, col + 16 of its categories
+#pragma unroll 1
+        for (int sub = 0; sub < SUB; sub++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int q = M::row(g, r);
+                const int p = p_first + 16 * sub + q;
+                const bool may_scale = !tiptip && I.invar[p] == 0.0;
+                int kblock = 0;
+                if (!safe && may_scale) {
+                    T bm = mx_sh[(sub * C) * 16 + q];
+                    for (int c = 1; c < C; c++) bm = fmax(bm, mx_sh[(sub * C + c) * 16 + q]);
+                    if (Prec<T>::underflow(bm)) kblock = Prec<T>::exponent(bm);
+                }
+#pragma unroll
+                for (int j = 0; j < CPW; j++) {
+                    const int c = wave + CW * j;
+                    if (c >= C) break;
+                    int kx = kblock;
+                    if (safe && may_scale) {
+                        const T cm = mx_sh[(sub * C + c) * 16 + q];
+                        kx = Prec<T>::underflow(cm) ? Prec<T>::exponent(cm) : 0;
+                    }
+                    if (kx) {
+                        T* row = dst + (size_t)p * block + c * N;
+#pragma unroll
+                        for (int ib = 0; ib < NT; ib++) {
+                            const int i = 16 * ib + col;
+                            if (i < N) row[i] = Prec<T>::apply(row[i], kx);
                         }
                     }
-            }
-        }
-        // per-pattern maxima: register r holds pattern p0 + row(g, r)
-        T cmax[CPW][4];
-#pragma unroll
-        for (int j = 0; j < CPW; j++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                T m = (T)0;
-                if (active && wave + CW * j < C) {
-#pragma unroll
-                    for (int ib = 0; ib < NT; ib++)
-                        if (16 * ib + col < N) m = fmax(m, fabs(d[j][ib][r]));
-                }
-#pragma unroll
-                for (int dl = 1; dl < 16; dl <<= 1) m = fmax(m, __shfl_xor(m, dl));
-                cmax[j][r] = m;
-                wmax[r] = fmax(wmax[r], m);
-            }
-        if (!safe) {
-            // the block maximum spans every wave's categories
-            if (col == 0)
-#pragma unroll
-                for (int r = 0; r < 4; r++) red[wave * 16 + M::row(g, r)] = wmax[r];
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                T m = red[M::row(g, r)];
-                for (int w = 1; w < CW; w++) m = fmax(m, red[w * 16 + M::row(g, r)]);
-                wmax[r] = m;
-            }
-        }
-        if (!active) continue;
-        T* dst = partial + (size_t)op.dad_slot * I.slot_stride;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int p = p0 + M::row(g, r);
-            const bool may_scale = !tiptip && I.invar[p] == 0.0;
-            const int kblock = !safe && may_scale && Prec<T>::underflow(wmax[r]) ? Prec<T>::exponent(wmax[r]) : 0;
-            T* row = dst + (size_t)p * block;
-#pragma unroll
-            for (int j = 0; j < CPW; j++) {
-                const int c = wave + CW * j;
-                if (c >= C) break;
-                const int kx = safe ? (may_scale && Prec<T>::underflow(cmax[j][r]) ? Prec<T>::exponent(cmax[j][r]) : 0)
-                                    : kblock;
-#pragma unroll
-                for (int ib = 0; ib < NT; ib++) {
-                    const int i = 16 * ib + col;
-                    if (i < N) {
-                        T v = d[j][ib][r];
-                        if (kx) v = Prec<T>::apply(v, kx);
-                        row[c * N + i] = v;
+                    // counters: per category (safe) or once per pattern by wave 0
+                    if (col == 0 && (safe || c == 0)) {
+                        const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
+                        int sc = kx;
+                        if (!ch[0].is_tip) sc += scale[(size_t)ch[0].id * I.scale_stride + sidx];
+                        if (!ch[1].is_tip) sc += scale[(size_t)ch[1].id * I.scale_stride + sidx];
+                        scale[(size_t)op.dad_slot * I.scale_stride + sidx] = (ST)sc;
                     }
                 }
-                // counters: per category (safe) or once per pattern by wave 0
-                if (col == 0 && (safe || c == 0)) {
-                    const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
-                    int sc = kx;
-                    if (!ch[0].is_tip) sc += scale[(size_t)ch[0].id * I.scale_stride + sidx];
-                    if (!ch[1].is_tip) sc += scale[(size_t)ch[1].id * I.scale_stride + sidx];
-                    scale[(size_t)op.dad_slot * I.scale_stride + sidx] = (ST)sc;
-                }
             }
         }
-        }   // sub-tiles
     }
 }
 
@@ -234,9 +239,9 @@ k_traverse_mfma(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
     using L = MfmaSmall<T, N, C>;
     __shared__ T inv_sh[L::kInv];
     __shared__ T tab_sh[2 * L::kTab];
-    __shared__ T red_sh[2 * 4 * 16];
+    __shared__ T mx_sh[C * 16];
     stage_inv<T>(m.inv_evec, inv_sh, N, L::S);
-    mfma_ops_tile<T, N, C, 1>(I, ops, nops, tables, table_size, inv_sh, tab_sh, red_sh,
+    mfma_ops_tile<T, N, C, 1>(I, ops, nops, tables, table_size, inv_sh, tab_sh, mx_sh,
                               blockIdx.x * kMfmaTile);
 }
 

@@ -407,6 +407,39 @@ def test_partition_batch_equals_per_partition(ctx):
         inst.close()
 
 
+def test_batch_protein_scaling_bit_identical(ctx):
+    """Batched protein tiles (four 16-pattern MFMA sub-tiles per workgroup)
+    with partials crossing the scaling threshold, in normal and safe mode,
+    FP64 and FP32: every partial and scale counter the batch leaves equals
+    the lone traversal kernel's, and the lnL agrees to rounding."""
+    specs = [(False, ia.PREC_F64), (True, ia.PREC_F64), (False, ia.PREC_F32), (True, ia.PREC_F32)]
+    jobs, twins = [], []
+    for k, (safe, prec) in enumerate(specs):
+        w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=150, nsite=300 + 70 * k, ncat=4, mean_len=0.5,
+                        alpha=1.0, seed=17 + k)
+        planner = ia.PhyloTree(None, w)
+        ops, node, dad, length = planner.plan(*planner.branches()[k])
+        a = ia.Instance.from_workload(ctx, w, precision=prec, safe_numeric=safe)
+        b = ia.Instance.from_workload(ctx, w, precision=prec, safe_numeric=safe)
+        jobs.append((a, ops, node, dad, length))
+        twins.append((b, ops, node, dad, length))
+    batched = ia.likelihood_branch_batch(ctx, jobs)
+    single = [b.likelihood_branch(ops, node, dad, length) for b, ops, node, dad, length in twins]
+    np.testing.assert_allclose(batched, np.array(single), rtol=1e-12, atol=0)
+    for (a, ops, *_), (b, *_) in zip(jobs, twins):
+        scaled = 0
+        for op in ops:
+            pa, sa = a.read_partial(int(op["dad_slot"]))
+            pb, sb = b.read_partial(int(op["dad_slot"]))
+            np.testing.assert_array_equal(pa, pb)
+            np.testing.assert_array_equal(sa, sb)
+            scaled = max(scaled, int(sb.max()))
+        assert scaled >= 1, "the workload was meant to cross the scaling threshold"
+    for (a, *_), (b, *_) in zip(jobs, twins):
+        a.close()
+        b.close()
+
+
 def test_batch_mixed_shapes(ctx):
     """A batch mixing fusable items with 61-state and 2-category items: the
     fused ones share one launch, the others run alone, and every result

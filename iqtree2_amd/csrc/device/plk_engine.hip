@@ -142,6 +142,8 @@ struct plk_context {
     int device = 0;
     int num_cu = 0;
     int sweep_blocks_per_cu = 2;
+    bool sweep_lds = true;              // theta resident in LDS during sweeps when it fits
+    int lds_per_cu = 0;                 // bytes of LDS per CU
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev;
     size_t prof_next = 0;
@@ -715,7 +717,9 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     HIP_TRY(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipDeviceGetAttribute(&ctx->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     if (const char* e = std::getenv("PLK_SWEEP_BLOCKS_PER_CU")) ctx->sweep_blocks_per_cu = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PLK_SWEEP_LDS")) ctx->sweep_lds = std::atoi(e) != 0;
     if (const char* e = std::getenv("PLK_BATCH_BLOCKS_PER_CU")) ctx->batch_blocks_per_cu = std::max(1, std::atoi(e));
     if (std::getenv("PLK_NO_AUTOTUNE")) ctx->autotune = false;
     ctx->budget = (cfg && cfg->hbm_budget) ? cfg->hbm_budget : (size_t)(0.9 * (double)free_b);
@@ -1082,17 +1086,43 @@ plk_status grow(plk_context* ctx, Instance* in, T** p, int& cap, int need) {
 
 template <typename T, int N, int C>
 plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& grid_out) {
-    const SweepSmem L(N, C, in->d.state_unknown, in->table_size, a.nedges, (int)sizeof(T));
-    if (L.total > 64 * 1024) return fail(PLK_ERR_UNSUPPORTED, "sweep LDS need %d bytes", L.total);
+    constexpr int kLdsMax = 64 * 1024;
+    const SweepSmem L0(N, C, in->d.state_unknown, in->table_size, a.nedges, (int)sizeof(T), 0);
+    if (L0.total > kLdsMax) return fail(PLK_ERR_UNSUPPORTED, "sweep LDS need %d bytes", L0.total);
     int nb = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, N, C>, kThreads, L.total));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, N, C>, kThreads, L0.total));
     if (nb < 1) return fail(PLK_ERR_UNSUPPORTED, "sweep kernel does not fit a CU");
     const int PB = kThreads / C;
     const int ntiles = in->Ppad / PB;
-    int grid = ctx->num_cu * std::min(nb, ctx->sweep_blocks_per_cu);
+    const int per_cu = std::min(nb, ctx->sweep_blocks_per_cu);
+    int grid = ctx->num_cu * per_cu;
     grid = std::max(1, std::min(grid, std::min(ntiles, 2 * kThreads)));   // the exchange polls 2 per thread
-    // the same number of tiles for every workgroup: each exchange waits for
-    // the busiest one, so extra workgroups with a tile less only add slots
+    // theta (and the per-pattern scalars) of each workgroup's tiles resident
+    // in LDS when they fit without lowering the workgroups per CU
+    int lds_tiles = 0, smem = L0.total;
+    if (ctx->sweep_lds) {
+        const int tiles = (ntiles + grid - 1) / grid;
+        const SweepSmem L1(N, C, in->d.state_unknown, in->table_size, a.nedges, (int)sizeof(T), tiles);
+        int nb1 = 0;
+        if (L1.total <= kLdsMax)
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb1, k_sweep<T, N, C>, kThreads, L1.total));
+        if (nb1 >= per_cu) {
+            lds_tiles = tiles;
+            smem = L1.total;
+        }
+    }
+    a.lds_tiles = lds_tiles;
+    // exactly per_cu workgroups on every CU: with room for one more, the
+    // dispatcher may stack three on some CUs and leave others with one, and
+    // every grid-wide exchange waits for the slowest CU
+    if (ctx->lds_per_cu > 0) {
+        const int pad = ctx->lds_per_cu / (per_cu + 1) + 256;
+        int nbp = 0;
+        if (pad > smem && pad <= kLdsMax &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbp, k_sweep<T, N, C>, kThreads, pad) == hipSuccess &&
+            nbp == per_cu)
+            smem = pad;
+    }
     plk_status st = grow(ctx, in, &in->sw_part, in->sw_part_cap, 6 * grid);   // three slot buffers
     if (st != PLK_OK) return st;
     HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(in->sw_part), kSlotEmptyWord,
@@ -1105,7 +1135,7 @@ plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& gri
     a.result = in->sw_out + 2;
     void* args[] = {&a};
     HIP_TRY(hipLaunchCooperativeKernel((const void*)k_sweep<T, N, C>, dim3(grid), dim3(kThreads),
-                                       args, (unsigned)L.total, ctx->stream));
+                                       args, (unsigned)smem, ctx->stream));
     grid_out = grid;
     return PLK_OK;
 }
@@ -1591,6 +1621,7 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
     a.nderv = in->sw_nderv;
     a.out = in->sw_out;
     a.timing = nullptr;
+    a.lds_tiles = 0;
     static const bool want_timing = std::getenv("PLK_SWEEP_TIMING") != nullptr;
     if (want_timing) {
         if (!in->sw_timing) {
@@ -1610,16 +1641,18 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
     HIP_TRY(hipMemcpyAsync(nd.data(), in->sw_nderv, sizeof(int) * nbranches, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(out, in->sw_out, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    in->theta_ready = true;
-    in->ptn_lh_ready = true;
+    // theta and pattern lnL hold the last branch only after a complete sweep
+    in->theta_ready = out[1] == 0.0;
+    in->ptn_lh_ready = out[1] == 0.0;
     if (a.timing) {
         long long t[PH_COUNT];
         int khz = 0;
         HIP_TRY(hipMemcpy(t, a.timing, sizeof(t), hipMemcpyDeviceToHost));
         HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
         const double us = 1e3 / (double)khz;
-        std::fprintf(stderr, "[plk sweep timing] grid %d: ops %.1f us, theta %.1f us, eval work %.1f us, "
-                     "exchange %.1f us, total %.1f us (cumulative)\n", grid, t[PH_OPS] * us,
+        std::fprintf(stderr, "[plk sweep timing] grid %d, LDS tiles %d: ops %.1f us, theta %.1f us, "
+                     "eval work %.1f us, exchange %.1f us, total %.1f us (cumulative)\n", grid,
+                     a.lds_tiles, t[PH_OPS] * us,
                      t[PH_THETA] * us, t[PH_EVAL_WORK] * us, t[PH_EXCHANGE] * us, t[PH_TOTAL] * us);
     }
     if (n_derv) std::memcpy(n_derv, nd.data(), sizeof(int) * nbranches);

@@ -55,6 +55,8 @@ struct SweepArgs {
     int* nderv;           // [nbranches]
     double* out;          // [0] lnL from buffer after the sweep, [1] status
     long long* timing;    // nullable: wall-clock ticks of workgroup 0 per phase
+    int lds_tiles;        // > 0: theta, buffer_scale, freq and invar of the workgroup's
+                          // tiles (at most lds_tiles) live in LDS for the whole launch
 };
 
 // phases accumulated in SweepArgs::timing (workgroup 0, thread 0)
@@ -177,8 +179,10 @@ __device__ inline bool grid_sum2(const SweepArgs& a, double v0, double v1, unsig
 
 // LDS layout shared by the host (size) and the kernel (offsets)
 struct SweepSmem {
-    int tip, ex, val, red, lens, ctl, inv, tab, total;
-    __host__ __device__ SweepSmem(int N, int C, int SU, int table_size, int nedges, int tsize) {
+    int tip, ex, val, red, lens, ctl, inv, tab, th, bs, fq, iv, total;
+    __host__ __device__ SweepSmem(int N, int C, int SU, int table_size, int nedges, int tsize,
+                                  int lds_tiles) {
+        const int pats = lds_tiles * (kThreads / C);   // patterns resident per workgroup
         int off = 0;
         tip = off;  off += (SU + 1) * N * 8;
         ex = off;   off += 2 * C * N * 8;
@@ -187,9 +191,26 @@ struct SweepSmem {
         lens = off; off += nedges * 8;
         ctl = off;  off += 8 * 8;
         inv = off;  off += (N * N * tsize + 15) / 16 * 16;
-        tab = off;  off += 2 * table_size * tsize;
+        tab = off;  off += (2 * table_size * tsize + 15) / 16 * 16;
+        bs = off;   off += pats * 8;
+        fq = off;   off += pats * 8;
+        iv = off;   off += pats * 8;
+        th = off;   off += pats * N * C * tsize;
         total = (off + 15) / 16 * 16;
     }
+};
+
+// Where a workgroup's per-pattern state lives: LDS copies for its own tiles
+// (resident pattern pl = local tile * PB + pattern in tile) or, with th ==
+// nullptr, the instance's global arrays. LDS theta is state-major,
+// th[i][local tile][thread], so a wave's reads of one state are contiguous.
+template <typename T>
+struct SweepRes {
+    T* th;
+    double* bs;
+    double* fq;
+    double* iv;
+    int stride;     // elements between states: resident tiles * kThreads
 };
 
 // Transition tables of both children of an op, built by the whole workgroup
@@ -234,8 +255,9 @@ __device__ inline void build_tables(const ModelDev& m, int SU, int table_size, c
 // at x2 (x2 < 0: only x), writing pattern lnL for x when store_ptn. Results
 // land in res[0..1] of every workgroup.
 template <typename T, int N, int C>
-__device__ inline bool sweep_eval(const SweepArgs& a, int mode, double x, double x2, bool store_ptn,
-                                  unsigned& epoch, double* val_sh, double* red_sh, double* res) {
+__device__ inline bool sweep_eval(const SweepArgs& a, const SweepRes<T>& R, int mode, double x, double x2,
+                                  bool store_ptn, unsigned& epoch, double* val_sh, double* red_sh,
+                                  double* res) {
     const long long t0 = wall_clock64();
     constexpr int PB = kThreads / C;
     constexpr int block = N * C;
@@ -258,16 +280,19 @@ __device__ inline bool sweep_eval(const SweepArgs& a, int mode, double x, double
     const int c = threadIdx.x % C;
     const int ntiles = I.Ppad / PB;
     double acc[2] = {0.0, 0.0};
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++) {
         const int p = tile * PB + threadIdx.x / C;
-        const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
+        const int pl = lt * PB + threadIdx.x / C;
+        const T* th = R.th ? R.th + lt * kThreads + threadIdx.x
+                           : reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
+        const int ts = R.th ? R.stride : 1;
         const double* v0 = val_sh + c * N;
         const double* v1 = val_sh + N * C + c * N;
         const double* v2 = val_sh + 2 * N * C + c * N;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-            const double t = (double)th[i];
+            const double t = (double)th[i * ts];
             s0 = fma(v0[i], t, s0);
             s1 = fma(v1[i], t, s1);
             if (mode == 0) s2 = fma(v2[i], t, s2);
@@ -276,19 +301,21 @@ __device__ inline bool sweep_eval(const SweepArgs& a, int mode, double x, double
         s1 = group_sum_ordered<C>(s1);
         if (mode == 0) s2 = group_sum_ordered<C>(s2);
         if (c == 0 && p < I.P) {
-            const double f = I.freq[p];
+            const double f = R.th ? R.fq[pl] : I.freq[p];
+            const double pinv = R.th ? R.iv[pl] : I.invar[p];
+            const double bsc = mode == 0 ? 0.0 : (R.th ? R.bs[pl] : I.buffer_scale[p]);
             if (mode == 0) {
-                const double lh = fabs(s0) + I.invar[p];
+                const double lh = fabs(s0) + pinv;
                 const double inv = 1.0 / lh;
                 const double df_frac = s1 * inv, ddf_frac = s2 * inv;
                 const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
                 acc[0] += tmp1;
                 acc[1] += tmp2 - tmp1 * df_frac;
             } else {
-                const double lnp = log(fabs(s0) + I.invar[p]) + I.buffer_scale[p];
+                const double lnp = log(fabs(s0) + pinv) + bsc;
                 if (store_ptn) I.ptn_lh[p] = lnp;
                 acc[0] += lnp * f;
-                if (x2 >= 0.0) acc[1] += (log(fabs(s1) + I.invar[p]) + I.buffer_scale[p]) * f;
+                if (x2 >= 0.0) acc[1] += (log(fabs(s1) + pinv) + bsc) * f;
             }
         }
     }
@@ -308,9 +335,10 @@ __device__ inline bool sweep_eval(const SweepArgs& a, int mode, double x, double
 // Newton evaluation (df, ddf) at x: the values summed are the stored theta
 // values, so the result equals a separate theta pass followed by sweep_eval
 template <typename T, int N, int C>
-__device__ inline bool theta_first_eval(const SweepArgs& a, plk_end node, plk_end dad, double x,
-                                        unsigned& epoch, const double* tip_sh, double* val_sh,
-                                        double* red_sh, double* res) {
+__device__ inline bool theta_first_eval(const SweepArgs& a, const SweepRes<T>& R, bool write_global,
+                                        plk_end node, plk_end dad, double x, unsigned& epoch,
+                                        const double* tip_sh, double* val_sh, double* red_sh,
+                                        double* res) {
     const long long t0 = wall_clock64();
     constexpr int PB = kThreads / C;
     constexpr int block = N * C;
@@ -328,12 +356,15 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, plk_end node, plk_en
     const int c = threadIdx.x % C;
     const int ntiles = I.Ppad / PB;
     double acc[2] = {0.0, 0.0};
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const bool to_global = !R.th || write_global;
+    for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++) {
         const int p = tile * PB + threadIdx.x / C;
+        const int pl = lt * PB + threadIdx.x / C;
         double nv[N], dv[N], adjust;
         int minsc;
         load_branch<T, N, C>(I, node, dad, tip_sh, p, c, nv, dv, minsc, adjust);
-        T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
+        T* thg = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
+        T* thl = R.th ? R.th + lt * kThreads + threadIdx.x : nullptr;
         const double* v0 = val_sh + c * N;
         const double* v1 = val_sh + N * C + c * N;
         const double* v2 = val_sh + 2 * N * C + c * N;
@@ -341,19 +372,24 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, plk_end node, plk_en
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const T tv = (T)(nv[i] * dv[i] * adjust);
-            th[i] = tv;
+            if (thl) thl[i * R.stride] = tv;
+            if (to_global) thg[i] = tv;
             const double t = (double)tv;
             s0 = fma(v0[i], t, s0);
             s1 = fma(v1[i], t, s1);
             s2 = fma(v2[i], t, s2);
         }
-        if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
+        if (c == 0) {
+            const double bsc = (double)minsc * Prec<T>::log_unit;
+            if (R.th) R.bs[pl] = bsc;
+            if (to_global) I.buffer_scale[p] = bsc;
+        }
         s0 = group_sum_ordered<C>(s0);
         s1 = group_sum_ordered<C>(s1);
         s2 = group_sum_ordered<C>(s2);
         if (c == 0 && p < I.P) {
-            const double f = I.freq[p];
-            const double lh = fabs(s0) + I.invar[p];
+            const double f = R.th ? R.fq[pl] : I.freq[p];
+            const double lh = fabs(s0) + (R.th ? R.iv[pl] : I.invar[p]);
             const double inv = 1.0 / lh;
             const double df_frac = s1 * inv, ddf_frac = s2 * inv;
             const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
@@ -377,7 +413,7 @@ template <typename T, int N, int C>
 __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const InstDev& I = a.I;
-    const SweepSmem L(N, C, I.SU, a.table_size, a.nedges, (int)sizeof(T));
+    const SweepSmem L(N, C, I.SU, a.table_size, a.nedges, (int)sizeof(T), a.lds_tiles);
     double* tip_sh = reinterpret_cast<double*>(smem + L.tip);
     double* ex_sh = reinterpret_cast<double*>(smem + L.ex);
     double* val_sh = reinterpret_cast<double*>(smem + L.val);
@@ -393,6 +429,20 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)a.m.inv_evec[t];
     for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) tip_sh[t] = a.m.tip[t];
     for (int t = threadIdx.x; t < a.nedges; t += kThreads) lens[t] = a.len[t];
+    SweepRes<T> R{nullptr, nullptr, nullptr, nullptr, 0};
+    if (a.lds_tiles > 0) {
+        R.th = reinterpret_cast<T*>(smem + L.th);
+        R.bs = reinterpret_cast<double*>(smem + L.bs);
+        R.fq = reinterpret_cast<double*>(smem + L.fq);
+        R.iv = reinterpret_cast<double*>(smem + L.iv);
+        R.stride = a.lds_tiles * kThreads;
+        // weights and +I mass of the resident patterns, constant for the launch
+        for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++)
+            for (int t = threadIdx.x; t < PB; t += kThreads) {
+                R.fq[lt * PB + t] = I.freq[tile * PB + t];
+                R.iv[lt * PB + t] = I.invar[tile * PB + t];
+            }
+    }
     __syncthreads();
 
     unsigned epoch = 0;
@@ -429,7 +479,11 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         rts = len0;
         if (rts < x1) rts = x1;
         if (rts > x2) rts = x2;
-        if (!theta_first_eval<T, N, C>(a, B.node, B.dad, rts, epoch, tip_sh, val_sh, red_sh, ctl)) {
+        // the instance keeps the last branch's theta (plk_likelihood_derv
+        // after the sweep); earlier ones live in LDS only
+        const bool last_branch = j == a.nbranches - 1;
+        if (!theta_first_eval<T, N, C>(a, R, last_branch, B.node, B.dad, rts, epoch, tip_sh, val_sh,
+                                       red_sh, ctl)) {
             status = PLK_ERR_DEVICE;
             break;
         }
@@ -462,7 +516,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
                     if (temp == rts) { result = rts; done = true; break; }
                 }
                 if (fabs(dx) < xacc || it == a.max_nr) { result = rts_old; done = true; break; }
-                if (!sweep_eval<T, N, C>(a, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+                if (!sweep_eval<T, N, C>(a, R, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
                     status = PLK_ERR_DEVICE;
                     break;
                 }
@@ -481,7 +535,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         double optx = result;
         // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625)
         if (optx > x2 * 0.95) {
-            if (!sweep_eval<T, N, C>(a, 1, optx, len0, false, epoch, val_sh, red_sh, ctl)) {
+            if (!sweep_eval<T, N, C>(a, R, 1, optx, len0, false, epoch, val_sh, red_sh, ctl)) {
                 status = PLK_ERR_DEVICE;
                 break;
             }
@@ -498,7 +552,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     // computeLikelihoodFromBuffer at the last branch
     if (status == 0 && a.nbranches > 0) {
         const double last = lens[a.br[a.nbranches - 1].edge];
-        if (!sweep_eval<T, N, C>(a, 1, last, -1.0, true, epoch, val_sh, red_sh, ctl))
+        if (!sweep_eval<T, N, C>(a, R, 1, last, -1.0, true, epoch, val_sh, red_sh, ctl))
             status = PLK_ERR_DEVICE;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -147,37 +147,95 @@ k_prep_root(ModelDev m, int n, int C, int SU, double length, int tip_dad, double
 }
 
 // ---------------------------------------------------------------------------
+// A pattern's C category values sit in C adjacent lanes (C = 1, 2, 4, 8).
+// lane_get<C, K>(v): the value of lane K of this lane's group. For C <= 4 the
+// group lies inside one quad and a DPP quad permutation moves the value (no
+// LDS round trip); 8 lanes use a bpermute.
+template <int C, int K>
+__device__ inline int lane_get_i32(int v) {
+    if constexpr (C == 1) {
+        return v;
+    } else if constexpr (C <= 4) {
+        constexpr int a = K, b = C == 4 ? K : 2 + K;   // C = 2: lanes {0,1} and {2,3}
+        constexpr int ctrl = a | (a << 2) | (b << 4) | (b << 6);
+        return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xf, 0xf, false);
+    } else {
+        const int base = (threadIdx.x & 63) & ~(C - 1);
+        return __shfl(v, base + K);
+    }
+}
+template <int C, int K>
+__device__ inline double lane_get(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = lane_get_i32<C, K>((int)(unsigned)(x & 0xffffffffll));
+    const int hi = lane_get_i32<C, K>((int)(x >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// max over the 16 lanes of this lane's DPP row (every lane gets it): quad
+// swaps, then half-row and row mirrors pair each quad with the others
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ inline float dpp_move(float v) { return dpp_f32<CTRL>(v); }
+template <int CTRL>
+__device__ inline double dpp_move(double v) { return dpp_f64<CTRL>(v); }
+template <typename T>
+__device__ inline T row16_max(T m) {
+    m = fmax(m, dpp_move<0xB1>(m));    // quad_perm [1,0,3,2]
+    m = fmax(m, dpp_move<0x4E>(m));    // quad_perm [2,3,0,1]
+    m = fmax(m, dpp_move<0x141>(m));   // row_half_mirror
+    m = fmax(m, dpp_move<0x140>(m));   // row_mirror
+    return m;
+}
+
+// the C values of this lane's group, in category order
+template <int C, int K = 0>
+__device__ inline void lane_gather(double v, double* out) {
+    if constexpr (K < C) {
+        out[K] = lane_get<C, K>(v);
+        lane_gather<C, K + 1>(v, out);
+    }
+}
+
 template <int C>
 __device__ inline double group_max(double v) {
+    double vals[C];
+    lane_gather<C>(v, vals);
+    double m = vals[0];
 #pragma unroll
-    for (int d = 1; d < C; d <<= 1) v = fmax(v, __shfl_xor(v, d));
-    return v;
+    for (int c = 1; c < C; c++) m = fmax(m, vals[c]);
+    return m;
 }
-template <int C>
+template <int C, int K = 0>
 __device__ inline int group_min_int(int v) {
-#pragma unroll
-    for (int d = 1; d < C; d <<= 1) v = min(v, __shfl_xor(v, d));
-    return v;
+    if constexpr (K + 1 < C) return min(lane_get_i32<C, K>(v), group_min_int<C, K + 1>(v));
+    else return lane_get_i32<C, K>(v);
 }
 // sum over the C lanes of a pattern in category order 0..C-1 (the order the
 // reference accumulates lh_cat), result in every lane of the group
 template <int C>
 __device__ inline double group_sum_ordered(double v) {
-    const int lane = threadIdx.x & 63;
-    const int base = lane & ~(C - 1);
-    double s = __shfl(v, base);
+    double vals[C];
+    lane_gather<C>(v, vals);
+    double s = vals[0];
 #pragma unroll
-    for (int c = 1; c < C; c++) s += __shfl(v, base + c);
+    for (int c = 1; c < C; c++) s += vals[c];
     return s;
 }
 // sumVec<VectorClass, true> grouping over categories (tree/phylokernelnew.h:43-71)
 template <int C>
 __device__ inline double group_sum_sumvec(double v) {
-    const int lane = threadIdx.x & 63;
-    const int base = lane & ~(C - 1);
     double vals[C];
-#pragma unroll
-    for (int c = 0; c < C; c++) vals[c] = __shfl(v, base + c);
+    lane_gather<C>(v, vals);
     double x = 0.0;
     constexpr int cruft = C & 3;
     if (cruft < C) {
@@ -227,57 +285,79 @@ __device__ inline void block_sum(double (&v)[NV], double* sh) {
 // tmp[x] = prod_children (tip: leaf[s][c][x]; internal: sum_i E[c][x][i] child[i]),
 // dad = U^-1 tmp, then the reference's rescaling rule. tL / tR are the
 // children's tables (tip lookup or transition rows), a tip child goes left.
+// Split into the loads (op_load) and the arithmetic and stores (op_finish)
+// so that a caller can have the loads of several patterns in flight.
+template <typename T, int N>
+struct OpIn {
+    T l[N], r[N];   // internal children's partials (unused for a tip child)
+    int sl, sr;     // tip children's states
+    int scl, scr;   // internal children's scale counters
+};
+
 template <typename T, int N, int C>
-__device__ inline void op_apply(const InstDev& I, const T* inv_sh, const T* tL, const T* tR,
-                                int l_tip, int l_id, int r_tip, int r_id, int dad_slot, int p,
-                                int c) {
+__device__ inline void op_load(const InstDev& I, int l_tip, int l_id, int r_tip, int r_id, int p, int c,
+                               OpIn<T, N>& in) {
+    using ST = typename Prec<T>::scale_t;
+    constexpr int block = N * C;
+    const T* partial = reinterpret_cast<const T*>(I.partial);
+    const ST* scale = reinterpret_cast<const ST*>(I.scale);
+    const size_t sidx = I.safe ? (size_t)p * C + c : (size_t)p;
+    in.scl = in.scr = 0;
+    if (l_tip) {
+        in.sl = I.states[(size_t)l_id * I.Ppad + p];
+    } else {
+        const T* src = partial + (size_t)l_id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) in.l[i] = src[i];
+        in.scl = scale[(size_t)l_id * I.scale_stride + sidx];
+    }
+    if (r_tip) {
+        in.sr = I.states[(size_t)r_id * I.Ppad + p];
+    } else {
+        const T* src = partial + (size_t)r_id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) in.r[i] = src[i];
+        in.scr = scale[(size_t)r_id * I.scale_stride + sidx];
+    }
+}
+
+template <typename T, int N, int C>
+__device__ inline void op_finish(const InstDev& I, const T* inv_sh, const T* tL, const T* tR, int l_tip,
+                                 int r_tip, int dad_slot, int p, int c, const OpIn<T, N>& in) {
     using ST = typename Prec<T>::scale_t;
     constexpr int block = N * C;
     T* partial = reinterpret_cast<T*>(I.partial);
     ST* scale = reinterpret_cast<ST*>(I.scale);
     const bool safe = I.safe != 0;
     T tmp[N];
-    int sc_l = 0, sc_r = 0;
     const size_t sidx = safe ? (size_t)p * C + c : (size_t)p;
     if (l_tip) {
-        const int s = I.states[(size_t)l_id * I.Ppad + p];
-        const T* lut = tL + (s * C + c) * N;
+        const T* lut = tL + (in.sl * C + c) * N;
 #pragma unroll
         for (int x = 0; x < N; x++) tmp[x] = lut[x];
     } else {
-        const T* src = partial + (size_t)l_id * I.slot_stride + (size_t)p * block + c * N;
-        T cv[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) cv[i] = src[i];
         const T* E = tL + c * N * N;
 #pragma unroll
         for (int x = 0; x < N; x++) {
-            T acc = E[x * N] * cv[0];
+            T acc = E[x * N] * in.l[0];
 #pragma unroll
-            for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
+            for (int i = 1; i < N; i++) acc = fma(E[x * N + i], in.l[i], acc);
             tmp[x] = acc;
         }
-        sc_l = scale[(size_t)l_id * I.scale_stride + sidx];
     }
     if (r_tip) {
-        const int s = I.states[(size_t)r_id * I.Ppad + p];
-        const T* lut = tR + (s * C + c) * N;
+        const T* lut = tR + (in.sr * C + c) * N;
 #pragma unroll
         for (int x = 0; x < N; x++) tmp[x] *= lut[x];
     } else {
-        const T* src = partial + (size_t)r_id * I.slot_stride + (size_t)p * block + c * N;
-        T cv[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) cv[i] = src[i];
         const T* E = tR + c * N * N;
 #pragma unroll
         for (int x = 0; x < N; x++) {
-            T acc = E[x * N] * cv[0];
+            T acc = E[x * N] * in.r[0];
 #pragma unroll
-            for (int i = 1; i < N; i++) acc = fma(E[x * N + i], cv[i], acc);
+            for (int i = 1; i < N; i++) acc = fma(E[x * N + i], in.r[i], acc);
             tmp[x] *= acc;
         }
-        sc_r = scale[(size_t)r_id * I.scale_stride + sidx];
     }
     // back to the eigen basis: dad[i] = sum_x inv_evec[i][x] tmp[x]
     T dadv[N];
@@ -290,7 +370,7 @@ __device__ inline void op_apply(const InstDev& I, const T* inv_sh, const T* tL, 
         dadv[i] = acc;
         lmax = fmax(lmax, fabs(acc));
     }
-    int sc = sc_l + sc_r;
+    int sc = in.scl + in.scr;
     if (!(l_tip && r_tip)) {   // tip-tip steps are never rescaled
         const bool no_invar = I.invar[p] == 0.0;
         const T m = safe ? lmax : (T)group_max<C>((double)lmax);
@@ -305,6 +385,15 @@ __device__ inline void op_apply(const InstDev& I, const T* inv_sh, const T* tL, 
 #pragma unroll
     for (int i = 0; i < N; i++) dst[i] = dadv[i];
     if (safe || c == 0) scale[(size_t)dad_slot * I.scale_stride + sidx] = (ST)sc;
+}
+
+template <typename T, int N, int C>
+__device__ inline void op_apply(const InstDev& I, const T* inv_sh, const T* tL, const T* tR,
+                                int l_tip, int l_id, int r_tip, int r_id, int dad_slot, int p,
+                                int c) {
+    OpIn<T, N> in;
+    op_load<T, N, C>(I, l_tip, l_id, r_tip, r_id, p, c, in);
+    op_finish<T, N, C>(I, inv_sh, tL, tR, l_tip, r_tip, dad_slot, p, c, in);
 }
 
 // Branch ends for (p, c): node partial nv, dad partial (or tip vector from

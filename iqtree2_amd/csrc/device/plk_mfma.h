@@ -177,8 +177,7 @@ __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict_
                                 m = fmax(m, fabs(d[ib][r]));
                             }
                         }
-#pragma unroll
-                        for (int dl = 1; dl < 16; dl <<= 1) m = fmax(m, __shfl_xor(m, dl));
+                        m = row16_max(m);   // over the 16 lanes (states) of this pattern row
                         if (col == 0) mx_sh[(sub * C + c) * 16 + M::row(g, r)] = m;
                     }
                 }

@@ -33,6 +33,10 @@
 namespace plkdev {
 
 constexpr long kSpinLimit = 1L << 24;   // ~0.5 s of polling before giving up
+// pattern tiles whose branch-end loads a workgroup keeps in flight together
+// in the theta pass (more did not pay for the partial updates)
+template <int N, int C>
+constexpr int kThetaTiles = N <= 4 ? 2 : 1;
 constexpr int kFlagStride = 32;         // one 128-byte line per flag
 constexpr unsigned long long kSlotEmpty = 0x7FF4DEAD7FF4DEADull;   // signalling NaN
 constexpr unsigned kSlotEmptyWord = 0x7FF4DEADu;                   // its 32-bit halves
@@ -201,15 +205,21 @@ struct SweepSmem {
 };
 
 // Where a workgroup's per-pattern state lives: LDS copies for its own tiles
-// (resident pattern pl = local tile * PB + pattern in tile) or, with th ==
-// nullptr, the instance's global arrays. LDS theta is state-major,
-// th[i][local tile][thread], so a wave's reads of one state are contiguous.
+// (resident pattern pl = local tile * PB + pattern in tile) when on, the
+// instance's global arrays otherwise. The pointers always point into the
+// workgroup's LDS (so every access through them is an LDS access) and are
+// only used when on. LDS theta is state-major, th[i][local tile][thread],
+// so a wave's reads of one state are contiguous.
+template <typename T>
+using Lds = __attribute__((address_space(3))) T;   // LDS-qualified (ds_* accesses only)
+
 template <typename T>
 struct SweepRes {
-    T* th;
-    double* bs;
-    double* fq;
-    double* iv;
+    bool on;
+    Lds<T>* th;
+    Lds<double>* bs;
+    Lds<double>* fq;
+    Lds<double>* iv;
     int stride;     // elements between states: resident tiles * kThreads
 };
 
@@ -283,16 +293,23 @@ __device__ inline bool sweep_eval(const SweepArgs& a, const SweepRes<T>& R, int 
     for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++) {
         const int p = tile * PB + threadIdx.x / C;
         const int pl = lt * PB + threadIdx.x / C;
-        const T* th = R.th ? R.th + lt * kThreads + threadIdx.x
-                           : reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
-        const int ts = R.th ? R.stride : 1;
+        T tv[N];
+        if (R.on) {
+            const Lds<T>* th = R.th + lt * kThreads + threadIdx.x;
+#pragma unroll
+            for (int i = 0; i < N; i++) tv[i] = th[i * R.stride];
+        } else {
+            const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) tv[i] = th[i];
+        }
         const double* v0 = val_sh + c * N;
         const double* v1 = val_sh + N * C + c * N;
         const double* v2 = val_sh + 2 * N * C + c * N;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-            const double t = (double)th[i * ts];
+            const double t = (double)tv[i];
             s0 = fma(v0[i], t, s0);
             s1 = fma(v1[i], t, s1);
             if (mode == 0) s2 = fma(v2[i], t, s2);
@@ -301,9 +318,9 @@ __device__ inline bool sweep_eval(const SweepArgs& a, const SweepRes<T>& R, int 
         s1 = group_sum_ordered<C>(s1);
         if (mode == 0) s2 = group_sum_ordered<C>(s2);
         if (c == 0 && p < I.P) {
-            const double f = R.th ? R.fq[pl] : I.freq[p];
-            const double pinv = R.th ? R.iv[pl] : I.invar[p];
-            const double bsc = mode == 0 ? 0.0 : (R.th ? R.bs[pl] : I.buffer_scale[p]);
+            const double f = R.on ? R.fq[pl] : I.freq[p];
+            const double pinv = R.on ? R.iv[pl] : I.invar[p];
+            const double bsc = mode == 0 ? 0.0 : (R.on ? R.bs[pl] : I.buffer_scale[p]);
             if (mode == 0) {
                 const double lh = fabs(s0) + pinv;
                 const double inv = 1.0 / lh;
@@ -356,15 +373,53 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, const SweepRes<T>& R
     const int c = threadIdx.x % C;
     const int ntiles = I.Ppad / PB;
     double acc[2] = {0.0, 0.0};
-    const bool to_global = !R.th || write_global;
-    for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++) {
+    const bool to_global = !R.on || write_global;
+    using ST = typename Prec<T>::scale_t;
+    const T* partial = reinterpret_cast<const T*>(I.partial);
+    const ST* scale = reinterpret_cast<const ST*>(I.scale);
+    constexpr int TM = kThetaTiles<N, C>;
+    for (int t0 = blockIdx.x, lt0 = 0; t0 < ntiles; t0 += TM * gridDim.x, lt0 += TM) {
+    // the loads of TM tiles in flight first (load_branch's reads)
+    T nr[TM][N], dr[TM][N];
+    int scr[TM], sr[TM];
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int tile = t0 + m * gridDim.x;
+        if (tile >= ntiles) break;
+        const int p = tile * PB + threadIdx.x / C;
+        const size_t sidx = I.safe ? (size_t)p * C + c : (size_t)p;
+        const T* nsrc = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+        for (int i = 0; i < N; i++) nr[m][i] = nsrc[i];
+        scr[m] = scale[(size_t)node.id * I.scale_stride + sidx];
+        if (dad.is_tip) {
+            sr[m] = I.states[(size_t)dad.id * I.Ppad + p];
+        } else {
+            const T* dsrc = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) dr[m][i] = dsrc[i];
+            scr[m] += scale[(size_t)dad.id * I.scale_stride + sidx];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int tile = t0 + m * gridDim.x, lt = lt0 + m;
+        if (tile >= ntiles) break;
         const int p = tile * PB + threadIdx.x / C;
         const int pl = lt * PB + threadIdx.x / C;
-        double nv[N], dv[N], adjust;
-        int minsc;
-        load_branch<T, N, C>(I, node, dad, tip_sh, p, c, nv, dv, minsc, adjust);
+        double nv[N], dv[N], adjust = 1.0;
+        int minsc = scr[m];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            nv[i] = (double)nr[m][i];
+            dv[i] = dad.is_tip ? tip_sh[sr[m] * N + i] : (double)dr[m][i];
+        }
+        if (I.safe) {
+            minsc = group_min_int<C>(scr[m]);
+            adjust = Prec<T>::safe_adjust(scr[m] - minsc);
+        }
         T* thg = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
-        T* thl = R.th ? R.th + lt * kThreads + threadIdx.x : nullptr;
+        Lds<T>* thl = R.th + lt * kThreads + threadIdx.x;
         const double* v0 = val_sh + c * N;
         const double* v1 = val_sh + N * C + c * N;
         const double* v2 = val_sh + 2 * N * C + c * N;
@@ -372,7 +427,7 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, const SweepRes<T>& R
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const T tv = (T)(nv[i] * dv[i] * adjust);
-            if (thl) thl[i * R.stride] = tv;
+            if (R.on) thl[i * R.stride] = tv;
             if (to_global) thg[i] = tv;
             const double t = (double)tv;
             s0 = fma(v0[i], t, s0);
@@ -381,21 +436,22 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, const SweepRes<T>& R
         }
         if (c == 0) {
             const double bsc = (double)minsc * Prec<T>::log_unit;
-            if (R.th) R.bs[pl] = bsc;
+            if (R.on) R.bs[pl] = bsc;
             if (to_global) I.buffer_scale[p] = bsc;
         }
         s0 = group_sum_ordered<C>(s0);
         s1 = group_sum_ordered<C>(s1);
         s2 = group_sum_ordered<C>(s2);
         if (c == 0 && p < I.P) {
-            const double f = R.th ? R.fq[pl] : I.freq[p];
-            const double lh = fabs(s0) + (R.th ? R.iv[pl] : I.invar[p]);
+            const double f = R.on ? R.fq[pl] : I.freq[p];
+            const double lh = fabs(s0) + (R.on ? R.iv[pl] : I.invar[p]);
             const double inv = 1.0 / lh;
             const double df_frac = s1 * inv, ddf_frac = s2 * inv;
             const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
             acc[0] += tmp1;
             acc[1] += tmp2 - tmp1 * df_frac;
         }
+    }
     }
     block_sum<2>(acc, red_sh);
     epoch++;
@@ -429,13 +485,9 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)a.m.inv_evec[t];
     for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) tip_sh[t] = a.m.tip[t];
     for (int t = threadIdx.x; t < a.nedges; t += kThreads) lens[t] = a.len[t];
-    SweepRes<T> R{nullptr, nullptr, nullptr, nullptr, 0};
-    if (a.lds_tiles > 0) {
-        R.th = reinterpret_cast<T*>(smem + L.th);
-        R.bs = reinterpret_cast<double*>(smem + L.bs);
-        R.fq = reinterpret_cast<double*>(smem + L.fq);
-        R.iv = reinterpret_cast<double*>(smem + L.iv);
-        R.stride = a.lds_tiles * kThreads;
+    SweepRes<T> R{a.lds_tiles > 0, (Lds<T>*)(smem + L.th), (Lds<double>*)(smem + L.bs),
+                  (Lds<double>*)(smem + L.fq), (Lds<double>*)(smem + L.iv), a.lds_tiles * kThreads};
+    if (R.on) {
         // weights and +I mass of the resident patterns, constant for the launch
         for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++)
             for (int t = threadIdx.x; t < PB; t += kThreads) {

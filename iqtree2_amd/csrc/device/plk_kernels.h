@@ -254,14 +254,33 @@ __device__ inline double group_sum_sumvec(double v) {
     return x;
 }
 
-// Deterministic workgroup sum: xor-tree inside each wavefront, then the
+// Sum over the 64 lanes of a wavefront with the same bits in every lane:
+// DPP quad swaps and row mirrors (each step adds the same two operands in
+// both partner lanes) give every lane its 16-lane row sum, then the four row
+// sums are read as scalars and added in a fixed order.
+__device__ inline double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    const long long x = __double_as_longlong(v);
+    const int lo = (int)(unsigned)(x & 0xffffffffll), hi = (int)(x >> 32);
+    double r[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const unsigned l = (unsigned)__builtin_amdgcn_readlane(lo, 16 * k);
+        const unsigned h = (unsigned)__builtin_amdgcn_readlane(hi, 16 * k);
+        r[k] = __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
+    }
+    return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
+// Deterministic workgroup sum: wave_sum inside each wavefront, then the
 // wavefront partials in index order. Result valid in thread 0.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double* sh) {
 #pragma unroll
-    for (int j = 0; j < NV; j++)
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) v[j] += __shfl_xor(v[j], d);
+    for (int j = 0; j < NV; j++) v[j] = wave_sum(v[j]);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __syncthreads();
     if (lane == 0)

@@ -116,14 +116,11 @@ __device__ inline bool wait_slots(const double* const* slot, int n, double* out,
 }
 
 // Fixed-order sum over the workgroup's 256 partial sums; res[0..1] in LDS.
-// The xor butterfly gives every lane the same bits (each level adds the same
-// two operands, in either order), the four wave sums are added in wave order.
+// wave_sum gives every lane of a wave the same bits, the four wave sums are
+// added in wave order.
 __device__ inline void tree_sum2(double s0, double s1, double* red_sh, double* res) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        s0 += __shfl_xor(s0, d);
-        s1 += __shfl_xor(s1, d);
-    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red_sh[2 * wave] = s0;

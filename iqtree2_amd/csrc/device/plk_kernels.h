@@ -276,13 +276,15 @@ __device__ inline double wave_sum(double v) {
 }
 
 // Deterministic workgroup sum: wave_sum inside each wavefront, then the
-// wavefront partials in index order. Result valid in thread 0.
-template <int NV>
+// wavefront partials in index order. Result valid in thread 0. GUARD = false
+// skips the barrier that protects sh from a previous reader, for callers
+// whose last use of sh already ended with a barrier.
+template <int NV, bool GUARD = true>
 __device__ inline void block_sum(double (&v)[NV], double* sh) {
 #pragma unroll
     for (int j = 0; j < NV; j++) v[j] = wave_sum(v[j]);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
+    if (GUARD) __syncthreads();
     if (lane == 0)
 #pragma unroll
         for (int j = 0; j < NV; j++) sh[wave * NV + j] = v[j];

@@ -37,6 +37,9 @@ constexpr long kSpinLimit = 1L << 24;   // ~0.5 s of polling before giving up
 // in the theta pass (more did not pay for the partial updates)
 template <int N, int C>
 constexpr int kThetaTiles = N <= 4 ? 2 : 1;
+// tiles of LDS-resident theta read together by a derivative evaluation
+template <int N>
+constexpr int kEvalTiles = N <= 4 ? 4 : 1;
 constexpr int kFlagStride = 32;         // one 128-byte line per flag
 constexpr unsigned long long kSlotEmpty = 0x7FF4DEAD7FF4DEADull;   // signalling NaN
 constexpr unsigned kSlotEmptyWord = 0x7FF4DEADu;                   // its 32-bit halves
@@ -287,29 +290,111 @@ __device__ inline bool sweep_eval(const SweepArgs& a, const SweepRes<T>& R, int 
     const int c = threadIdx.x % C;
     const int ntiles = I.Ppad / PB;
     double acc[2] = {0.0, 0.0};
-    for (int tile = blockIdx.x, lt = 0; tile < ntiles; tile += gridDim.x, lt++) {
+    if constexpr (N <= 4 && C <= 4) {
+        if (R.on) {
+            // one thread per resident pattern: the thread adds its pattern's
+            // category sums itself, in category order (as group_sum_ordered)
+            const int mine = blockIdx.x < (unsigned)ntiles ? (ntiles - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+            // the evaluation's coefficients in registers when they are few
+            constexpr bool kRegW = C * N <= 16;
+            constexpr int NW = kRegW ? C * N : 1;
+            double w[3][NW];
+            if constexpr (kRegW) {
+#pragma unroll
+                for (int k = 0; k < NW; k++) {
+                    w[0][k] = val_sh[k];
+                    w[1][k] = val_sh[N * C + k];
+                    w[2][k] = mode == 0 ? val_sh[2 * N * C + k] : 0.0;
+                }
+            }
+            for (int pl = threadIdx.x; pl < mine * PB; pl += kThreads) {
+                const int lt = pl / PB, q = pl - lt * PB;
+                const int p = ((int)blockIdx.x + lt * (int)gridDim.x) * PB + q;
+                const Lds<T>* th = R.th + lt * kThreads + q * C;
+                double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+#pragma unroll
+                for (int cc = 0; cc < C; cc++) {
+                    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; i++) {
+                        const double t = (double)th[i * R.stride + cc];
+                        const int k = cc * N + i;
+                        s0 = fma(kRegW ? w[0][kRegW ? k : 0] : val_sh[k], t, s0);
+                        s1 = fma(kRegW ? w[1][kRegW ? k : 0] : val_sh[N * C + k], t, s1);
+                        if (mode == 0) s2 = fma(kRegW ? w[2][kRegW ? k : 0] : val_sh[2 * N * C + k], t, s2);
+                    }
+                    if (cc == 0) { S0 = s0; S1 = s1; S2 = s2; }
+                    else { S0 += s0; S1 += s1; S2 += s2; }
+                }
+                if (p < I.P) {
+                    const double f = R.fq[pl], pinv = R.iv[pl];
+                    if (mode == 0) {
+                        const double lh = fabs(S0) + pinv;
+                        const double inv = 1.0 / lh;
+                        const double df_frac = S1 * inv, ddf_frac = S2 * inv;
+                        const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
+                        acc[0] += tmp1;
+                        acc[1] += tmp2 - tmp1 * df_frac;
+                    } else {
+                        const double bsc = R.bs[pl];
+                        const double lnp = log(fabs(S0) + pinv) + bsc;
+                        if (store_ptn) I.ptn_lh[p] = lnp;
+                        acc[0] += lnp * f;
+                        if (x2 >= 0.0) acc[1] += (log(fabs(S1) + pinv) + bsc) * f;
+                    }
+                }
+            }
+            goto reduce;
+        }
+    }
+    {
+    // this thread's category coefficients in registers (small state spaces)
+    constexpr bool kHoist = N <= 4;
+    double cf[3][kHoist ? N : 1];
+    if constexpr (kHoist) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            cf[0][i] = val_sh[c * N + i];
+            cf[1][i] = val_sh[N * C + c * N + i];
+            cf[2][i] = mode == 0 ? val_sh[2 * N * C + c * N + i] : 0.0;
+        }
+    }
+    // the theta reads of TM tiles issued together, then each tile in order
+    constexpr int TM = kEvalTiles<N>;
+    for (int t0 = blockIdx.x, lt0 = 0; t0 < ntiles; t0 += TM * gridDim.x, lt0 += TM) {
+    T tvs[TM][N];
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int tile = t0 + m * gridDim.x;
+        if (tile >= ntiles) break;
+        if (R.on) {
+            const Lds<T>* th = R.th + (lt0 + m) * kThreads + threadIdx.x;
+#pragma unroll
+            for (int i = 0; i < N; i++) tvs[m][i] = th[i * R.stride];
+        } else {
+            const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)(tile * PB + threadIdx.x / C) * block + c * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) tvs[m][i] = th[i];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < TM; m++) {
+        const int tile = t0 + m * gridDim.x, lt = lt0 + m;
+        if (tile >= ntiles) break;
         const int p = tile * PB + threadIdx.x / C;
         const int pl = lt * PB + threadIdx.x / C;
-        T tv[N];
-        if (R.on) {
-            const Lds<T>* th = R.th + lt * kThreads + threadIdx.x;
-#pragma unroll
-            for (int i = 0; i < N; i++) tv[i] = th[i * R.stride];
-        } else {
-            const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
-#pragma unroll
-            for (int i = 0; i < N; i++) tv[i] = th[i];
-        }
-        const double* v0 = val_sh + c * N;
-        const double* v1 = val_sh + N * C + c * N;
-        const double* v2 = val_sh + 2 * N * C + c * N;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-            const double t = (double)tv[i];
-            s0 = fma(v0[i], t, s0);
-            s1 = fma(v1[i], t, s1);
-            if (mode == 0) s2 = fma(v2[i], t, s2);
+            const double t = (double)tvs[m][i];
+            const double w0 = kHoist ? cf[0][kHoist ? i : 0] : val_sh[c * N + i];
+            const double w1 = kHoist ? cf[1][kHoist ? i : 0] : val_sh[N * C + c * N + i];
+            s0 = fma(w0, t, s0);
+            s1 = fma(w1, t, s1);
+            if (mode == 0) {
+                const double w2 = kHoist ? cf[2][kHoist ? i : 0] : val_sh[2 * N * C + c * N + i];
+                s2 = fma(w2, t, s2);
+            }
         }
         s0 = group_sum_ordered<C>(s0);
         s1 = group_sum_ordered<C>(s1);
@@ -333,7 +418,10 @@ __device__ inline bool sweep_eval(const SweepArgs& a, const SweepRes<T>& R, int 
             }
         }
     }
-    block_sum<2>(acc, red_sh);
+    }
+    }
+reduce:
+    block_sum<2, false>(acc, red_sh);   // red_sh: last read before tree_sum2's final barrier
     epoch++;
     const long long t1 = wall_clock64();
     const bool ok = grid_sum2(a, acc[0], acc[1], epoch, red_sh, res);
@@ -450,7 +538,7 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, const SweepRes<T>& R
         }
     }
     }
-    block_sum<2>(acc, red_sh);
+    block_sum<2, false>(acc, red_sh);   // red_sh: last read before tree_sum2's final barrier
     epoch++;
     const long long t1 = wall_clock64();
     const bool ok = grid_sum2(a, acc[0], acc[1], epoch, red_sh, res);

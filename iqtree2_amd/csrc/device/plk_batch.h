@@ -216,9 +216,10 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
         T* inv_m = reinterpret_cast<T*>(smem);
         T* tab_m = inv_m + L::kInv;
         T* red_m = tab_m + 2 * L::kTab;
-        stage_inv<T>(it.m.inv_evec, inv_m, N, L::S);
-        mfma_ops_tile<T, N, C, kBatchSub>(I, it.ops, it.nops, tables, it.table_size, inv_m, tab_m, red_m,
-                                  tile * PB);
+        stage_inv<T, N>(it.m.inv_evec, inv_m);
+        // prefetched table staging costs FP64 an occupancy step here
+        mfma_ops_tile<T, N, C, kBatchSub, sizeof(T) == 4>(I, it.ops, it.nops, tables, it.table_size, inv_m,
+                                                         tab_m, red_m, tile * PB);
     } else {
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)it.m.inv_evec[t];
     for (int o = 0; o < it.nops; o++) {

@@ -25,6 +25,31 @@
 
 namespace plkdev {
 
+// Workgroup copy of n <= NMAX elements, src[t] -> dst[map(t)]: every thread
+// issues all its loads before its LDS stores (a strided loop with a
+// data-dependent trip count pays one L2 round trip per element otherwise).
+// CHUNK bounds the loads a thread holds at once (registers).
+template <int NMAX, int CHUNK = 64, typename TD, typename TS, typename Map>
+__device__ inline void stage_lds(TD* dst, const TS* __restrict__ src, int n, Map map) {
+    constexpr int IT = (NMAX + kThreads - 1) / kThreads;
+    constexpr int CH = IT < CHUNK ? IT : CHUNK;
+#pragma unroll
+    for (int j0 = 0; j0 < IT; j0 += CH) {
+        TS v[CH];
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const int t = (int)threadIdx.x + (j0 + j) * kThreads;
+            if (j0 + j < IT && t < n) v[j] = src[t];
+        }
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const int t = (int)threadIdx.x + (j0 + j) * kThreads;
+            if (j0 + j < IT && t < n) dst[map(t)] = (TD)v[j];
+        }
+    }
+}
+
+
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
@@ -72,7 +97,7 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
     T* partial = reinterpret_cast<T*>(I.partial);
     ST* scale = reinterpret_cast<ST*>(I.scale);
 
-    for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[(t / N) * S + t % N] = (T)m.inv_evec[t];
+    stage_lds<N * N>(inv_sh, m.inv_evec, N * N, [](int t) { return (t / N) * S + t % N; });
     __syncthreads();
 
     for (int o = 0; o < nops; o++) {
@@ -102,7 +127,7 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
                 } else {
                     __syncthreads();   // earlier readers of e_sh are done
                     const T* e = gt[k] + (size_t)c * N * N;
-                    for (int t = threadIdx.x; t < N * N; t += kThreads) e_sh[(t / N) * S + t % N] = e[t];
+                    stage_lds<N * N>(e_sh, e, N * N, [](int t) { return (t / N) * S + t % N; });
                     __syncthreads();
                     const T* cv = partial + (size_t)ch[k].id * I.slot_stride + (size_t)pw * block + c * N;
 #pragma unroll

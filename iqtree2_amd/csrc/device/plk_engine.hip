@@ -144,6 +144,7 @@ struct plk_context {
     int sweep_blocks_per_cu = 2;
     bool sweep_lds = true;              // theta resident in LDS during sweeps when it fits
     int lds_per_cu = 0;                 // bytes of LDS per CU
+    int mfma_sub = 1;                   // 16-pattern sub-tiles per protein traversal workgroup
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev;
     size_t prof_next = 0;
@@ -499,11 +500,16 @@ plk_status launch_traverse_mfma_c(plk_context* ctx, Instance* in, int nops, int 
                                   plk_end dad) {
     constexpr int N = 20;
     if (nops > 0) {
-        const int grid = in->Ppad / kMfmaTile;
+        const int sub = ctx->mfma_sub;
+        const int grid = in->Ppad / (kMfmaTile * sub);
         const int pe = prof_begin(ctx);
-        hipLaunchKernelGGL((k_traverse_mfma<T, N, C>), dim3(grid), dim3(kThreads), 0, ctx->stream,
-                           inst_dev(in), model_dev(in), in->ops, nops,
-                           reinterpret_cast<const T*>(in->tables), in->table_size);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, ctx->stream, inst_dev(in), model_dev(in),
+                               in->ops, nops, reinterpret_cast<const T*>(in->tables), in->table_size);
+        };
+        if (sub == 4) go(k_traverse_mfma<T, N, C, 4>);
+        else if (sub == 2) go(k_traverse_mfma<T, N, C, 2>);
+        else go(k_traverse_mfma<T, N, C, 1>);
         HIP_TRY(hipGetLastError());
         prof_end(ctx, pe, K_TRAVERSE_MFMA, in->staged_bytes * in->d.npatterns,
                  in->staged_flops * in->d.npatterns);
@@ -720,6 +726,10 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
     HIP_TRY(hipDeviceGetAttribute(&ctx->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     if (const char* e = std::getenv("PLK_SWEEP_BLOCKS_PER_CU")) ctx->sweep_blocks_per_cu = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PLK_SWEEP_LDS")) ctx->sweep_lds = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PLK_MFMA_SUB")) {
+        const int v = std::atoi(e);
+        ctx->mfma_sub = v == 4 ? 4 : v == 2 ? 2 : 1;
+    }
     if (const char* e = std::getenv("PLK_BATCH_BLOCKS_PER_CU")) ctx->batch_blocks_per_cu = std::max(1, std::atoi(e));
     if (std::getenv("PLK_NO_AUTOTUNE")) ctx->autotune = false;
     ctx->budget = (cfg && cfg->hbm_budget) ? cfg->hbm_budget : (size_t)(0.9 * (double)free_b);

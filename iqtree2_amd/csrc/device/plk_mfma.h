@@ -31,9 +31,10 @@ struct MfmaSmall {
     static_assert(kBytes <= 64 * 1024, "tables must fit the workgroup's LDS");
 };
 
-template <typename T>
-__device__ inline void stage_inv(const double* __restrict__ inv, T* inv_sh, int N, int S) {
-    for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[(t / N) * S + t % N] = (T)inv[t];
+template <typename T, int N>
+__device__ inline void stage_inv(const double* __restrict__ inv, T* inv_sh) {
+    constexpr int S = N + 1;
+    stage_lds<N * N>(inv_sh, inv, N * N, [](int t) { return (t / N) * S + t % N; });
 }
 
 // Patterns per workgroup: 16, one MFMA N tile; the four waves split the
@@ -49,8 +50,10 @@ constexpr int kMfmaTile = 16;
 // are read back, scaled and stored again, and the scale counters written.
 // The stored values equal scaling the computed ones directly. U^-1 is staged
 // in inv_sh (row stride N + 1); tab_sh receives the two child tables of each
-// step (all categories); mx_sh holds SUB x C x 16 maxima.
-template <typename T, int N, int C, int SUB>
+// step (all categories); mx_sh holds SUB x C x 16 maxima. PREFETCH stages
+// the tables with every thread's loads issued first (one L2 round trip
+// instead of one per element; it costs registers).
+template <typename T, int N, int C, int SUB, bool PREFETCH = true>
 __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict__ ops, int nops,
                                      const T* __restrict__ tables, int table_size, const T* inv_sh,
                                      T* tab_sh, T* mx_sh, int p_first) {
@@ -83,10 +86,17 @@ __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict_
         __syncthreads();
         for (int k = 0; k < 2; k++) {
             if (ch[k].is_tip) continue;
-            T* tdst = tab_sh + k * L::kTab;
-            for (int t = threadIdx.x; t < C * N * N; t += kThreads) {
-                const int c = t / (N * N), r = t - c * N * N;
-                tdst[c * N * S + (r / N) * S + r % N] = gt[k][t];
+            if constexpr (PREFETCH) {
+                stage_lds<C * N * N>(tab_sh + k * L::kTab, gt[k], C * N * N, [](int t) {
+                    const int c = t / (N * N), r = t - c * N * N;
+                    return c * N * S + (r / N) * S + r % N;
+                });
+            } else {
+                T* tdst = tab_sh + k * L::kTab;
+                for (int t = threadIdx.x; t < C * N * N; t += kThreads) {
+                    const int c = t / (N * N), r = t - c * N * N;
+                    tdst[c * N * S + (r / N) * S + r % N] = gt[k][t];
+                }
             }
         }
         __syncthreads();
@@ -231,17 +241,18 @@ __device__ inline void mfma_ops_tile(const InstDev& I, const plk_op* __restrict_
     }
 }
 
-template <typename T, int N, int C>
+// SUB 16-pattern sub-tiles per workgroup share each step's staged tables
+template <typename T, int N, int C, int SUB>
 __global__ void __launch_bounds__(kThreads)
 k_traverse_mfma(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
                 const T* __restrict__ tables, int table_size) {
     using L = MfmaSmall<T, N, C>;
     __shared__ T inv_sh[L::kInv];
     __shared__ T tab_sh[2 * L::kTab];
-    __shared__ T mx_sh[C * 16];
-    stage_inv<T>(m.inv_evec, inv_sh, N, L::S);
-    mfma_ops_tile<T, N, C, 1>(I, ops, nops, tables, table_size, inv_sh, tab_sh, mx_sh,
-                              blockIdx.x * kMfmaTile);
+    __shared__ T mx_sh[SUB * C * 16];
+    stage_inv<T, N>(m.inv_evec, inv_sh);
+    mfma_ops_tile<T, N, C, SUB>(I, ops, nops, tables, table_size, inv_sh, tab_sh, mx_sh,
+                                blockIdx.x * kMfmaTile * SUB);
 }
 
 }  // namespace plkdev

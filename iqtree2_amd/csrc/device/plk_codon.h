@@ -125,18 +125,26 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
                             a[xb][r] = x < N ? lut[x] : (T)0;
                         }
                 } else {
+                    // the child's values first (they do not wait for the
+                    // tables), then the tables staged while they arrive
+                    const T* cv = partial + (size_t)ch[k].id * I.slot_stride + (size_t)pw * block + c * N;
+                    T bv[16];
+#pragma unroll
+                    for (int st = 0; st < 16; st++) {
+                        const int kk = 4 * st + g;
+                        bv[st] = kk < N ? cv[kk] : (T)0;
+                    }
                     __syncthreads();   // earlier readers of e_sh are done
                     const T* e = gt[k] + (size_t)c * N * N;
                     stage_lds<N * N>(e_sh, e, N * N, [](int t) { return (t / N) * S + t % N; });
                     __syncthreads();
-                    const T* cv = partial + (size_t)ch[k].id * I.slot_stride + (size_t)pw * block + c * N;
 #pragma unroll
                     for (int xb = 0; xb < 4; xb++) a[xb] = acc_t{0, 0, 0, 0};
-#pragma unroll 4
+#pragma unroll
                     for (int st = 0; st < 16; st++) {
                         const int kk = 4 * st + g;
                         const bool kin = kk < N;
-                        const T b = kin ? cv[kk] : (T)0;
+                        const T b = bv[st];
 #pragma unroll
                         for (int xb = 0; xb < 4; xb++) {
                             const int x = 16 * xb + col;

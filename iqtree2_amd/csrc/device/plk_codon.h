@@ -214,6 +214,9 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
                 const int pp = t / N, i = t - pp * N;
                 dst[(size_t)(p0 + pp) * block + c * N + i] = stage[t];
             }
+            // the write-out stores other threads' patterns: the next step's
+            // child loads (issued before its first barrier) must follow it
+            __syncthreads();
         }
     }
 }

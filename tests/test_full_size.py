@@ -8,7 +8,8 @@ the oracle (which would take minutes at these sizes):
 - one optimizeAllBranches pass (the device-resident sweep) never lowers the
   lnL and leaves every length inside the bracket;
 - the FP32-storage form agrees with FP64 to FP32 rounding;
-- the batched partition launch equals one evaluation per partition.
+- the batched partition launch equals one evaluation per partition;
+- large 61-state requests give the same bits on repeated evaluation.
 
 Sizes: DNA GTR+I+G4 200 taxa x 100,000 sites; protein +G4 100 taxa x 20,000
 sites; 500 partitions of 100-5,000 sites on one 64-taxon tree."""
@@ -93,6 +94,30 @@ def test_protein_100x20k_invariants(ctx):
         assert abs(t32.compute_likelihood() - lnl0) <= 1e-6 * abs(lnl0)
     finally:
         t32.close()
+
+
+def test_codon_serving_requests_deterministic(ctx):
+    """61-state requests of the serving workload (43 taxa x 34,991 patterns
+    and 148 taxa x 27,677 patterns; each traversal step's output tile is
+    written cooperatively by the workgroup, and the next step must read it
+    only after a barrier): repeated evaluations give the same bits, FP64
+    and FP32."""
+    import bench
+    reqs = bench.make_requests(ia, 62)
+    for k in (29, 61):
+        r = reqs[k]
+        assert r.nstates == 61
+        d = r.desc
+        for prec in (ia.PREC_F64, ia.PREC_F32):
+            inst = ia.Instance(ctx, d.ntaxa, d.nstates, d.ncat, d.npatterns, d.nslots, d.state_unknown,
+                               prec, safe_numeric=True)
+            try:
+                inst.set_patterns(r.states, r.freq, r.invar)
+                inst.set_model(*r.model)
+                vals = [inst.likelihood_branch(r.ops, r.node, r.dad, r.length) for _ in range(3)]
+                assert vals[0] == vals[1] == vals[2], (k, prec, vals)
+            finally:
+                inst.close()
 
 
 def partitions_500(seed=20261202 + 3):

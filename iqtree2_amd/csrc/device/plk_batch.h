@@ -191,15 +191,24 @@ __host__ __device__ inline size_t batch_smem_bytes(int n, int C, int SU, int tab
     return (size_t)n * n * elem + (tab > root ? tab : root);
 }
 
-// patterns per work-list tile: one pattern per C threads (64 with four
-// categories; for 20 states four 16-pattern matrix-core tiles)
-__host__ __device__ constexpr int batch_tile_patterns(int, int C) { return kThreads / C; }
+// patterns per thread of a 4-state batch tile: two, so that each step's
+// table staging and barriers serve twice the data and two patterns' loads
+// are in flight together
+constexpr int kDnaPatternsPerThread = 2;
+
+// patterns per work-list tile: 4 states, kDnaPatternsPerThread patterns per
+// C threads (128 with four categories); 20 states, one pattern per C threads
+// (four 16-pattern matrix-core tiles)
+__host__ __device__ constexpr int batch_tile_patterns(int n, int C) {
+    return (n == 4 ? kDnaPatternsPerThread : 1) * kThreads / C;
+}
 
 // all ops of one tile of one item, then its root lnL; returns the tile's
 // lnL sum (thread 0)
 template <typename T, int N, int C>
 __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsigned char* smem) {
     constexpr int PB = batch_tile_patterns(N, C);
+    constexpr int kPP = N == 4 ? kDnaPatternsPerThread : 1;   // patterns per thread
     constexpr int block = N * C;
     T* inv_sh = reinterpret_cast<T*>(smem);
     T* tab_sh = inv_sh + N * N;
@@ -235,8 +244,16 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
         for (int t = threadIdx.x; t < szL; t += kThreads) tab_sh[t] = gL[t];
         for (int t = threadIdx.x; t < szR; t += kThreads) tab_sh[it.table_size + t] = gR[t];
         __syncthreads();
-        op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + it.table_size, L.is_tip, L.id, R.is_tip, R.id,
-                          op.dad_slot, p, c);
+        // the barrier above also orders the previous step's scale counters
+        // (written by the category-0 lane) before these loads
+        OpIn<T, N> in[kPP];
+#pragma unroll
+        for (int j = 0; j < kPP; j++)
+            op_load<T, N, C>(I, L.is_tip, L.id, R.is_tip, R.id, p + j * (kThreads / C), c, in[j]);
+#pragma unroll
+        for (int j = 0; j < kPP; j++)
+            op_finish<T, N, C>(I, inv_sh, tab_sh, tab_sh + it.table_size, L.is_tip, R.is_tip, op.dad_slot,
+                               p + j * (kThreads / C), c, in[j]);
     }
     }
     double* root_sh = reinterpret_cast<double*>(tab_sh);
@@ -247,43 +264,51 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
         if (dad.is_tip)
             for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = it.m.tip[t];
         __syncthreads();
-        if constexpr (N == 20) branch_end_stream<T, N, C>(I, node, dad, MODE_THETA, root_sh, p, c, it.m.tip);
-        else theta_apply<T, N, C>(I, node, dad, root_sh, p, c);
+        if constexpr (N == 20) {
+            branch_end_stream<T, N, C>(I, node, dad, MODE_THETA, root_sh, p, c, it.m.tip);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPP; j++) theta_apply<T, N, C>(I, node, dad, root_sh, p + j * (kThreads / C), c);
+        }
         return 0.0;
     }
     const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
     for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = it.root[t];
     __syncthreads();
-    double lh0;
-    int minsc;
-    if constexpr (N == 20) {
-        // streamed (no per-thread state arrays), same arithmetic as below
-        lh0 = branch_end_stream<T, N, C>(I, node, dad, MODE_BRANCH, root_sh, p, c, it.m.tip, &minsc);
-    } else {
-        double nv[N], dv[N], adjust;
-        load_branch<T, N, C>(I, node, dad, nullptr, p, c, nv, dv, minsc, adjust);
-        double lhc;
-        if (dad.is_tip) {
-            const int s = I.states[(size_t)dad.id * I.Ppad + p];
-            const double* lut = root_sh + block + (s * C + c) * N;
-            lhc = lut[0] * nv[0];
-#pragma unroll
-            for (int i = 1; i < N; i++) lhc = fma(lut[i], nv[i], lhc);
-        } else {
-            const double* val = root_sh + c * N;
-            lhc = val[0] * dv[0] * nv[0];
-#pragma unroll
-            for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
-        }
-        lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
-    }
     double contrib = 0.0;
-    if (c == 0) {
-        const double lh = fabs(lh0) + I.invar[p];
-        const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
-        if (p < I.P) {
-            I.ptn_lh[p] = lnp;
-            contrib = lnp * I.freq[p];
+#pragma unroll
+    for (int j = 0; j < kPP; j++) {
+        const int pj = p + j * (kThreads / C);
+        double lh0;
+        int minsc;
+        if constexpr (N == 20) {
+            // streamed (no per-thread state arrays), same arithmetic as below
+            lh0 = branch_end_stream<T, N, C>(I, node, dad, MODE_BRANCH, root_sh, pj, c, it.m.tip, &minsc);
+        } else {
+            double nv[N], dv[N], adjust;
+            load_branch<T, N, C>(I, node, dad, nullptr, pj, c, nv, dv, minsc, adjust);
+            double lhc;
+            if (dad.is_tip) {
+                const int s = I.states[(size_t)dad.id * I.Ppad + pj];
+                const double* lut = root_sh + block + (s * C + c) * N;
+                lhc = lut[0] * nv[0];
+#pragma unroll
+                for (int i = 1; i < N; i++) lhc = fma(lut[i], nv[i], lhc);
+            } else {
+                const double* val = root_sh + c * N;
+                lhc = val[0] * dv[0] * nv[0];
+#pragma unroll
+                for (int i = 1; i < N; i++) lhc = fma(val[i] * dv[i], nv[i], lhc);
+            }
+            lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
+        }
+        if (c == 0) {
+            const double lh = fabs(lh0) + I.invar[pj];
+            const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
+            if (pj < I.P) {
+                I.ptn_lh[pj] = lnp;
+                contrib += lnp * I.freq[pj];
+            }
         }
     }
     double v[1] = {contrib};

@@ -1142,7 +1142,6 @@ plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& gri
     HIP_TRY(hipMemsetAsync(in->sw_flags, 0, sizeof(unsigned) * nflags, ctx->stream));
     a.part = in->sw_part;
     a.release = in->sw_flags;
-    a.result = in->sw_out + 2;
     void* args[] = {&a};
     HIP_TRY(hipLaunchCooperativeKernel((const void*)k_sweep<T, N, C>, dim3(grid), dim3(kThreads),
                                        args, (unsigned)smem, ctx->stream));
@@ -1625,9 +1624,7 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
     a.max_nr = max_nr;
     a.table_size = in->table_size;
     a.part = nullptr;
-    a.arrive = nullptr;
     a.release = nullptr;
-    a.result = nullptr;
     a.nderv = in->sw_nderv;
     a.out = in->sw_out;
     a.timing = nullptr;

@@ -56,9 +56,7 @@ struct SweepArgs {
     int max_nr;
     int table_size;
     double* part;         // [3][gridDim.x][2] partial-sum slots (kSlotEmpty per launch)
-    unsigned* arrive;     // unused
     unsigned* release;    // [1] abort flag (zeroed per launch)
-    double* result;       // unused
     int* nderv;           // [nbranches]
     double* out;          // [0] lnL from buffer after the sweep, [1] status
     long long* timing;    // nullable: wall-clock ticks of workgroup 0 per phase

@@ -357,6 +357,71 @@ def test_device_newton_sweep_matches_host_newton(ctx, seq_type, precision, safe,
     assert abs(dev.compute_likelihood() - ld) <= 1e-8 * abs(ld)
 
 
+def context_with(env):
+    """A second context created with the environment knobs in env set (they
+    are read once, when a context is created)."""
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return ia.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("precision", [ia.PREC_F64, ia.PREC_F32])
+def test_device_sweep_theta_in_lds_or_hbm(ctx, precision):
+    """The device Newton pass with theta resident in LDS (one thread per
+    pattern in the evaluations) and with theta in HBM (the path of
+    alignments too large for LDS) take the same Newton path."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=48, nsite=20000, ncat=4, mean_len=0.1, alpha=0.5,
+                    p_inv=0.1, gap_frac=0.02, seed=4343)
+    ctx_hbm = context_with({"PLK_SWEEP_LDS": "0"})
+    try:
+        lds = ia.PhyloTree(ctx, w, precision=precision)
+        hbm = ia.PhyloTree(ctx_hbm, w, precision=precision)
+        l1, n1 = lds.optimize_all_branches(iterations=2, tolerance=1e-3, max_nr=100)
+        l2, n2 = hbm.optimize_all_branches(iterations=2, tolerance=1e-3, max_nr=100)
+        assert n1 == n2
+        assert abs(l1 - l2) <= 1e-9 * abs(l2), (l1, l2)
+        for a, b in lds.branches():
+            x1, x2 = lds.get_length(a, b), hbm.get_length(a, b)
+            assert abs(x1 - x2) <= 1e-9 * max(x2, 1e-6), (a, b, x1, x2)
+        lds.close()
+        hbm.close()
+    finally:
+        ctx_hbm.close()
+
+
+@pytest.mark.parametrize("sub", ["2", "4"])
+def test_protein_traversal_workgroup_shapes_bit_identical(ctx, sub):
+    """Protein traversals with 2 or 4 sub-tiles of 16 patterns per workgroup
+    give the same partials and lnL bits as one sub-tile per workgroup."""
+    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=60, nsite=3000, ncat=4, mean_len=0.5, alpha=1.0,
+                    seed=91)
+    planner = ia.PhyloTree(None, w)
+    ops, node, dad, length = planner.plan(*planner.branches()[3])
+    ctx_sub = context_with({"PLK_MFMA_SUB": sub})
+    try:
+        for prec in (ia.PREC_F64, ia.PREC_F32):
+            a = ia.Instance.from_workload(ctx, w, precision=prec)
+            b = ia.Instance.from_workload(ctx_sub, w, precision=prec)
+            assert a.likelihood_branch(ops, node, dad, length) == b.likelihood_branch(ops, node, dad, length)
+            for op in ops[::7]:
+                pa, sa = a.read_partial(int(op["dad_slot"]))
+                pb, sb = b.read_partial(int(op["dad_slot"]))
+                np.testing.assert_array_equal(pa, pb)
+                np.testing.assert_array_equal(sa, sb)
+            a.close()
+            b.close()
+    finally:
+        ctx_sub.close()
+
+
 def partition_set(n_parts, ntaxa=24, seed=7, f32_every=0):
     """Partitions of mixed data type and length on one shared tree."""
     base = ia.Workload(seq_type=ia.DNA, ntaxa=ntaxa, nsite=10, seed=seed)

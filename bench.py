@@ -155,8 +155,8 @@ def bench_partitions(ia, ctx, nparts, reps, cpu_seconds, threads):
             res["plain_fp64_ms"] = 1e3 * dt
             lnl["plain"] = np.array(single)
         plan = ia.BatchPlan(ctx, jobs)              # marshalled once, as a C++ caller holds its items
-        for _ in range(2):                          # the recurring shape is autotuned once
-            plan.run()
+        for _ in range(10):                         # the recurring shape's first calls each time
+            plan.run()                              # one launch-shape candidate (8 at most)
         ctx.synchronize()
         t = time.perf_counter()
         for _ in range(reps):

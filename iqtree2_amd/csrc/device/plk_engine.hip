@@ -186,9 +186,16 @@ struct plk_context {
     std::unordered_map<void*, size_t> live;
     std::multimap<size_t, void*> cache;
     size_t cached = 0;
-    // autotuner: key (class mix, log2 tiles) -> (split launches, blocks per CU)
-    std::map<long, std::pair<int, int>> batch_tune;
-    std::map<long, int> batch_seen;   // occurrences of each key before tuning
+    // autotuner: key (class mix, log2 tiles) -> candidate shapes tried so far
+    // (one per call: the launch a call makes anyway is the trial) and the best
+    struct BatchTune {
+        int next = 0;            // next candidate to time
+        float best_ms = 1e30f;
+        int split = 1, bpc = 2;  // best so far (default until every candidate ran)
+        bool done = false;
+    };
+    std::map<long, BatchTune> batch_tune;
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
     bool autotune = true;
 };
 
@@ -749,6 +756,8 @@ plk_status plk_destroy(plk_context* ctx) {
     for (int i = 0; i < (int)ctx->inst.size(); i++)
         if (ctx->inst[i]) destroy_instance(ctx, i);
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     // every block still live (batch buffers) and every cached one
     for (auto& kv : ctx->live) (void)hipFree(kv.first);
     for (auto& kv : ctx->cache) (void)hipFree(kv.second);
@@ -1353,40 +1362,37 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     while ((1 << lg) < total_tiles) lg++;
     const long key = (long)mix * 64 + lg;
     int split = 1, bpc = ctx->batch_blocks_per_cu > 0 ? ctx->batch_blocks_per_cu : 2;
-    auto tuned = ctx->batch_tune.find(key);
-    if (ctx->batch_blocks_per_cu == 0 && tuned != ctx->batch_tune.end()) {
-        split = tuned->second.first;
-        bpc = tuned->second.second;
-    } else if (ctx->batch_blocks_per_cu == 0 && ctx->autotune && total_tiles >= 4 * ctx->num_cu &&
-               ++ctx->batch_seen[key] >= 2) {
-        // time every candidate once on this batch (results are identical);
-        // only a shape that recurs is tuned (a partitioned tree is evaluated
-        // many times; a one-off batch of requests is not worth the trials),
-        // and batches of fewer tiles than a few per CU take the default
-        hipEvent_t e0, e1;
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
-        float best = 1e30f;
-        const int splits[2] = {0, 1}, bpcs[4] = {1, 2, 4, 8};
-        for (int sp : splits) {
-            if (sp == 1 && ranges.size() == 1) continue;
-            for (int b : bpcs) {
-                HIP_TRY(hipEventRecord(e0, ctx->stream));
-                if ((st = launch(sp, b)) != PLK_OK) return st;
-                HIP_TRY(hipEventRecord(e1, ctx->stream));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; split = sp; bpc = b; }
+    // Online tuning: a recurring shape's first calls each run one candidate
+    // (fused or per-class launches x 1, 2, 4, 8 workgroups per CU) and time
+    // it; results do not depend on the shape, so no call repeats work.
+    // Batches of fewer tiles than a few per CU keep the default.
+    // likely-good shapes first, so exploring costs little
+    static const std::pair<int, int> kOrder[8] = {{1, 2}, {1, 4}, {0, 2}, {0, 4}, {1, 8}, {0, 8}, {1, 1}, {0, 1}};
+    std::pair<int, int> cands[8];
+    int ncand = 0;
+    for (const auto& cd : kOrder)
+        if (cd.first == 0 || ranges.size() > 1) cands[ncand++] = cd;
+    plk_context::BatchTune* tune = nullptr;
+    if (ctx->batch_blocks_per_cu == 0 && ctx->autotune && total_tiles >= 4 * ctx->num_cu) {
+        plk_context::BatchTune& bt = ctx->batch_tune[key];
+        if (bt.done) {
+            split = bt.split;
+            bpc = bt.bpc;
+        } else {
+            split = cands[bt.next].first;
+            bpc = cands[bt.next].second;
+            tune = &bt;
+            if (!ctx->tune_ev[0]) {
+                HIP_TRY(hipEventCreate(&ctx->tune_ev[0]));
+                HIP_TRY(hipEventCreate(&ctx->tune_ev[1]));
             }
+            HIP_TRY(hipEventRecord(ctx->tune_ev[0], ctx->stream));
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        ctx->batch_tune[key] = {split, bpc};
     }
     pe = prof_begin(ctx);
     if ((st = launch(split, bpc)) != PLK_OK) return st;
     prof_end(ctx, pe, K_BATCH, bytes, 0.0);
+    if (tune) HIP_TRY(hipEventRecord(ctx->tune_ev[1], ctx->stream));
     hipLaunchKernelGGL(k_batch_reduce, dim3((nitems + kThreads - 1) / kThreads), dim3(kThreads), 0,
                        ctx->stream, d_items, nitems, ctx->bt_tile_sums, ctx->bt_vals);
     HIP_TRY(hipGetLastError());
@@ -1394,6 +1400,16 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     HIP_TRY(hipMemcpyAsync(out.data(), ctx->bt_vals, sizeof(double) * nitems, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->h_stage_inflight = false;
+    if (tune) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->tune_ev[0], ctx->tune_ev[1]));
+        if (ms < tune->best_ms) {
+            tune->best_ms = ms;
+            tune->split = split;
+            tune->bpc = bpc;
+        }
+        if (++tune->next >= ncand) tune->done = true;
+    }
     bool nonfinite = false;
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];

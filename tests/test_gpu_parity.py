@@ -441,8 +441,8 @@ def partition_set(n_parts, ntaxa=24, seed=7, f32_every=0):
 
 def test_partition_batch_equals_per_partition(ctx):
     """plk_likelihood_branch_batch over mixed DNA/protein, FP64/FP32
-    partitions equals one plk_likelihood_branch per partition, is
-    reproducible, and every autotuned launch shape gives the same result."""
+    partitions equals one plk_likelihood_branch per partition and is
+    reproducible."""
     parts = partition_set(40, f32_every=5)
     jobs, single = [], []
     for k, (w, prec) in enumerate(parts):
@@ -454,9 +454,11 @@ def test_partition_batch_equals_per_partition(ctx):
         jobs.append((inst, ops, node, dad, length))
     for inst, ops, node, dad, length in jobs:
         single.append(inst.likelihood_branch(ops, node, dad, length))
-    first = ia.likelihood_branch_batch(ctx, jobs)      # default launch shape
-    second = ia.likelihood_branch_batch(ctx, jobs)     # recurring shape: autotuning trials
-    third = ia.likelihood_branch_batch(ctx, jobs)      # tuned shape
+    # repeated calls of one shape (each a different launch-shape candidate
+    # once a batch has enough tiles to be tuned) give the same bits
+    first = ia.likelihood_branch_batch(ctx, jobs)
+    second = ia.likelihood_branch_batch(ctx, jobs)
+    third = ia.likelihood_branch_batch(ctx, jobs)
     np.testing.assert_array_equal(first, second)
     np.testing.assert_array_equal(first, third)
     np.testing.assert_allclose(first, np.array(single), rtol=1e-12, atol=0)

@@ -468,6 +468,9 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(ia, w, args.cpu_seconds, threads)
+        # beside it, one core on a shorter sample of the same step
+        one = cpu_baseline(ia, w, max(1.0, args.cpu_seconds / 2), 1)
+        cpu["one_core"] = {"value": one["value"], "unit": one["unit"], "sample": one["sample"]}
         log(f"cpu baseline: {json.dumps(cpu)}")
 
     out = {

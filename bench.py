@@ -390,8 +390,8 @@ def main():
                     help="only the protein/codon traversal leg (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the likelihood path shards by partition/request; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the likelihood path "
+              "would shard by partition or request, one instance set per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import iqtree2_amd as ia

@@ -728,9 +728,18 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
         ctx->own_stream = true;
     }
     size_t free_b = 0, total_b = 0;
-    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    HIP_TRY(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipDeviceGetAttribute(&ctx->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    {
+        // a failure from here on releases what was created so far
+        hipError_t q = hipMemGetInfo(&free_b, &total_b);
+        if (q == hipSuccess) q = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (q == hipSuccess)
+            q = hipDeviceGetAttribute(&ctx->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+        if (q != hipSuccess) {
+            if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+            delete ctx;
+            return fail(PLK_ERR_DEVICE, "device query: %s", hipGetErrorString(q));
+        }
+    }
     if (const char* e = std::getenv("PLK_SWEEP_BLOCKS_PER_CU")) ctx->sweep_blocks_per_cu = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PLK_SWEEP_LDS")) ctx->sweep_lds = std::atoi(e) != 0;
     if (const char* e = std::getenv("PLK_MFMA_SUB")) {

@@ -1,16 +1,27 @@
-"""Benchmark: site-likelihood evaluations per second on one MI355X.
+"""Benchmark: pattern-likelihood evaluations per second on one MI355X.
 
-Workload (DNA GTR+I+G4, 200 taxa x 100,000 sites, ~60k unique patterns, 2% gap /
-ambiguity cells, p_inv 0.2, Gamma alpha 0.5, Exp(0.1) branch lengths, seeded):
-one step = one full post-order traversal with the tree lnL, then one
-optimizeAllBranches sweep of Newton-Raphson over all 397 branches, starting
-from the same initial branch lengths every step.
+Headline workload: 500 partitions on one 64-taxon tree (partition lengths
+lognormal(ln 800, 0.8) clipped to [100, 5000] sites; 70% DNA GTR+G4, 30%
+protein +G4, each with independently seeded parameters; seeded simulation).
+One step evaluates every partition's tree likelihood - a full post-order
+traversal plus the root branch lnL - in one persistent launch, FP64 (the
+drop-in precision), as PhyloSuperTree::computeLikelihood does partition by
+partition.
 
-Unit: a site-likelihood evaluation is one unique pattern's likelihood under
-one whole tree; every tree-level evaluation (the traversal lnL, each Newton
-derivative evaluation, each lnL-from-buffer) counts the P patterns once.
-value = P * evaluations / second, FP64 form (the drop-in). The FP32-storage
-form's rate and its |delta lnL| against FP64 are reported beside it.
+Unit: a pattern-likelihood evaluation (PLE) is one unique site pattern's
+likelihood under one whole tree (every internal node's partial plus the root
+lnL); value = sum of the partitions' pattern counts x steps / seconds.
+Inputs are resident in HBM when timing starts; nothing is uploaded inside
+the timed region.
+
+Beside it on the same line: the plain FP64 form (one evaluation call per
+partition), the FP32-storage form with its per-tree |dlnL| against FP64 and
+whether the 1e-6 gate holds, the ratios the targets are quoted on (for the
+form that meets the gate), the roofline of the dominant kernel from HIP
+events on the engine's stream, the oracle on the host cores, and legs for
+the other configurations: DNA GTR+I+G4 200 taxa x 100,000 sites (traversal
+PLE/s, Newton pattern-derivative evaluations/s), protein 100 x 20,000 and a
+codon case on the matrix cores, and the request-serving stream (p50/p99).
 """
 import argparse
 import json
@@ -22,176 +33,170 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-FP64_PEAK = 78.6e12        # dense FP64, flop/s
+FP64_PEAK = 78.6e12        # dense FP64 MFMA, flop/s
+FP32_PEAK = 157.3e12       # dense FP32 MFMA, flop/s
+GATE = 1e-6                # |dlnL| per tree against the FP64 result
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(ia, args):
-    return ia.Workload(seq_type=ia.DNA, ntaxa=args.taxa, nsite=args.sites, ncat=4, mean_len=0.1,
-                       alpha=0.5, p_inv=0.2, gap_frac=0.02, seed=20261202 + 1)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def run_steps(ctx, tree, steps, profile=False):
-    evals = 0
-    lnls = []
-    if profile:
-        ctx.profile_start()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        tree.reset()
-        lnl0 = tree.compute_likelihood()
-        lnl, nd = tree.optimize_all_branches(iterations=1, tolerance=1e-3, max_nr=100)
-        evals += 1 + 1 + nd + 1      # traversal lnL, sweep's initial lnL, derivatives, from-buffer
-        lnls.append((lnl0, lnl))
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = ctx.profile_stop() if profile else None
-    return elapsed, evals, lnls, prof
+def host_threads():
+    # the GPU box gives one GPU a 16-CPU share, whatever os.cpu_count() reports
+    return max(1, min(16, os.cpu_count() or 1))
 
 
-def time_traversal(ctx, tree, reps):
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        tree.reset()
-        lnl = tree.compute_likelihood()
-    ctx.synchronize()
-    return (time.perf_counter() - t0) / reps, lnl
+def kernel_table(prof):
+    return {k: dict(launches=v["launches"], ms=round(v["total_ms"], 4),
+                    avg_us=round(1e3 * v["total_ms"] / v["launches"], 2),
+                    gbps=round(v["bytes"] / max(v["total_ms"], 1e-9) * 1e-6, 1),
+                    tflops=round(v["flops"] / max(v["total_ms"], 1e-9) * 1e-9, 2))
+            for k, v in prof.items()}
 
 
-def cpu_baseline(ia, w, budget_s, threads):
-    """Oracle (FP64 CPU restatement, AVX2/FMA, OpenMP) on the same step - the
-    full traversal, then Newton on every branch in the same pre-order - from
-    the same initial lengths, repeated until the time budget is spent (at
-    least once; a step that outlasts the budget stops after the branch that
-    crosses it)."""
-    from oracle.binding import OracleTree
-    planner = ia.PhyloTree(None, w)
-    orc = OracleTree.from_workload(w, nthreads=threads)
-    branches = planner.branches()
-    evals = 0
-    steps = 0
-    nbr = 0
-    t0 = time.perf_counter()
-    while True:
-        planner.reset()
-        a0, b0 = branches[0]
-        ops, node, dad, length = planner.plan(b0, a0)
-        orc.likelihood_branch(ops, node, dad, length)
-        evals += 1
-        nbr = 0
-        for a, b in branches:
-            ops, node, dad, length = planner.plan(b, a)
-            x, _, nd, st = orc.optimize_branch(ops, node, dad, length)
-            evals += nd + 1
-            planner.set_length(a, b, x)
-            nbr += 1
-            if steps == 0 and time.perf_counter() - t0 > budget_s:
-                break
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or nbr < len(branches):
-            break
-    elapsed = time.perf_counter() - t0
-    if nbr == len(branches):
-        what = f"{steps} full steps (traversal + Newton on all {len(branches)} branches)"
-    else:
-        what = f"full traversal + Newton on the first {nbr} of {len(branches)} branches"
-    return dict(value=w.npatterns * evals / elapsed, unit="site-lik evals/s", cores=threads,
-                kind="port",
-                sample=f"{what}: {evals} tree evaluations x {w.npatterns} patterns, {elapsed:.1f} s")
+def roofline_of(prof, names, traffic_key):
+    """Roofline entry of the kernel with the most device time among `names`:
+    algorithmic bytes per launch over the average launch duration (HIP
+    events recorded around each launch on the engine's stream), against
+    the HBM peak; HBM traffic per launch from the committed rocprofv3 PMC
+    summary of the same kernel on the same workload."""
+    cand = {k: v for k, v in prof.items() if k in names}
+    name, k = max(cand.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_s = k["total_ms"] * 1e-3 / k["launches"]
+    bytes_per = k["bytes"] / k["launches"]
+    flops_per = k["flops"] / k["launches"]
+    achieved = bytes_per / avg_s
+    traffic, src = None, None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            entry = json.load(f).get(traffic_key.format(kernel=name))
+        if entry:
+            traffic, src = entry["traffic_bytes"], entry["source"]
+    return {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "traffic_source": src, "avg_launch_us": avg_s * 1e6, "bytes_per_launch": bytes_per,
+            "flops_per_launch": flops_per, "tflops": flops_per / avg_s / 1e12}
 
 
-def make_partitions(ia, nparts, seed=20261202 + 3):
-    """Config 4: one 64-taxon tree (Exp mean 0.1), partition lengths
-    lognormal(ln 800, 0.8) clipped to [100, 5000], 70% DNA GTR+G4 / 30%
-    protein +G4, independently seeded parameters (alpha ~ U(0.2, 2))."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    base = ia.Workload(seq_type=ia.DNA, ntaxa=64, nsite=10, mean_len=0.1, seed=seed)
-    edges = base.edges()
-    parts = []
-    for k in range(nparts):
-        nsite = int(np.clip(round(rng.lognormal(np.log(800.0), 0.8)), 100, 5000))
-        st = ia.PROTEIN if rng.random() < 0.3 else ia.DNA
-        parts.append(ia.Workload(seq_type=st, ntaxa=64, nsite=nsite, ncat=4, alpha=-1.0,
-                                 seed=seed * 1000 + k, edges=edges))
-    return parts
-
-
-def bench_partitions(ia, ctx, nparts, reps, cpu_seconds, threads):
-    import numpy as np
-    t0 = time.perf_counter()
-    parts = make_partitions(ia, nparts)
-    plans = []
+# ---------------------------------------------------------------------------
+# headline: 500 partitions
+def partition_jobs(ia, ctx, parts, prec):
+    insts, jobs = [], []
     for w in parts:
         planner = ia.PhyloTree(None, w)
         a, b = planner.branches()[0]
-        plans.append(planner.plan(a, b))
-    total_p = sum(w.npatterns for w in parts)
-    total_sites = sum(w.nsite for w in parts)
-    log(f"partitions: {nparts}, {total_sites} sites, {total_p} patterns "
-        f"({time.perf_counter() - t0:.1f} s to generate)")
-    res = {"partitions": nparts, "sites": total_sites, "patterns": total_p}
+        plan = planner.plan(a, b)
+        planner.close()
+        inst = ia.Instance.from_workload(ctx, w, precision=prec)
+        insts.append(inst)
+        jobs.append((inst, *plan))
+    return insts, jobs
+
+
+def time_batch(ctx, plan, steps):
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        out = plan.run()
+    ctx.synchronize()
+    return time.perf_counter() - t, out
+
+
+def leg_partitions(ia, ctx, parts, args):
+    import numpy as np
+    P = sum(w.npatterns for w in parts)
+    res = {"partitions": len(parts), "sites": sum(w.nsite for w in parts), "patterns": P,
+           "protein_partitions": sum(w.nstates == 20 for w in parts)}
     lnl = {}
     for prec, name in ((ia.PREC_F64, "fp64"), (ia.PREC_F32, "fp32")):
-        insts = [ia.Instance.from_workload(ctx, w, precision=prec) for w in parts]
-        jobs = [(inst, *plan) for inst, plan in zip(insts, plans)]
+        insts, jobs = partition_jobs(ia, ctx, parts, prec)
+        plan = ia.BatchPlan(ctx, jobs)       # marshalled once, as a C++ caller holds its items
+        # the autotuner times one launch shape per call for a recurring batch
+        # shape (eight candidates at most): untimed, before the warmup steps
+        for _ in range(10):
+            plan.run()
+        for _ in range(args.warmup):
+            plan.run()
+        dt, out = time_batch(ctx, plan, args.steps)
+        res[name] = {"ple_per_s": P * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps}
+        lnl[name] = out
+        # the same steps again under per-launch HIP events (roofline)
+        ctx.profile_start()
+        time_batch(ctx, plan, args.steps)
+        prof = ctx.profile_stop()
+        res[name]["kernels"] = kernel_table(prof)
+        res[name]["roofline"] = roofline_of(prof, ("batch", "batch_dna", "batch_protein"),
+                                            "{kernel}_partitions_500_" + name)
         if prec == ia.PREC_F64:
-            # plain form: one computeLikelihoodBranch launch per partition
+            # plain form: one computeLikelihoodBranch call per partition
             for inst, ops, node, dad, length in jobs:
                 inst.likelihood_branch(ops, node, dad, length)
             ctx.synchronize()
+            reps = max(1, min(args.steps, 5))
             t = time.perf_counter()
             for _ in range(reps):
                 single = [inst.likelihood_branch(ops, node, dad, length)
                           for inst, ops, node, dad, length in jobs]
             ctx.synchronize()
-            dt = (time.perf_counter() - t) / reps
-            res["plain_fp64_per_partition_evals_per_s"] = total_p / dt
-            res["plain_fp64_ms"] = 1e3 * dt
+            dt1 = (time.perf_counter() - t) / reps
+            res["plain_fp64"] = {"ple_per_s": P / dt1, "ms_per_step": 1e3 * dt1}
             lnl["plain"] = np.array(single)
-        plan = ia.BatchPlan(ctx, jobs)              # marshalled once, as a C++ caller holds its items
-        for _ in range(10):                         # the recurring shape's first calls each time
-            plan.run()                              # one launch-shape candidate (8 at most)
-        ctx.synchronize()
-        t = time.perf_counter()
-        for _ in range(reps):
-            out = plan.run()
-        ctx.synchronize()
-        dt = (time.perf_counter() - t) / reps
-        res[f"batched_{name}_evals_per_s"] = total_p / dt
-        res[f"batched_{name}_ms"] = 1e3 * dt
-        lnl[name] = out
         for inst in insts:
             inst.close()
-    res["batched_fp64_vs_plain"] = res["batched_fp64_evals_per_s"] / res["plain_fp64_per_partition_evals_per_s"]
-    res["batched_fp32_vs_plain_fp64"] = res["batched_fp32_evals_per_s"] / res["plain_fp64_per_partition_evals_per_s"]
-    res["max_abs_dlnl_fp32_per_tree"] = float(np.max(np.abs(lnl["fp32"] - lnl["fp64"])))
-    res["max_rel_dlnl_fp32_per_tree"] = float(np.max(np.abs(lnl["fp32"] - lnl["fp64"]) / np.abs(lnl["fp64"])))
-    res["max_abs_dlnl_batched_vs_plain_fp64"] = float(np.max(np.abs(lnl["fp64"] - lnl["plain"])))
-    if cpu_seconds > 0:
-        res["cpu_baseline"] = cpu_partitions(ia, parts, plans, cpu_seconds, threads)
-        res["batched_fp32_vs_cpu"] = res["batched_fp32_evals_per_s"] / res["cpu_baseline"]["value"]
-    return res
+    d32 = np.abs(lnl["fp32"] - lnl["fp64"])
+    res["fp32"]["max_abs_dlnl_per_tree"] = float(d32.max())
+    res["fp32"]["median_abs_dlnl_per_tree"] = float(np.median(d32))
+    res["fp32"]["max_rel_dlnl_per_tree"] = float(np.max(d32 / np.abs(lnl["fp64"])))
+    res["fp32"]["trees_within_gate"] = int(np.sum(d32 <= GATE))
+    res["fp32"]["meets_gate"] = bool(np.all(d32 <= GATE))
+    res["fp64"]["max_abs_dlnl_vs_plain"] = float(np.max(np.abs(lnl["fp64"] - lnl["plain"])))
+    return res, lnl
 
 
-def cpu_partitions(ia, parts, plans, budget_s, threads):
-    """Oracle over partitions the way PhyloSuperTree runs them: partitions in
-    parallel, one thread each, largest first; the whole set is evaluated
-    repeatedly until the time budget is spent (oracle trees built beforehand,
-    untimed, as the partitions' PhyloTrees exist before an evaluation)."""
+def gate_vs_oracle(part, lnl, ref):
+    """|dlnL| per tree of the batched FP64 and FP32 forms against the oracle."""
+    import numpy as np
+    for name in ("fp64", "fp32"):
+        d = np.abs(lnl[name] - ref)
+        part[name]["max_abs_dlnl_vs_oracle"] = float(d.max())
+        part[name]["trees_within_gate_vs_oracle"] = int(np.sum(d <= GATE))
+        part[name]["meets_gate_vs_oracle"] = bool(np.all(d <= GATE))
+
+
+def cpu_partitions(ia, parts, budget_s, threads):
+    """The oracle (FP64 restatement of the reference kernels, AVX2/FMA) over
+    the partitions the way PhyloSuperTree runs them: partitions in parallel,
+    one thread each, largest first; the whole set repeatedly until the time
+    budget is spent (oracle trees built beforehand, untimed)."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle.binding import OracleTree
     order = sorted(range(len(parts)), key=lambda k: -parts[k].npatterns * parts[k].nstates ** 2)
-    orcs = {k: OracleTree.from_workload(parts[k], nthreads=1) for k in order}
+    plans, orcs = {}, {}
+    for k in order:
+        planner = ia.PhyloTree(None, parts[k])
+        plans[k] = planner.plan(*planner.branches()[0])
+        planner.close()
+        orcs[k] = OracleTree.from_workload(parts[k], nthreads=1)
+
+    import numpy as np
+    ref = np.zeros(len(parts))
 
     def run(k):
         ops, node, dad, length = plans[k]
-        orcs[k].likelihood_branch(ops, node, dad, length)
+        ref[k] = orcs[k].likelihood_branch(ops, node, dad, length)
         return parts[k].npatterns
 
     total, rounds = 0, 0
@@ -204,52 +209,155 @@ def cpu_partitions(ia, parts, plans, budget_s, threads):
                 break
     elapsed = time.perf_counter() - t0
     return dict(value=total / elapsed, unit="site-lik evals/s", cores=threads, kind="port",
+                cpu_model=cpu_model(),
                 sample=f"{rounds} evaluations of all {len(parts)} partitions (full traversal + "
-                       f"lnL each), {elapsed:.1f} s")
+                       f"root lnL each, FP64), {elapsed:.1f} s on {threads} threads"), ref
 
 
-def bench_mfma(ia, ctx, reps=3):
-    """Config 3 (protein LG-like+G4, 100 taxa x 20,000 sites, Exp(0.2)
-    lengths, alpha 0.8) and a codon GY94+G4 case (64 taxa x 5,000 codons):
-    full post-order traversals with the tree lnL on the matrix-core kernels,
-    FP64 and FP32 storage, with |dlnL| of FP32 against FP64."""
-    cases = [("protein_100x20k", ia.Workload(seq_type=ia.PROTEIN, ntaxa=100, nsite=20000, ncat=4,
-                                               mean_len=0.2, alpha=0.8, seed=20261202 + 2)),
-             ("codon_64x5k", ia.Workload(seq_type=ia.CODON, ntaxa=64, nsite=5000, ncat=4,
-                                           mean_len=0.1, alpha=0.5, seed=77))]
+# ---------------------------------------------------------------------------
+# DNA 200 x 100,000: traversal and the Newton pass
+def leg_dna(ia, ctx, w, args):
+    P = w.npatterns
+    tree = ia.PhyloTree(ctx, w, precision=ia.PREC_F64)
+    out = {"taxa": w.ntaxa, "sites": w.nsite, "patterns": P}
+    try:
+        tree.compute_likelihood()
+        ctx.synchronize()
+        reps = 5
+        t = time.perf_counter()
+        for _ in range(reps):
+            tree.clear_all_partial()
+            lnl = tree.compute_likelihood()
+        ctx.synchronize()
+        trav = (time.perf_counter() - t) / reps
+        out["traversal_ms"] = 1e3 * trav
+        out["traversal_ple_per_s"] = P / trav
+        # one optimizeAllBranches pass (device-resident Newton sweep) from
+        # the same initial lengths each time
+        tree.reset()
+        tree.optimize_all_branches(iterations=1, tolerance=1e-3, max_nr=100)
+        ctx.profile_start()
+        t = time.perf_counter()
+        nd_total = 0
+        for _ in range(reps):
+            tree.reset()
+            tree.compute_likelihood()
+            lnl1, nd = tree.optimize_all_branches(iterations=1, tolerance=1e-3, max_nr=100)
+            nd_total += nd
+        ctx.synchronize()
+        step = (time.perf_counter() - t) / reps
+        prof = ctx.profile_stop()
+        sweep = prof["newton_sweep"]
+        sweep_s = sweep["total_ms"] * 1e-3 / sweep["launches"]
+        out["newton"] = {"branches": 2 * w.ntaxa - 3, "derivative_evals": nd_total / reps,
+                         "sweep_ms": 1e3 * sweep_s,
+                         "pattern_derivative_evals_per_s": P * (nd_total / reps) / sweep_s,
+                         "step_ms_traversal_plus_pass": 1e3 * step, "lnl_before": lnl,
+                         "lnl_after": lnl1}
+        out["kernels"] = kernel_table(prof)
+        out["roofline_sweep"] = roofline_of(prof, ("newton_sweep",), "{kernel}_dna_200x100k")
+        out["roofline_sweep"]["bound"] = "latency (grid exchanges)"
+    finally:
+        tree.close()
+    if not args.no_fp32:
+        t32 = ia.PhyloTree(ctx, w, precision=ia.PREC_F32)
+        try:
+            t32.compute_likelihood()
+            ctx.synchronize()
+            t = time.perf_counter()
+            for _ in range(5):
+                t32.clear_all_partial()
+                l32 = t32.compute_likelihood()
+            ctx.synchronize()
+            d = abs(l32 - lnl)
+            out["fp32"] = {"traversal_ms": 1e3 * (time.perf_counter() - t) / 5,
+                           "abs_dlnl": d, "rel_dlnl": d / abs(lnl), "meets_gate": d <= GATE}
+        finally:
+            t32.close()
+    return out
+
+
+def cpu_dna(ia, w, budget_s, threads):
+    """The oracle on the same DNA step: full traversal, then Newton on every
+    branch in the same pre-order, from the same initial lengths, until the
+    time budget is spent."""
+    from oracle.binding import OracleTree
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w, nthreads=threads)
+    branches = planner.branches()
+    trav = nd_total = 0
+    t_trav = t_newton = 0.0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        planner.reset()
+        a0, b0 = branches[0]
+        ops, node, dad, length = planner.plan(b0, a0)
+        t = time.perf_counter()
+        orc.likelihood_branch(ops, node, dad, length)
+        t_trav += time.perf_counter() - t
+        trav += 1
+        t = time.perf_counter()
+        for a, b in branches:
+            ops, node, dad, length = planner.plan(b, a)
+            x, _, nd, st = orc.optimize_branch(ops, node, dad, length)
+            nd_total += nd
+            planner.set_length(a, b, x)
+            if time.perf_counter() - t0 > 2 * budget_s:
+                break
+        t_newton += time.perf_counter() - t
+    planner.close()
+    P = w.npatterns
+    return {"traversal_ple_per_s": P * trav / t_trav,
+            "pattern_derivative_evals_per_s": P * nd_total / t_newton, "cores": threads,
+            "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"{trav} traversal + Newton passes, {time.perf_counter() - t0:.1f} s"}
+
+
+# ---------------------------------------------------------------------------
+# matrix-core traversals: protein 100 x 20,000 and codon 64 x 5,000
+def leg_mfma(ia, ctx, reps=3):
+    from iqtree2_amd import workloads as wl
     out = {}
-    for name, w in cases:
+    for name, w in (("protein_100x20k", wl.protein_100x20k()), ("codon_64x5k", wl.codon_64x5k())):
         res = {"patterns": w.npatterns}
         lnl = {}
         for prec, pn in ((ia.PREC_F64, "fp64"), (ia.PREC_F32, "fp32")):
             tree = ia.PhyloTree(ctx, w, precision=prec)
-            tree.compute_likelihood()
-            ctx.profile_start()
-            t = time.perf_counter()
-            for _ in range(reps):
-                tree.clear_all_partial()
-                lnl[pn] = tree.compute_likelihood()
-            ctx.synchronize()
-            dt = (time.perf_counter() - t) / reps
-            k = ctx.profile_stop()["traverse_mfma"]
-            res[pn] = {"traversal_ms": 1e3 * dt,
-                       "site_lik_evals_per_s": w.npatterns / dt,
+            try:
+                tree.compute_likelihood()
+                ctx.profile_start()
+                t = time.perf_counter()
+                for _ in range(reps):
+                    tree.clear_all_partial()
+                    lnl[pn] = tree.compute_likelihood()
+                ctx.synchronize()
+                dt = (time.perf_counter() - t) / reps
+                prof = ctx.profile_stop()
+            finally:
+                tree.close()
+            k = prof["traverse_mfma"]
+            peak = FP64_PEAK if prec == ia.PREC_F64 else FP32_PEAK
+            res[pn] = {"traversal_ms": 1e3 * dt, "ple_per_s": w.npatterns / dt,
                        "kernel_ms": k["total_ms"] / k["launches"],
                        "tflops": k["flops"] / k["total_ms"] * 1e-9,
-                       "gbps": k["bytes"] / k["total_ms"] * 1e-6}
-            tree.close()
-        res["rel_dlnl_fp32"] = abs(lnl["fp32"] - lnl["fp64"]) / abs(lnl["fp64"])
+                       "mfma_frac": k["flops"] / (k["total_ms"] * 1e-3) / peak,
+                       "gbps": k["bytes"] / k["total_ms"] * 1e-6,
+                       "hbm_frac": k["bytes"] / (k["total_ms"] * 1e-3) / HBM_PEAK}
+        d = abs(lnl["fp32"] - lnl["fp64"])
+        res["fp32_abs_dlnl"] = d
+        res["fp32_rel_dlnl"] = d / abs(lnl["fp64"])
         out[name] = res
     return out
 
 
+# ---------------------------------------------------------------------------
+# serving stream
 def make_requests(ia, nreq, seed=20261202 + 4):
-    """Config 5: per request N ~ log-uniform[8, 256] taxa and L ~
-    log-uniform[500, 50,000] sites on a seeded Exp(0.1) tree; DNA GTR+G4
-    (60%), protein +G4 (30%) or codon GY94 +G4 (10%, omega ~ Gamma(2, 0.25),
-    kappa ~ U(1, 4), equal codon frequencies); op full lnL, single-branch
-    Newton or one multinomial(L, freq/L) bootstrap reweighting, in equal
-    shares."""
+    """Per request N ~ log-uniform[8, 256] taxa and L ~ log-uniform[500,
+    50,000] sites on a seeded Exp(0.1) tree; DNA GTR+G4 (60%), protein +G4
+    (30%) or codon GY94 +G4 (10%, omega ~ Gamma(2, 0.25), kappa ~ U(1, 4),
+    equal codon frequencies); op full lnL, single-branch Newton or one
+    multinomial(L, freq/L) bootstrap reweighting, in equal shares."""
     import numpy as np
     from iqtree2_amd.serving import Request
     rng = np.random.default_rng(seed)
@@ -268,10 +376,11 @@ def make_requests(ia, nreq, seed=20261202 + 4):
             weights = rng.multinomial(int(f.sum()), f / f.sum())[None, :]
         branch = int(rng.integers(2 * ntaxa - 3))
         reqs.append(Request(w, kind, branch=branch, weights=weights))
+        w.close()
     return reqs
 
 
-def bench_serving(ia, ctx, nreq, seed=20261202 + 4):
+def leg_serving(ia, ctx, nreq, seed=20261202 + 4):
     """Capacity with every request queued at once, then a Poisson stream at
     70% of it; latency = completion - submission on the server clock."""
     import threading
@@ -280,8 +389,7 @@ def bench_serving(ia, ctx, nreq, seed=20261202 + 4):
     t0 = time.perf_counter()
     reqs = make_requests(ia, nreq, seed)
     kinds = {0: "lnl", 1: "newton", 2: "rell"}
-    log(f"serving: {nreq} requests, {sum(r.npatterns for r in reqs)} patterns, largest "
-        f"footprint {max(r.footprint() for r in reqs) / 2**30:.2f} GiB "
+    log(f"serving: {nreq} requests, {sum(r.npatterns for r in reqs)} patterns "
         f"({time.perf_counter() - t0:.1f} s to generate)")
 
     def evals(r, res):
@@ -290,21 +398,17 @@ def bench_serving(ia, ctx, nreq, seed=20261202 + 4):
             return r.npatterns * (res["n_derv"] + 2)
         return r.npatterns
 
-    # warm-up: a few requests through a server (kernel loads, autotuning)
-    with Server(ctx) as srv:
+    with Server(ctx) as srv:         # warm-up: kernel loads, autotuning
         for r in reqs[:8]:
             srv.wait(srv.submit(r))
-    # capacity: all queued at once
     with Server(ctx) as srv:
         c0 = srv.clock()
-        tickets = [srv.submit(r) for r in reqs]
-        closed = [srv.wait(t) for t in tickets]
+        closed = [srv.wait(t) for t in [srv.submit(r) for r in reqs]]
         cap_s = srv.clock() - c0
         cap_stats = srv.stats()
     capacity = nreq / cap_s
     lam = 0.7 * capacity
-    gaps = np.random.default_rng(seed + 1).exponential(1.0 / lam, nreq)
-    arrivals = np.cumsum(gaps)
+    arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / lam, nreq))
     results = [None] * nreq
     with Server(ctx) as srv:
         tk = [None] * nreq
@@ -328,44 +432,35 @@ def bench_serving(ia, ctx, nreq, seed=20261202 + 4):
         span = srv.clock() - s0
         stats = srv.stats()
     lat = np.array([r["finish_s"] - r["submit_s"] for r in results]) * 1e3
-    ok = all(r["status"] == 0 for r in results)
-    same = all(a["lnl"] == b["lnl"] and a["length"] == b["length"] for a, b in zip(results, closed))
-    total_evals = sum(evals(r, res) for r, res in zip(reqs, results))
-    mix = {}
-    for r in reqs:
-        key = f"{ {4: 'dna', 20: 'protein', 61: 'codon'}[r.nstates]}_{kinds[r.kind]}"
-        mix[key] = mix.get(key, 0) + 1
+    model = {4: "dna", 20: "protein", 61: "codon"}
     by_model = {}
     for r, l in zip(reqs, lat):
-        m = {4: "dna", 20: "protein", 61: "codon"}[r.nstates]
-        by_model.setdefault(m, []).append(l)
+        by_model.setdefault(model[r.nstates], []).append(l)
+    mix = {}
+    for r in reqs:
+        key = f"{model[r.nstates]}_{kinds[r.kind]}"
+        mix[key] = mix.get(key, 0) + 1
     return {
-        "requests": nreq,
-        "arrival_rate_per_s": lam,
-        "capacity_req_per_s": capacity,
-        "capacity_site_lik_evals_per_s": sum(evals(r, res) for r, res in zip(reqs, closed)) / cap_s,
-        "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)),
-        "mean_ms": float(lat.mean()),
-        "max_ms": float(lat.max()),
+        "requests": nreq, "arrival_rate_per_s": lam, "capacity_req_per_s": capacity,
+        "capacity_ple_per_s": sum(evals(r, x) for r, x in zip(reqs, closed)) / cap_s,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
         "p50_ms_by_model": {m: float(np.percentile(v, 50)) for m, v in by_model.items()},
         "p99_ms_by_model": {m: float(np.percentile(v, 99)) for m, v in by_model.items()},
         "throughput_req_per_s": nreq / span,
-        "site_lik_evals_per_s": total_evals / span,
-        "all_ok": ok,
-        "results_identical_to_closed_loop_run": same,
+        "ple_per_s": sum(evals(r, x) for r, x in zip(reqs, results)) / span,
+        "all_ok": all(r["status"] == 0 for r in results),
+        "results_identical_to_closed_loop_run": all(
+            a["lnl"] == b["lnl"] and a["length"] == b["length"] for a, b in zip(results, closed)),
         "mix": mix,
-        "max_taxa": max(r.ntaxa for r in reqs),
-        "max_patterns": max(r.npatterns for r in reqs),
         "scheduler": {"iterations": stats["iterations"],
                       "traversal_batches": stats["traversal_batches"],
                       "derv_batches": stats["derv_batches"],
                       "peak_inflight": stats["peak_inflight"],
                       "peak_gib": stats["peak_bytes"] / 2**30,
-                      "closed_loop_peak_inflight": cap_stats["peak_inflight"],
+                      "closed_loop_span_s": cap_s,
                       "closed_loop_s": {k: round(cap_stats[k], 4) for k in
                                         ("admit_s", "traverse_s", "derv_s", "retire_s")},
-                      "closed_loop_span_s": cap_s,
                       "stream_s": {k: round(stats[k], 4) for k in
                                    ("admit_s", "traverse_s", "derv_s", "retire_s")}},
     }
@@ -374,138 +469,111 @@ def bench_serving(ia, ctx, nreq, seed=20261202 + 4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--taxa", type=int, default=200)
-    ap.add_argument("--sites", type=int, default=100000)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--partitions", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--partitions", type=int, default=500)
-    ap.add_argument("--no-partitions", action="store_true")
     ap.add_argument("--requests", type=int, default=160)
+    ap.add_argument("--no-dna", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
-    ap.add_argument("--mfma-only", action="store_true",
-                    help="only the protein/codon traversal leg (for profiling its kernels)")
+    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving"),
+                    help="run one leg alone (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: there is no multi-GPU path (the likelihood path "
-              "would shard by partition or request, one instance set per GPU)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here (the likelihood path "
+              "shards by partition or request, one instance set per GPU; see the design notes)",
+              file=sys.stderr)
         sys.exit(2)
 
     import iqtree2_amd as ia
-    if args.mfma_only:
-        ctx = ia.Context(0)
-        print(json.dumps({"mfma_traversals": bench_mfma(ia, ctx)}), flush=True)
-        ctx.close()
-        return
-    t0 = time.perf_counter()
-    w = make_workload(ia, args)
-    log(f"workload: {w.ntaxa} taxa, {w.nsite} sites, {w.npatterns} patterns "
-        f"({time.perf_counter() - t0:.1f} s to generate)")
+    from iqtree2_amd import workloads as wl
     ctx = ia.Context(0)
-    tree = ia.PhyloTree(ctx, w, precision=ia.PREC_F64)
-    for i in range(args.warmup):
-        run_steps(ctx, tree, 1)
-        log(f"warmup step {i + 1} done")
-    elapsed, evals, lnls, prof = run_steps(ctx, tree, args.steps, profile=True)
-    P = w.npatterns
-    value = P * evals / elapsed
-    log(f"fp64: {args.steps} steps in {elapsed:.3f} s, {evals} tree evaluations, "
-        f"lnL {lnls[-1][0]:.6f} -> {lnls[-1][1]:.6f}")
-    trav_s, trav_lnl = time_traversal(ctx, tree, 5)
+    try:
+        if args.only:
+            if args.only == "partitions":
+                parts = wl.partitions_500(args.partitions)
+                out = leg_partitions(ia, ctx, parts, args)[0]
+            elif args.only == "dna":
+                out = leg_dna(ia, ctx, wl.dna_200x100k(), args)
+            elif args.only == "mfma":
+                out = leg_mfma(ia, ctx)
+            else:
+                out = leg_serving(ia, ctx, args.requests)
+            print(json.dumps({args.only: out}), flush=True)
+            return
+        t0 = time.perf_counter()
+        parts = wl.partitions_500(args.partitions)
+        log(f"partitions: {len(parts)}, {sum(w.npatterns for w in parts)} patterns "
+            f"({time.perf_counter() - t0:.1f} s to generate)")
+        part, lnl = leg_partitions(ia, ctx, parts, args)
+        cpu = None
+        if not args.no_cpu:
+            cpu, ref = cpu_partitions(ia, parts, args.cpu_seconds, host_threads())
+            gate_vs_oracle(part, lnl, ref)
+            log(f"cpu baseline: {json.dumps(cpu)}")
+        log(f"partitions: {json.dumps(part)}")
+        dna = mfma = serving = None
+        if not args.no_dna:
+            w = wl.dna_200x100k()
+            dna = leg_dna(ia, ctx, w, args)
+            if not args.no_cpu:
+                dna["cpu_baseline"] = cpu_dna(ia, w, args.cpu_seconds / 2, host_threads())
+            w.close()
+            log(f"dna: {json.dumps(dna)}")
+        if not args.no_mfma:
+            mfma = leg_mfma(ia, ctx)
+            log(f"mfma: {json.dumps(mfma)}")
+        if not args.no_serving:
+            serving = leg_serving(ia, ctx, args.requests)
+            log(f"serving: {json.dumps(serving)}")
+    finally:
+        ctx.close()
 
-    # dominant kernel and its roofline; HBM traffic per launch from the
-    # committed rocprofv3 PMC summary of the same kernel on this workload
-    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
-    achieved = dom["bytes"] / (dom["total_ms"] * 1e-3)
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tpath) and (args.taxa, args.sites) == (200, 100000):
-        with open(tpath) as f:
-            entry = json.load(f).get(dom_name)
-        if entry:
-            traffic, traffic_src = entry["traffic_bytes"], entry["source"]
-    kernels = {k: dict(launches=v["launches"], ms=round(v["total_ms"], 3),
-                       avg_us=round(1e3 * v["total_ms"] / v["launches"], 2),
-                       gbps=round(v["bytes"] / max(v["total_ms"], 1e-9) * 1e-6, 1))
-               for k, v in prof.items()}
-    log(f"kernels: {json.dumps(kernels)}")
-
-    fp32 = None
-    if not args.no_fp32:
-        tree32 = ia.PhyloTree(ctx, w, precision=ia.PREC_F32)
-        run_steps(ctx, tree32, 1)
-        e32, ev32, l32, _ = run_steps(ctx, tree32, args.steps)
-        t32_s, t32_lnl = time_traversal(ctx, tree32, 5)
-        fp32 = dict(value=P * ev32 / e32, ms_per_step=1e3 * e32 / args.steps,
-                    traversal_ms=1e3 * t32_s,
-                    abs_dlnl_traversal=abs(t32_lnl - trav_lnl),
-                    abs_dlnl_after_newton=abs(l32[-1][1] - lnls[-1][1]),
-                    rel_dlnl_traversal=abs(t32_lnl - trav_lnl) / abs(trav_lnl))
-        log(f"fp32: {json.dumps(fp32)}")
-        tree32.close()
-
-    parts = None
-    if not args.no_partitions:
-        parts = bench_partitions(ia, ctx, args.partitions, max(1, args.steps),
-                                 0.0 if args.no_cpu else args.cpu_seconds,
-                                 min(16, os.cpu_count() or 1))
-        log(f"partitions: {json.dumps(parts)}")
-
-    mfma = None
-    if not args.no_mfma:
-        mfma = bench_mfma(ia, ctx)
-        log(f"mfma traversals: {json.dumps(mfma)}")
-
-    serving = None
-    if not args.no_serving:
-        serving = bench_serving(ia, ctx, args.requests)
-        log(f"serving: {json.dumps(serving)}")
-
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(ia, w, args.cpu_seconds, threads)
-        # beside it, one core on a shorter sample of the same step
-        one = cpu_baseline(ia, w, max(1.0, args.cpu_seconds / 2), 1)
-        cpu["one_core"] = {"value": one["value"], "unit": one["unit"], "sample": one["sample"]}
-        log(f"cpu baseline: {json.dumps(cpu)}")
-
+    f64, f32 = part["fp64"], part["fp32"]
+    plain = part["plain_fp64"]["ple_per_s"]
+    # the >=5x / >=20x targets are quoted for a form whose per-tree |dlnL|
+    # against the FP64 oracle is within the 1e-6 gate
+    targets = {
+        "gate_abs_dlnl_per_tree": GATE,
+        "batched_fp64": {"vs_plain_fp64": f64["ple_per_s"] / plain,
+                         "vs_cpu": f64["ple_per_s"] / cpu["value"] if cpu else None,
+                         "max_abs_dlnl_vs_oracle": f64.get("max_abs_dlnl_vs_oracle"),
+                         "meets_gate": f64.get("meets_gate_vs_oracle")},
+        "batched_fp32_storage": {"vs_plain_fp64": f32["ple_per_s"] / plain,
+                                 "vs_cpu": f32["ple_per_s"] / cpu["value"] if cpu else None,
+                                 "max_abs_dlnl_vs_oracle": f32.get("max_abs_dlnl_vs_oracle"),
+                                 "meets_gate": f32.get("meets_gate_vs_oracle")},
+    }
     out = {
         "metric": "site-lik evals/s FP64 vs batched FP32 at |dlnL|<=1e-6; p50/p99 req latency",
-        "value": value,
+        "value": f64["ple_per_s"],
         "unit": "site-lik evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": f64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded simulation down a random tree)",
-        "config": {"workload": "DNA GTR+I+G4, 200 taxa x 100,000 sites: full post-order "
-                               "traversal + Newton over all 397 branches",
-                   "taxa": w.ntaxa, "sites": w.nsite, "patterns": P,
-                   "tree_evals_per_step": evals / args.steps,
-                   "traversal_ms": 1e3 * trav_s,
-                   "traversal_site_lik_evals_per_s": P / trav_s},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": traffic, "traffic_source": traffic_src,
-                     "avg_launch_us": 1e3 * dom["total_ms"] / dom["launches"],
-                     "bytes_per_launch": dom["bytes"] / dom["launches"]},
-        "kernels": kernels,
-        "fp32_storage": fp32,
-        "partition_batch": parts,
+        "data": "synthetic (seeded simulation down a seeded random tree)",
+        "config": {"workload": "500 partitions on one 64-taxon tree (lognormal 100-5,000 sites, "
+                               "70% DNA GTR+G4 / 30% protein +G4): every partition's full "
+                               "traversal + root lnL in one persistent launch, FP64",
+                   "partitions": part["partitions"], "patterns": part["patterns"],
+                   "sites": part["sites"], "unit_note": "one PLE = one unique pattern under one "
+                   "whole tree (post-order over 62 internal nodes + root lnL)"},
+        "roofline": f64["roofline"],
+        "cpu_baseline": cpu,
+        "targets": targets,
+        "partition_batch": part,
+        "dna_200x100k": dna,
         "mfma_traversals": mfma,
         "serving": serving,
-        "cpu_baseline": cpu,
     }
-    tree.close()
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -210,13 +210,17 @@ typedef struct {
 } plk_sweep_op;
 
 /* one branch of the sweep: ops[op_begin .. op_begin+op_count) refresh the
- * partials at its ends, then theta(node, dad) and Newton on edge `edge` */
+ * partials at its ends, then theta(node, dad) and Newton on edge `edge`;
+ * lnl_index >= 0 asks for computeLikelihoodFromBuffer at the optimised
+ * length right after this branch, stored in lnl_out[lnl_index]
+ * (plk_optimize_sweep_ex), -1 for none */
 typedef struct {
     int op_begin;
     int op_count;
     plk_end node;
     plk_end dad;
     int edge;
+    int lnl_index;
 } plk_sweep_branch;
 
 /* One pass of PhyloTree::optimizeAllBranches over `branches` in order
@@ -227,7 +231,32 @@ typedef struct {
  * receives derivative evaluations per branch; lnl receives
  * computeLikelihoodFromBuffer after the last branch. The op list must be the
  * one a host planner produces when every optimised branch invalidates the
- * partials that contain it (clearReversePartialLh). */
+ * partials that contain it (clearReversePartialLh). A pass of more than
+ * PLK_SWEEP_MAX_BRANCHES branches, or one whose branch-length array does not
+ * fit the kernel's LDS layout, returns PLK_ERR_UNSUPPORTED before any partial
+ * is touched (the caller then runs minimizeNewton per branch with
+ * plk_optimize_branch). A non-finite derivative stops the pass with
+ * PLK_ERR_NONFINITE when safe_numeric is 0; with safe_numeric 1 it counts as
+ * df = ddf = 0 and the pass continues, as the reference's safe kernel does.
+ * After any status other than PLK_OK the partials of the instance are
+ * undefined. */
+#define PLK_SWEEP_MAX_BRANCHES 4096
+
+/* The same sweep with more outputs, for callers that optimise a planned
+ * sequence of branches rather than one pass over a tree - the NNI
+ * evaluation of getBestNNIForBran (tree/phylotree.cpp:4069-4300): per move,
+ * the five branches around the swapped edge with optimizeOneBranch(clearLH =
+ * false, NNI_MAX_NR_STEP), then computeLikelihoodFromBuffer. Ops that write
+ * the NNI buffers and edge ids that give each evaluation its own copy of
+ * the five lengths are the caller's plan. branch_len (nullable,
+ * [nbranches]) receives each branch's optimised length; lnl_out [nlnl]
+ * receives the lnL of every branch with lnl_index >= 0. A non-finite lnL
+ * there ends the sweep with PLK_ERR_NONFINITE. */
+plk_status plk_optimize_sweep_ex(plk_context* ctx, int instance, const plk_sweep_op* ops, int nops,
+                                 const plk_sweep_branch* branches, int nbranches, double* lengths,
+                                 int nedges, double min_len, double max_len, int max_nr,
+                                 int* n_derv, double* branch_len, double* lnl_out, int nlnl,
+                                 double* lnl);
 plk_status plk_optimize_sweep(plk_context* ctx, int instance, const plk_sweep_op* ops, int nops,
                               const plk_sweep_branch* branches, int nbranches, double* lengths,
                               int nedges, double min_len, double max_len, int max_nr,

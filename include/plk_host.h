@@ -117,6 +117,45 @@ int plkh_tree_optimize_branch(plkh_tree* t, int a, int b, int max_nr, double* ne
  * lnl = final tree lnL, n_derv = derivative evaluations */
 int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double tolerance, int max_nr,
                                     double* lnl, int* n_derv);
+/* derivative evaluations of each branch in the last optimize_all_branches
+ * pass, in plkh_tree_branches order; *count = number of branches (values
+ * beyond cap are not written) */
+int plkh_tree_pass_counts(const plkh_tree* t, int32_t* n_derv, int cap, int* count);
+/* neighbours of a node in adjacency order (deg <= 3) */
+int plkh_tree_neighbors(const plkh_tree* t, int node, int32_t* nb, int* deg);
+/* the length of (a, b) alone, no partial invalidated
+ * (optimizeOneBranch(clearLH = false)) */
+int plkh_tree_set_length_keep(plkh_tree* t, int a, int b, double len);
+
+/* ---- NNI evaluation (getBestNNIForBran, tree/phylotree.cpp:4069-4300) ---- */
+/* node1 and node2 (adjacent internal nodes) move to the instance's two NNI
+ * buffers (slots nnodes - ntaxa and nnodes - ntaxa + 1, in the order the
+ * reference hands out nni_partial_lh), every orientation of them stale */
+int plkh_tree_nni_begin(plkh_tree* t, int node1, int node2);
+/* exchange neighbour x of node1 with neighbour y of node2 (edges keep their
+ * lengths; Node::updateNeighbor) */
+int plkh_tree_nni_swap(plkh_tree* t, int node1, int x, int node2, int y);
+/* PhyloNeighbor::clearPartialLh: the partial of node seen from `from` */
+int plkh_tree_clear_partial(plkh_tree* t, int node, int from);
+/* restores topology, lengths and the two nodes' buffers */
+int plkh_tree_nni_end(plkh_tree* t);
+/* inner branches (both ends internal) by edge id; *count = their number */
+int plkh_tree_inner_branches(plkh_tree* t, int32_t* a, int32_t* b, int cap, int* count);
+/* getBestNNIForBran(node1, node2) with nni5 and nni5_num_eval = 1: for both
+ * moves (node1's first neighbour exchanged with node2's first, then second
+ * neighbour) the tree lnL from the buffer lnl[2] and the five optimised
+ * lengths lens[5 * move + k] (k = 0 the central branch, 1-2 node1's other
+ * branches, 3-4 node2's, in adjacency order) with n_derv (nullable) the
+ * derivative evaluations of each; topology and lengths are restored.
+ * max_nr = NNI_MAX_NR_STEP (10 in the reference). One engine call per
+ * derivative evaluation. */
+int plkh_tree_nni_evaluate(plkh_tree* t, int node1, int node2, int max_nr, double* lnl,
+                           double* lens, int32_t* n_derv);
+/* plkh_tree_nni_evaluate for every inner branch, in plkh_tree_inner_branches
+ * order, as one device-resident sweep (plk_optimize_sweep_ex): lnl [2m],
+ * lens [10m], n_derv [10m] (nullable) for m inner branches; 4 and 20 states */
+int plkh_tree_nni_evaluate_all(plkh_tree* t, int max_nr, double* lnl, double* lens,
+                               int32_t* n_derv, int cap);
 /* 1 (default): optimize_all_branches runs each pass as one device-resident
  * Newton sweep (plk_optimize_sweep); 0: minimizeNewton on the host, one
  * engine call per derivative evaluation */

@@ -3,6 +3,7 @@
 The library is the product: when it is missing or fails to load this module
 raises ImportError; there is no fallback implementation.
 """
+import atexit
 import ctypes
 import os
 
@@ -176,9 +177,40 @@ _PROTOS = {
     "plkh_tree_optimize_all_branches": (c_int, [c_void_p, c_int, c_double, c_int, _DP, P(c_int)]),
     "plkh_tree_set_branch_bounds": (c_int, [c_void_p, c_double, c_double]),
     "plkh_tree_set_device_newton": (c_int, [c_void_p, c_int]),
+    "plkh_tree_pass_counts": (c_int, [c_void_p, _I32, c_int, P(c_int)]),
+    "plkh_tree_neighbors": (c_int, [c_void_p, c_int, _I32, P(c_int)]),
+    "plkh_tree_set_length_keep": (c_int, [c_void_p, c_int, c_int, c_double]),
+    "plkh_tree_nni_begin": (c_int, [c_void_p, c_int, c_int]),
+    "plkh_tree_nni_swap": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "plkh_tree_clear_partial": (c_int, [c_void_p, c_int, c_int]),
+    "plkh_tree_nni_end": (c_int, [c_void_p]),
+    "plkh_tree_inner_branches": (c_int, [c_void_p, _I32, _I32, c_int, P(c_int)]),
+    "plkh_tree_nni_evaluate": (c_int, [c_void_p, c_int, c_int, c_int, _DP, _DP, _I32]),
+    "plkh_tree_nni_evaluate_all": (c_int, [c_void_p, c_int, _DP, _DP, _I32, c_int]),
+    "plk_optimize_sweep_ex": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
+                                      c_double, c_double, c_int, _I32, _DP, _DP, c_int, _DP]),
 }
 
 _lib = None
+
+# Set when the interpreter starts shutting down: finalizers (__del__) then
+# make no further library calls. Device objects must be closed explicitly
+# before that (Context.close() closes what was created on it); a call into
+# the HIP runtime from a finalizer at exit can run after the runtime, or a
+# profiler attached to it, has been torn down.
+_shutting_down = False
+
+
+def _mark_shutdown():
+    global _shutting_down
+    _shutting_down = True
+
+
+atexit.register(_mark_shutdown)
+
+
+def shutting_down():
+    return _shutting_down
 
 
 def lib():

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <unordered_map>
 #include <mutex>
@@ -46,6 +47,7 @@ plk_status fail(plk_status st, const char* fmt, ...) {
     } while (0)
 
 constexpr int kPadPatterns = 256;  // Ppad is a multiple of every tile size
+constexpr int kSweepMaxBranches = PLK_SWEEP_MAX_BRANCHES;
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
@@ -74,17 +76,19 @@ struct Instance {
     int ops_cap = 0;
     // device-resident Newton sweep
     plk_sweep_op* sw_ops = nullptr;
-    int sw_ops_cap = 0;
+    size_t sw_ops_cap = 0;
     plk_sweep_branch* sw_br = nullptr;
-    int sw_br_cap = 0;
+    size_t sw_br_cap = 0;
     double* sw_len = nullptr;
-    int sw_len_cap = 0;
+    size_t sw_len_cap = 0;
     int* sw_nderv = nullptr;
     double* sw_part = nullptr;
-    int sw_part_cap = 0;
+    size_t sw_part_cap = 0;
     unsigned* sw_flags = nullptr;     // arrival flags, release flag, abort flag
-    int sw_flags_cap = 0;
+    size_t sw_flags_cap = 0;
     double* sw_out = nullptr;         // [0..1] lnL, status; [2..3] reduced sums
+    double* sw_res = nullptr;         // per-branch lengths, then requested lnL values
+    size_t sw_res_cap = 0;
     long long* sw_timing = nullptr;   // phase ticks of the last sweeps (PLK_SWEEP_TIMING)
     void* tables = nullptr;
     bool patterns_set = false, model_set = false, theta_ready = false, ptn_lh_ready = false;
@@ -126,11 +130,11 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
                 K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH,
-                K_TRAVERSE_MFMA, K_ROOT, K_NKIND };
+                K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
                                           "reduce", "rell", "newton_sweep", "batch", "prep_batch",
-                                          "traverse_mfma", "root"};
+                                          "traverse_mfma", "root", "batch_dna", "batch_protein"};
 
 struct ProfRec {
     int kind;
@@ -160,26 +164,26 @@ struct plk_context {
     bool h_stage_inflight = false;      // an upload from h_stage may still run
     // batched evaluation buffers (grown on demand)
     unsigned char* bt_upload = nullptr;   // items | ops | tile list | root lengths
-    int bt_upload_cap = 0;
+    size_t bt_upload_cap = 0;
     unsigned char* bt_tables = nullptr;
-    int bt_tables_cap = 0;
+    size_t bt_tables_cap = 0;
     double* bt_roots = nullptr;
-    int bt_roots_cap = 0;
+    size_t bt_roots_cap = 0;
     double* bt_tile_sums = nullptr;
-    int bt_tile_sums_cap = 0;
+    size_t bt_tile_sums_cap = 0;
     double* bt_vals = nullptr;      // lnl out
-    int bt_vals_cap = 0;
+    size_t bt_vals_cap = 0;
     unsigned* bt_counter = nullptr;
-    int bt_counter_cap = 0;
+    size_t bt_counter_cap = 0;
     // batched derivative buffers
     DervItem* db_items = nullptr;
-    int db_items_cap = 0;
+    size_t db_items_cap = 0;
     int* db_tile_item = nullptr;
-    int db_tile_cap = 0;
+    size_t db_tile_cap = 0;
     double* db_tile_sums = nullptr;
-    int db_tile_sums_cap = 0;
+    size_t db_tile_sums_cap = 0;
     double* db_out = nullptr;
-    int db_out_cap = 0;
+    size_t db_out_cap = 0;
     int batch_blocks_per_cu = 0;    // 0: autotuned
     // caching device allocator: live blocks by address, free blocks by size
     // class; `used` counts live bytes, `cached` the free blocks kept
@@ -284,7 +288,7 @@ plk_status destroy_instance(plk_context* ctx, int id) {
     void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
                     in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
                     in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
-                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing};
+                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res};
     for (void* b : bufs) dfree(ctx, nullptr, b, 0);
     delete in;
     ctx->inst[id] = nullptr;
@@ -1068,23 +1072,30 @@ plk_status plk_optimize_branch(plk_context* ctx, int id, const plk_op* ops, int 
     if (!(min_len > 0.0) || !(max_len > min_len)) return fail(PLK_ERR_ARG, "invalid bracket");
     plk_status st = plk_likelihood_buffer(ctx, id, ops, nops, node, dad);
     if (st != PLK_OK) return st;
+    const bool safe = get_instance(ctx, id)->d.safe_numeric != 0;
     // Optimization::minimizeNewton (utils/optimization.cpp:422-499) on
     // f = -dlnL/dt (PhyloTree::computeFuncDerv, tree/phylotree.cpp:2529-2536)
     plk::NewtonRun nr;
     nr.start(length, min_len, max_len, min_len, max_nr);
     while (!nr.done()) {
         double d1, d2;
-        if ((st = plk_likelihood_derv(ctx, id, nr.x(), &d1, &d2, nullptr)) != PLK_OK) return st;
+        st = plk_likelihood_derv(ctx, id, nr.x(), &d1, &d2, nullptr);
+        // a non-finite derivative is an error of the normal kernel; the safe
+        // kernel warns and continues with df = ddf = 0
+        // (tree/phylokernelnew.h:2577-2579, 2629-2632)
+        if (st == PLK_ERR_NONFINITE && safe) st = PLK_OK;
+        if (st != PLK_OK) return st;
         nr.feed(-d1, -d2);
     }
     if (nr.failed()) return fail(PLK_ERR_NEWTON, "maximum number of iterations exceeded in minimizeNewton");
     const int evals = nr.evals();
-    double result = nr.result();
-    double optx = result;
-    // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625)
+    double optx = nr.result();
+    // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625):
+    // lnL at the length the last derivative evaluation left in the tree
+    // against lnL at the starting length
     if (optx > max_len * 0.95) {
         double opt_lh, orig_lh;
-        if ((st = plk_likelihood_from_buffer(ctx, id, optx, &opt_lh, nullptr)) != PLK_OK) return st;
+        if ((st = plk_likelihood_from_buffer(ctx, id, nr.last_x(), &opt_lh, nullptr)) != PLK_OK) return st;
         if ((st = plk_likelihood_from_buffer(ctx, id, length, &orig_lh, nullptr)) != PLK_OK) return st;
         if (orig_lh > opt_lh) optx = length;
     }
@@ -1100,14 +1111,18 @@ plk_status plk_optimize_branch(plk_context* ctx, int id, const plk_op* ops, int 
 
 namespace {
 
+// Grows a device buffer to hold at least `need` elements (capacities are
+// element counts; byte sizes never pass through int).
 template <typename T>
-plk_status grow(plk_context* ctx, Instance* in, T** p, int& cap, int need) {
+plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need) {
     if (need <= cap && *p) return PLK_OK;
+    if (need > (std::numeric_limits<size_t>::max() / 2) / sizeof(T))
+        return fail(PLK_ERR_NOMEM, "buffer of %zu elements exceeds the address space", need);
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    dfree(ctx, in, *p, sizeof(T) * (size_t)cap);
+    dfree(ctx, in, *p, sizeof(T) * cap);
     *p = nullptr;
     cap = std::max(need, 2 * cap);
-    plk_status st = dmalloc(ctx, in, p, sizeof(T) * (size_t)cap);
+    plk_status st = dmalloc(ctx, in, p, sizeof(T) * cap);
     if (st != PLK_OK) cap = 0;
     return st;
 }
@@ -1151,12 +1166,12 @@ plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& gri
             nbp == per_cu)
             smem = pad;
     }
-    plk_status st = grow(ctx, in, &in->sw_part, in->sw_part_cap, 6 * grid);   // three slot buffers
+    plk_status st = grow(ctx, in, &in->sw_part, in->sw_part_cap, 6 * (size_t)grid);   // three slot buffers
     if (st != PLK_OK) return st;
     HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(in->sw_part), kSlotEmptyWord,
                               12 * (size_t)grid, ctx->stream));
     const int nflags = 2 * kFlagStride;
-    if ((st = grow(ctx, in, &in->sw_flags, in->sw_flags_cap, nflags)) != PLK_OK) return st;
+    if ((st = grow(ctx, in, &in->sw_flags, in->sw_flags_cap, (size_t)nflags)) != PLK_OK) return st;
     HIP_TRY(hipMemsetAsync(in->sw_flags, 0, sizeof(unsigned) * nflags, ctx->stream));
     a.part = in->sw_part;
     a.release = in->sw_flags;
@@ -1243,7 +1258,9 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     std::vector<size_t> op_off(nitems);
     size_t tab_bytes = 0, root_doubles = 0, smem = 0, smem_prep = 0, nops_total = 0;
     int total_tiles = 0, max_nops = 0;
-    double bytes = 0.0;
+    // algorithmic bytes and flops of the batch, per instance class (index:
+    // the BatchClass bit)
+    double cls_bytes[4] = {0, 0, 0, 0}, cls_flops[4] = {0, 0, 0, 0};
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];
         const Instance* in = ins[i];
@@ -1271,25 +1288,30 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         root_doubles += (size_t)(in->d.state_unknown + 2) * in->block;
         smem = std::max(smem, batch_smem_bytes(b.n, b.C, in->d.state_unknown, in->table_size, in->elem));
         smem_prep = std::max(smem_prep, (size_t)(b.n * b.n + in->d.state_unknown * b.n + 2 * in->block) * 8);
-        double ob = 0.0;
+        double ob = 0.0, of = 0.0;
         for (int o = 0; o < items[i].nops; o++) {
             double bb, ff;
             op_cost(in, items[i].ops[o], bb, ff);
             ob += bb;
+            of += ff;
         }
         const int nint = 1 + (items[i].dad.is_tip ? 0 : 1);
         ob += (double)in->block * in->elem * nint + (items[i].dad.is_tip ? 1 : 0) + in->scale_elem * nint + 24.0;
-        bytes += ob * in->d.npatterns;
+        of += 3.0 * in->block;
+        int cb = 0;
+        while (!((batch_class(b.n, b.f32) >> cb) & 1)) cb++;
+        cls_bytes[cb] += ob * in->d.npatterns;
+        cls_flops[cb] += of * in->d.npatterns;
     }
     const size_t off_ops = round_up(sizeof(BatchItem) * nitems, 256);
     const size_t off_tiles = off_ops + round_up(sizeof(plk_op) * std::max<size_t>(1, nops_total), 256);
     const size_t off_lens = off_tiles + round_up(sizeof(int) * total_tiles, 256);
     const size_t up_bytes = off_lens + sizeof(double) * nitems;
-    if ((st = grow(ctx, nullptr, &ctx->bt_upload, ctx->bt_upload_cap, (int)up_bytes)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_tables, ctx->bt_tables_cap, (int)std::max<size_t>(256, tab_bytes))) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_roots, ctx->bt_roots_cap, (int)root_doubles)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_tile_sums, ctx->bt_tile_sums_cap, total_tiles)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->bt_vals, ctx->bt_vals_cap, nitems)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_upload, ctx->bt_upload_cap, up_bytes)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_tables, ctx->bt_tables_cap, std::max<size_t>(256, tab_bytes))) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_roots, ctx->bt_roots_cap, root_doubles)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_tile_sums, ctx->bt_tile_sums_cap, (size_t)total_tiles)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->bt_vals, ctx->bt_vals_cap, (size_t)nitems)) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->bt_counter, ctx->bt_counter_cap, 64)) != PLK_OK) return st;
     unsigned char* hs = nullptr;
     if ((st = host_stage(ctx, up_bytes, &hs)) != PLK_OK) return st;
@@ -1332,10 +1354,24 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         else ranges.back().second = end;
     }
     bool counters_clear = true;   // k_prep_batch cleared them
+    // profiling record of one launch over the classes in mask
+    auto prof_mask = [&](int pe2, int mask) {
+        if (pe2 < 0) return;
+        double b2 = 0.0, f2 = 0.0;
+        for (int cb = 0; cb < 4; cb++)
+            if ((mask >> cb) & 1) {
+                b2 += cls_bytes[cb];
+                f2 += cls_flops[cb];
+            }
+        const int kind = mask == BC_DNA64 || mask == BC_DNA32 ? K_BATCH_DNA
+                         : mask == BC_PROT64 || mask == BC_PROT32 ? K_BATCH_PROT : K_BATCH;
+        prof_end(ctx, pe2, kind, b2, f2);
+    };
     auto launch = [&](int split, int bpc) -> plk_status {
         if (!counters_clear) HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned) * 16, ctx->stream));
         counters_clear = false;
         auto one = [&](int mask, int begin, int end, int k) -> plk_status {
+            const int pe2 = prof_begin(ctx);
             const void* fn;
             switch (mask) {
             case BC_DNA64: fn = (const void*)k_batch<BC_DNA64>; break;
@@ -1353,6 +1389,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
             double* a5 = ctx->bt_tile_sums;
             void* args[] = {(void*)&a0, (void*)&a1, (void*)&begin, (void*)&end, (void*)&a4, (void*)&a5};
             HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, smem, ctx->stream));
+            prof_mask(pe2, mask == BC_ALL ? mix : mask);
             return PLK_OK;
         };
         if (!split || ranges.size() == 1) {
@@ -1398,9 +1435,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
             HIP_TRY(hipEventRecord(ctx->tune_ev[0], ctx->stream));
         }
     }
-    pe = prof_begin(ctx);
     if ((st = launch(split, bpc)) != PLK_OK) return st;
-    prof_end(ctx, pe, K_BATCH, bytes, 0.0);
     if (tune) HIP_TRY(hipEventRecord(ctx->tune_ev[1], ctx->stream));
     hipLaunchKernelGGL(k_batch_reduce, dim3((nitems + kThreads - 1) / kThreads), dim3(kThreads), 0,
                        ctx->stream, d_items, nitems, ctx->bt_tile_sums, ctx->bt_vals);
@@ -1549,10 +1584,10 @@ plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, con
         bytes += ((double)in->block * in->elem + 24.0) * in->d.npatterns;
     }
     const int total = (int)tile_item.size();
-    if ((st = grow(ctx, nullptr, &ctx->db_items, ctx->db_items_cap, n)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->db_tile_item, ctx->db_tile_cap, total)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->db_tile_sums, ctx->db_tile_sums_cap, 3 * total)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->db_out, ctx->db_out_cap, 3 * n)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_items, ctx->db_items_cap, (size_t)n)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_tile_item, ctx->db_tile_cap, (size_t)total)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_tile_sums, ctx->db_tile_sums_cap, 3 * (size_t)total)) != PLK_OK) return st;
+    if ((st = grow(ctx, nullptr, &ctx->db_out, ctx->db_out_cap, 3 * (size_t)n)) != PLK_OK) return st;
     HIP_TRY(hipMemcpyAsync(ctx->db_items, hd.data(), sizeof(DervItem) * n, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->db_tile_item, tile_item.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
     const int pe = prof_begin(ctx);
@@ -1594,6 +1629,15 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
                               const plk_sweep_branch* branches, int nbranches, double* lengths,
                               int nedges, double min_len, double max_len, int max_nr,
                               int* n_derv, double* lnl) {
+    return plk_optimize_sweep_ex(ctx, id, ops, nops, branches, nbranches, lengths, nedges, min_len,
+                                 max_len, max_nr, n_derv, nullptr, nullptr, 0, lnl);
+}
+
+plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* ops, int nops,
+                                 const plk_sweep_branch* branches, int nbranches, double* lengths,
+                                 int nedges, double min_len, double max_len, int max_nr,
+                                 int* n_derv, double* branch_len, double* lnl_out, int nlnl,
+                                 double* lnl) {
     Instance* in = get_instance(ctx, id);
     plk_status st = ready(in);
     if (st != PLK_OK) return st;
@@ -1621,17 +1665,23 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
             return fail(PLK_ERR_ARG, "sweep branch %d: bad op range / edge", b);
         if ((st = check_end(in, B.node, false)) != PLK_OK) return st;
         if ((st = check_end(in, B.dad, true)) != PLK_OK) return st;
+        if (B.lnl_index < -1 || B.lnl_index >= nlnl)
+            return fail(PLK_ERR_ARG, "sweep branch %d: lnl_index %d outside [-1, %d)", b, B.lnl_index, nlnl);
     }
+    if (nlnl < 0 || (nlnl > 0 && !lnl_out)) return fail(PLK_ERR_ARG, "invalid lnl_out");
+    if (nbranches > kSweepMaxBranches)
+        return fail(PLK_ERR_UNSUPPORTED, "more than %d branches in one sweep", kSweepMaxBranches);
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if ((st = grow(ctx, in, &in->sw_ops, in->sw_ops_cap, std::max(nops, 1))) != PLK_OK) return st;
-    if ((st = grow(ctx, in, &in->sw_br, in->sw_br_cap, nbranches)) != PLK_OK) return st;
-    if ((st = grow(ctx, in, &in->sw_len, in->sw_len_cap, nedges)) != PLK_OK) return st;
-    int nd_cap = in->sw_br_cap;
+    if ((st = grow(ctx, in, &in->sw_ops, in->sw_ops_cap, (size_t)std::max(nops, 1))) != PLK_OK) return st;
+    if ((st = grow(ctx, in, &in->sw_br, in->sw_br_cap, (size_t)nbranches)) != PLK_OK) return st;
+    if ((st = grow(ctx, in, &in->sw_len, in->sw_len_cap, (size_t)nedges)) != PLK_OK) return st;
     if (!in->sw_nderv) {
-        if ((st = dmalloc(ctx, in, &in->sw_nderv, sizeof(int) * (size_t)std::max(nd_cap, 4096))) != PLK_OK) return st;
+        if ((st = dmalloc(ctx, in, &in->sw_nderv, sizeof(int) * (size_t)kSweepMaxBranches)) != PLK_OK) return st;
         if ((st = dmalloc(ctx, in, &in->sw_out, sizeof(double) * 8)) != PLK_OK) return st;
     }
-    if (nbranches > 4096) return fail(PLK_ERR_UNSUPPORTED, "more than 4096 branches in one sweep");
+    // per-branch lengths, then the requested lnL values
+    if ((st = grow(ctx, in, &in->sw_res, in->sw_res_cap, (size_t)nbranches + (size_t)std::max(nlnl, 1))) != PLK_OK)
+        return st;
     if (nops > 0)
         HIP_TRY(hipMemcpyAsync(in->sw_ops, ops, sizeof(plk_sweep_op) * nops, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(in->sw_br, branches, sizeof(plk_sweep_branch) * nbranches, hipMemcpyHostToDevice, ctx->stream));
@@ -1651,6 +1701,9 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
     a.part = nullptr;
     a.release = nullptr;
     a.nderv = in->sw_nderv;
+    a.br_len = in->sw_res;
+    a.lnl_out = in->sw_res + nbranches;
+    a.nlnl = nlnl;
     a.out = in->sw_out;
     a.timing = nullptr;
     a.lds_tiles = 0;
@@ -1672,7 +1725,13 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
     HIP_TRY(hipMemcpyAsync(lengths, in->sw_len, sizeof(double) * nedges, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(nd.data(), in->sw_nderv, sizeof(int) * nbranches, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(out, in->sw_out, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> res((size_t)nbranches + (size_t)nlnl);
+    if (branch_len || nlnl > 0)
+        HIP_TRY(hipMemcpyAsync(res.data(), in->sw_res, sizeof(double) * res.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (branch_len) std::memcpy(branch_len, res.data(), sizeof(double) * nbranches);
+    if (nlnl > 0) std::memcpy(lnl_out, res.data() + nbranches, sizeof(double) * nlnl);
     // theta and pattern lnL hold the last branch only after a complete sweep
     in->theta_ready = out[1] == 0.0;
     in->ptn_lh_ready = out[1] == 0.0;
@@ -1688,7 +1747,14 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
                      t[PH_THETA] * us, t[PH_EVAL_WORK] * us, t[PH_EXCHANGE] * us, t[PH_TOTAL] * us);
     }
     if (n_derv) std::memcpy(n_derv, nd.data(), sizeof(int) * nbranches);
-    if (lnl) *lnl = out[0];
+    double final_lnl = out[0];
+    bool final_nonfinite = false;
+    if (out[1] == 0.0 && !std::isfinite(final_lnl)) {
+        // computeLikelihoodFromBuffer's per-pattern fix-up (tree/phylokernelnew.h:3371-3380)
+        if ((st = fixup_nonfinite(ctx, in, &final_lnl, nullptr)) != PLK_OK) return st;
+        final_nonfinite = !in->d.safe_numeric;
+    }
+    if (lnl) *lnl = final_lnl;
     if (pe >= 0) {
         // algorithmic bytes: the ops, theta per branch, and every evaluation
         // pass over theta (+invar, freq; lnL passes also buffer_scale)
@@ -1716,6 +1782,8 @@ plk_status plk_optimize_sweep(plk_context* ctx, int id, const plk_sweep_op* ops,
         ctx->prof_recs.back().flops = flops * P;
     }
     const int status = (int)out[1];
+    if (final_nonfinite)
+        return fail(PLK_ERR_NONFINITE, "numerical underflow (lh-from-buffer) after the Newton sweep; use the safe kernel");
     if (status == PLK_ERR_NONFINITE) return fail(PLK_ERR_NONFINITE, "non-finite derivative in the Newton sweep");
     if (status == PLK_ERR_DEVICE) return fail(PLK_ERR_DEVICE, "grid reduction in the Newton sweep timed out");
     if (status != 0) return fail((plk_status)status, "Newton sweep failed (status %d)", status);

@@ -58,6 +58,9 @@ struct SweepArgs {
     double* part;         // [3][gridDim.x][2] partial-sum slots (kSlotEmpty per launch)
     unsigned* release;    // [1] abort flag (zeroed per launch)
     int* nderv;           // [nbranches]
+    double* br_len;       // [nbranches] optimised length of each branch
+    double* lnl_out;      // [nlnl] lnL after branches with lnl_index >= 0
+    int nlnl;
     double* out;          // [0] lnL from buffer after the sweep, [1] status
     long long* timing;    // nullable: wall-clock ticks of workgroup 0 per phase
     int lds_tiles;        // > 0: theta, buffer_scale, freq and invar of the workgroup's
@@ -623,8 +626,12 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
             break;
         }
         nd++;
+        double xlast = rts;   // length the last derivative evaluation used
         f = -ctl[0];
         df = -ctl[1];
+        // a non-finite derivative: the safe kernel continues with df = ddf = 0
+        // (tree/phylokernelnew.h:2629-2632), the normal kernel stops
+        if (!isfinite(f) && I.safe) f = df = -0.0;
         if (!isfinite(f) || !isfinite(df)) {
             status = PLK_ERR_NONFINITE;
             break;
@@ -656,8 +663,10 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
                     break;
                 }
                 nd++;
+                xlast = rts;
                 f = -ctl[0];
                 df = -ctl[1];
+                if (!isfinite(f) && I.safe) f = df = -0.0;
                 if (!isfinite(f) || !isfinite(df)) { status = PLK_ERR_NONFINITE; break; }
                 if (df > 0.0 && fabs(f) < xacc) { result = rts; done = true; break; }
                 if (f < 0.0) xl = rts;
@@ -668,9 +677,10 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
             if (!done) { status = PLK_ERR_NEWTON; break; }
         }
         double optx = result;
-        // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625)
+        // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625):
+        // lnL at the last evaluated length against lnL at the old one
         if (optx > x2 * 0.95) {
-            if (!sweep_eval<T, N, C>(a, R, 1, optx, len0, false, epoch, val_sh, red_sh, ctl)) {
+            if (!sweep_eval<T, N, C>(a, R, 1, xlast, len0, false, epoch, val_sh, red_sh, ctl)) {
                 status = PLK_ERR_DEVICE;
                 break;
             }
@@ -681,8 +691,21 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.len[B.edge] = optx;
             a.nderv[j] = nd;
+            a.br_len[j] = optx;
         }
         __syncthreads();
+        // computeLikelihoodFromBuffer at the new length (current_it->length)
+        if (B.lnl_index >= 0 && B.lnl_index < a.nlnl) {
+            if (!sweep_eval<T, N, C>(a, R, 1, optx, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+                status = PLK_ERR_DEVICE;
+                break;
+            }
+            if (!isfinite(ctl[0])) {
+                status = PLK_ERR_NONFINITE;
+                break;
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) a.lnl_out[B.lnl_index] = ctl[0];
+        }
     }
     // computeLikelihoodFromBuffer at the last branch
     if (status == 0 && a.nbranches > 0) {

@@ -330,6 +330,90 @@ int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double toleran
     return t->t->optimize_all_branches(iterations, tolerance, max_nr, lnl, n_derv);
 }
 
+int plkh_tree_pass_counts(const plkh_tree* t, int32_t* n_derv, int cap, int* count) {
+    if (!t || !count || cap < 0 || (cap > 0 && !n_derv)) return PLK_ERR_ARG;
+    const std::vector<int>& v = t->t->pass_counts();
+    *count = (int)v.size();
+    for (int k = 0; k < (int)v.size() && k < cap; k++) n_derv[k] = v[k];
+    return PLK_OK;
+}
+
+int plkh_tree_neighbors(const plkh_tree* t, int node, int32_t* nb, int* deg) {
+    if (!t || !nb || !deg) return PLK_ERR_ARG;
+    const Tree& tr = t->t->tree();
+    if (node < 0 || node >= tr.nnodes) return PLK_ERR_ARG;
+    *deg = tr.deg[node];
+    for (int k = 0; k < tr.deg[node]; k++) nb[k] = tr.adj[3 * node + k];
+    return PLK_OK;
+}
+
+int plkh_tree_set_length_keep(plkh_tree* t, int a, int b, double len) {
+    if (!t || !is_branch(t->t, a, b) || !(len >= 0.0)) return PLK_ERR_ARG;
+    t->t->set_length_keep(a, b, len);
+    return PLK_OK;
+}
+
+int plkh_tree_nni_begin(plkh_tree* t, int node1, int node2) {
+    if (!t) return PLK_ERR_ARG;
+    return t->t->nni_begin(node1, node2);
+}
+
+int plkh_tree_nni_swap(plkh_tree* t, int node1, int x, int node2, int y) {
+    if (!t) return PLK_ERR_ARG;
+    return t->t->nni_swap(node1, x, node2, y);
+}
+
+int plkh_tree_clear_partial(plkh_tree* t, int node, int from) {
+    if (!t || !is_branch(t->t, node, from)) return PLK_ERR_ARG;
+    t->t->clear_partial(node, from);
+    return PLK_OK;
+}
+
+int plkh_tree_nni_end(plkh_tree* t) {
+    if (!t) return PLK_ERR_ARG;
+    t->t->nni_end();
+    return PLK_OK;
+}
+
+int plkh_tree_inner_branches(plkh_tree* t, int32_t* a, int32_t* b, int cap, int* count) {
+    if (!t || !count || cap < 0 || (cap > 0 && (!a || !b))) return PLK_ERR_ARG;
+    std::vector<int> u, v;
+    t->t->inner_branches(u, v);
+    *count = (int)u.size();
+    for (int k = 0; k < (int)u.size() && k < cap; k++) {
+        a[k] = u[k];
+        b[k] = v[k];
+    }
+    return PLK_OK;
+}
+
+int plkh_tree_nni_evaluate(plkh_tree* t, int node1, int node2, int max_nr, double* lnl,
+                           double* lens, int32_t* n_derv) {
+    if (!t || !lnl || !lens || max_nr < 1) return PLK_ERR_ARG;
+    int nd[10] = {0};
+    const int st = t->t->nni_evaluate(node1, node2, max_nr, lnl, lens, nd);
+    if (n_derv)
+        for (int k = 0; k < 10; k++) n_derv[k] = nd[k];
+    return st;
+}
+
+int plkh_tree_nni_evaluate_all(plkh_tree* t, int max_nr, double* lnl, double* lens, int32_t* n_derv,
+                               int cap) {
+    if (!t || !lnl || !lens || max_nr < 1 || cap < 0) return PLK_ERR_ARG;
+    std::vector<int> u, v;
+    t->t->inner_branches(u, v);
+    if ((int)u.size() > cap) return PLK_ERR_ARG;
+    std::vector<double> l, ln;
+    std::vector<int> nd;
+    const int st = t->t->nni_evaluate_all(max_nr, l, ln, nd);
+    if (st != PLK_OK) return st;
+    std::memcpy(lnl, l.data(), sizeof(double) * l.size());
+    std::memcpy(lens, ln.data(), sizeof(double) * ln.size());
+    if (n_derv)
+        for (size_t k = 0; k < nd.size(); k++) n_derv[k] = nd[k];
+    return PLK_OK;
+}
+
 int plkh_tree_set_device_newton(plkh_tree* t, int on) {
     if (!t) return PLK_ERR_ARG;
     t->t->device_newton = on != 0;

@@ -32,9 +32,15 @@ public:
     int evals() const { return evals_; }
     // abscissa of the next derivative evaluation (valid while !done())
     double x() const { return rts_; }
+    // abscissa of the last derivative evaluation: the branch length that
+    // PhyloTree::computeFuncDerv left in the tree, which the divergence guard
+    // of optimizeOneBranch evaluates (tree/phylotree.cpp:2616-2618). It equals
+    // result() except when a bisection step makes no progress.
+    double last_x() const { return last_x_; }
 
     void feed(double f, double df) {
         evals_++;
+        last_x_ = rts_;
         f_ = f;
         df_ = df;
         if (first_) {
@@ -86,7 +92,7 @@ private:
     }
 
     double x1_ = 0, x2_ = 0, xacc_ = 0, rts_ = 0, rts_old_ = 0, xl_ = 0, xh_ = 0;
-    double dx_ = 0, dxold_ = 0, f_ = 0, df_ = 0, result_ = 0;
+    double dx_ = 0, dxold_ = 0, f_ = 0, df_ = 0, result_ = 0, last_x_ = 0;
     int j_ = 0, max_nr_ = 0, evals_ = 0;
     bool first_ = true, done_ = false, failed_ = false;
 };

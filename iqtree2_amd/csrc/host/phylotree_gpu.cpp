@@ -81,7 +81,7 @@ void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops, std::ve
         op.child[i].is_tip = ch < tree_.ntaxa;
         op.child[i].id = ch < tree_.ntaxa ? ch : slot(ch);
         op.child[i].length = tree_.length(node, ch);
-        if (edges) edges->push_back(tree_.edge_id(node, ch));
+        if (edges) edges->push_back(eid(node, ch));
     }
     ops.push_back(op);
 }
@@ -106,6 +106,7 @@ void PhyloTreeGPU::plan_sweep(const std::vector<int>& nodes, const std::vector<i
         B.node = plk_end{0, slot(nd)};
         B.dad = dd < tree_.ntaxa ? plk_end{1, dd} : plk_end{0, slot(dd)};
         B.edge = tree_.edge_id(n1, n2);
+        B.lnl_index = -1;
         for (size_t o = 0; o < ops.size(); o++) {
             plk_sweep_op so{};
             so.dad_slot = ops[o].dad_slot;
@@ -250,7 +251,7 @@ int PhyloTreeGPU::likelihood_from_buffer(double* lnl) {
 }
 
 // PhyloTree::optimizeOneBranch (tree/phylotree.cpp:2594-2644)
-int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, int* n_derv) {
+int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, int* n_derv, bool clear) {
     if (!ctx_) return PLK_ERR_STATE;
     cur_a_ = n1;
     cur_b_ = n2;
@@ -267,8 +268,10 @@ int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, i
     theta_b_ = n2;
     if (optx != len) {
         tree_.set_length(n1, n2, optx);
-        clear_reverse(n1, n2);
-        clear_reverse(n2, n1);
+        if (clear) {
+            clear_reverse(n1, n2);
+            clear_reverse(n2, n1);
+        }
     }
     if (new_len) *new_len = optx;
     if (n_derv) *n_derv = nd;
@@ -283,11 +286,16 @@ int PhyloTreeGPU::optimize_all_branches(int iterations, double tolerance, int ma
     // the device sweep is built for 4 and 20 states; codon models take the host loop
     if (!device_newton || (nstates_ != 4 && nstates_ != 20))
         return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
+    // a sweep holds a pass of at most PLK_SWEEP_MAX_BRANCHES branches
+    if (tree_.nbranch() > PLK_SWEEP_MAX_BRANCHES)
+        return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
     std::vector<int> nodes, nodes2;
     best_traversal(nodes, nodes2);
     double tree_lh;
+    // a non-finite lnL is fixed up per pattern and only warned about
+    // (tree/phylokernelnew.h:3106-3132)
     int st = likelihood_branch(nodes2[0], nodes[0], &tree_lh, nullptr);
-    if (st != PLK_OK) return st;
+    if (st != PLK_OK && st != PLK_ERR_NONFINITE) return st;
     int total_derv = 0;
     const int ne = tree_.nbranch();
     for (int it = 0; it < iterations; it++) {
@@ -301,9 +309,21 @@ int PhyloTreeGPU::optimize_all_branches(int iterations, double tolerance, int ma
         double new_lh = 0.0;
         st = plk_optimize_sweep(ctx_, inst_, sops.data(), (int)sops.size(), br.data(), (int)br.size(),
                                 lens.data(), ne, min_len, max_len, max_nr, nd.data(), &new_lh);
-        if (st != PLK_OK) return st;
+        if (st != PLK_OK) {
+            // plan_sweep marked the planned partials computed; a sweep that did
+            // not run, or stopped early, leaves some of them stale or rebuilt
+            // with lengths the host does not hold
+            clear_all_partial();
+            cur_a_ = cur_b_ = -1;
+            if (st == PLK_ERR_UNSUPPORTED && it == 0) {
+                // shapes the sweep is not built for (its LDS layout): the host loop
+                return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
+            }
+            return st;
+        }
         for (int e = 0; e < ne; e++) tree_.set_length(tree_.eu[e], tree_.ev[e], lens[e]);
         for (int x : nd) total_derv += x;
+        pass_nd_ = nd;
         theta_a_ = nodes.back();
         theta_b_ = nodes2.back();
         cur_a_ = theta_a_;
@@ -334,15 +354,19 @@ int PhyloTreeGPU::optimize_all_branches_host(int iterations, double tolerance, i
     std::vector<int> nodes, nodes2;
     best_traversal(nodes, nodes2);
     double tree_lh;
+    // a non-finite lnL is fixed up per pattern and only warned about
+    // (tree/phylokernelnew.h:3106-3132)
     int st = likelihood_branch(nodes2[0], nodes[0], &tree_lh, nullptr);
-    if (st != PLK_OK) return st;
+    if (st != PLK_OK && st != PLK_ERR_NONFINITE) return st;
     int total_derv = 0;
     for (int it = 0; it < iterations; it++) {
         std::vector<double> saved(tree_.len);
+        pass_nd_.assign(nodes.size(), 0);
         for (size_t j = 0; j < nodes.size(); j++) {
             int nd = 0;
             st = optimize_branch(nodes[j], nodes2[j], max_nr, nullptr, &nd);
             total_derv += nd;
+            pass_nd_[j] = nd;
             if (st != PLK_OK) return st;
         }
         double new_lh;
@@ -365,6 +389,263 @@ int PhyloTreeGPU::optimize_all_branches_host(int iterations, double tolerance, i
     }
     if (lnl) *lnl = tree_lh;
     if (n_derv) *n_derv = total_derv;
+    return PLK_OK;
+}
+
+// ---- NNI evaluation ----------------------------------------------------------
+
+int PhyloTreeGPU::nni_begin(int node1, int node2) {
+    const Tree& t = tree_;
+    if (nni_active() || node1 < t.ntaxa || node2 < t.ntaxa || node1 >= t.nnodes || node2 >= t.nnodes ||
+        t.deg[node1] != 3 || t.deg[node2] != 3 || t.neighbor_index(node1, node2) < 0)
+        return PLK_ERR_ARG;
+    nni_saved_tree_ = tree_;
+    for (int k = 0; k < 3; k++) {
+        nni_saved_comp_[0][k] = computed_[(size_t)node1 * 3 + k];
+        nni_saved_comp_[1][k] = computed_[(size_t)node2 * 3 + k];
+    }
+    nni_saved_owner_[0] = owner_[node1];
+    nni_saved_owner_[1] = owner_[node2];
+    // NNI buffers in the order getBestNNIForBran hands them out: the saved
+    // neighbours are node1->node2, node2->node1, then node1's and node2's
+    // other neighbours' back pointers, and the first that owns a buffer gets
+    // the first NNI buffer; node2's buffer is first only when the
+    // orientation it holds is the one seen from node1
+    const bool node2_first = owner_[node2] >= 0 && owner_[node2] == t.neighbor_index(node2, node1);
+    nni_node_[0] = node1;
+    nni_node_[1] = node2;
+    nni_slot_[0] = node2_first ? 1 : 0;
+    nni_slot_[1] = node2_first ? 0 : 1;
+    for (int k = 0; k < 3; k++) {
+        computed_[(size_t)node1 * 3 + k] = 0;
+        computed_[(size_t)node2 * 3 + k] = 0;
+    }
+    owner_[node1] = owner_[node2] = -1;
+    theta_a_ = theta_b_ = -1;
+    return PLK_OK;
+}
+
+int PhyloTreeGPU::nni_swap(int node1, int x, int node2, int y) {
+    Tree& t = tree_;
+    const int k1 = t.neighbor_index(node1, x), k2 = t.neighbor_index(node2, y);
+    if (!nni_active() || k1 < 0 || k2 < 0 || x == node2 || y == node1) return PLK_ERR_ARG;
+    const int kx = t.neighbor_index(x, node1), ky = t.neighbor_index(y, node2);
+    const int e1 = t.edge[3 * node1 + k1], e2 = t.edge[3 * node2 + k2];
+    const double l1 = t.len[3 * node1 + k1], l2 = t.len[3 * node2 + k2];
+    t.adj[3 * node1 + k1] = y;
+    t.adj[3 * node2 + k2] = x;
+    t.adj[3 * x + kx] = node2;
+    t.adj[3 * y + ky] = node1;
+    t.edge[3 * node1 + k1] = e2;
+    t.len[3 * node1 + k1] = l2;
+    t.edge[3 * node2 + k2] = e1;
+    t.len[3 * node2 + k2] = l1;
+    t.eu[e1] = x;
+    t.ev[e1] = node2;
+    t.eu[e2] = y;
+    t.ev[e2] = node1;
+    // the subtree sizes seen from the swapped nodes are stale
+    std::fill(size_.begin(), size_.end(), 0);
+    // every orientation of node1 and node2 contains the exchanged subtrees;
+    // x's and y's partials seen from their new neighbour are their own
+    // subtrees, unchanged
+    for (int k = 0; k < 3; k++) {
+        computed_[(size_t)node1 * 3 + k] = 0;
+        computed_[(size_t)node2 * 3 + k] = 0;
+    }
+    owner_[node1] = owner_[node2] = -1;
+    theta_a_ = theta_b_ = -1;
+    return PLK_OK;
+}
+
+void PhyloTreeGPU::clear_partial(int node, int from) {
+    if (node < tree_.ntaxa) return;
+    const int k = tree_.neighbor_index(node, from);
+    if (k < 0) return;
+    computed_[(size_t)node * 3 + k] = 0;
+    if (owner_[node] == k) owner_[node] = -1;
+}
+
+void PhyloTreeGPU::nni_end() {
+    if (!nni_active()) return;
+    const int node1 = nni_node_[0], node2 = nni_node_[1];
+    tree_ = nni_saved_tree_;
+    std::fill(size_.begin(), size_.end(), 0);
+    for (int k = 0; k < 3; k++) {
+        computed_[(size_t)node1 * 3 + k] = nni_saved_comp_[0][k];
+        computed_[(size_t)node2 * 3 + k] = nni_saved_comp_[1][k];
+    }
+    owner_[node1] = nni_saved_owner_[0];
+    owner_[node2] = nni_saved_owner_[1];
+    nni_node_[0] = nni_node_[1] = -1;
+    theta_a_ = theta_b_ = -1;
+    cur_a_ = cur_b_ = -1;
+}
+
+void PhyloTreeGPU::inner_branches(std::vector<int>& a, std::vector<int>& b) {
+    a.clear();
+    b.clear();
+    for (int e = 0; e < tree_.nbranch(); e++)
+        if (tree_.eu[e] >= tree_.ntaxa && tree_.ev[e] >= tree_.ntaxa) {
+            a.push_back(tree_.eu[e]);
+            b.push_back(tree_.ev[e]);
+        }
+}
+
+namespace {
+// node's neighbours other than `except`, in adjacency order
+std::vector<int> others(const Tree& t, int node, int except) {
+    std::vector<int> v;
+    for (int k = 0; k < t.deg[node]; k++)
+        if (t.adj[3 * node + k] != except) v.push_back(t.adj[3 * node + k]);
+    return v;
+}
+}  // namespace
+
+int PhyloTreeGPU::nni_evaluate(int node1, int node2, int max_nr, double lnl[2], double lens[10],
+                               int nd[10]) {
+    if (!ctx_) return PLK_ERR_STATE;
+    int st = nni_begin(node1, node2);
+    if (st != PLK_OK) return st;
+    const std::vector<int> n1 = others(tree_, node1, node2), n2 = others(tree_, node2, node1);
+    for (int cnt = 0; cnt < 2 && st == PLK_OK; cnt++) {
+        // the move exchanges node1's first neighbour with node2's cnt-th
+        const int a = n1[0], c = n2[cnt];
+        nni_swap(node1, a, node2, c);
+        clear_partial(node2, node1);   // node12_it
+        clear_partial(node1, node2);   // node21_it
+        double* L = lens + 5 * cnt;
+        int* D = nd + 5 * cnt;
+        int i = 1;
+        for (int x : others(tree_, node1, node2)) {
+            clear_partial(node1, x);
+            if ((st = optimize_branch(node1, x, max_nr, &L[i], &D[i], false)) != PLK_OK) break;
+            i++;
+        }
+        if (st != PLK_OK) break;
+        clear_partial(node1, node2);
+        if ((st = optimize_branch(node1, node2, max_nr, &L[0], &D[0], false)) != PLK_OK) break;
+        for (int y : others(tree_, node2, node1)) {
+            clear_partial(node2, y);
+            if ((st = optimize_branch(node2, y, max_nr, &L[i], &D[i], false)) != PLK_OK) break;
+            i++;
+        }
+        if (st != PLK_OK) break;
+        clear_partial(node2, node1);
+        st = likelihood_from_buffer(&lnl[cnt]);
+        if (st == PLK_ERR_NONFINITE && safe_) st = PLK_OK;   // fixed up per pattern
+        nni_swap(node1, c, node2, a);
+    }
+    nni_end();
+    return st;
+}
+
+int PhyloTreeGPU::nni_evaluate_all(int max_nr, std::vector<double>& lnl, std::vector<double>& lens,
+                                   std::vector<int>& nd) {
+    if (!ctx_) return PLK_ERR_STATE;
+    if (nstates_ != 4 && nstates_ != 20) return PLK_ERR_UNSUPPORTED;
+    std::vector<int> I1, I2;
+    inner_branches(I1, I2);
+    const int m = (int)I1.size(), ne = tree_.nbranch();
+    // every inner branch's five edges get lengths of their own (ids ne + 5j ..),
+    // initialised with the current lengths, so each evaluation starts from the
+    // tree's lengths and the moves of one evaluation carry them over
+    std::vector<double> L(ne + 5 * (size_t)m);
+    for (int e = 0; e < ne; e++) L[e] = tree_.length(tree_.eu[e], tree_.ev[e]);
+    std::vector<plk_sweep_op> sops;
+    std::vector<plk_sweep_branch> br;
+    edge_map_.assign(ne, -1);
+    int st = PLK_OK;
+    for (int j = 0; j < m && st == PLK_OK; j++) {
+        const int node1 = I1[j], node2 = I2[j];
+        if ((st = nni_begin(node1, node2)) != PLK_OK) break;
+        const std::vector<int> n1 = others(tree_, node1, node2), n2 = others(tree_, node2, node1);
+        const int own[5][2] = {{node1, node2}, {node1, n1[0]}, {node1, n1[1]}, {node2, n2[0]}, {node2, n2[1]}};
+        for (int q = 0; q < 5; q++) {
+            const int e = tree_.edge_id(own[q][0], own[q][1]);
+            edge_map_[e] = ne + 5 * j + q;
+            L[ne + 5 * j + q] = tree_.length(own[q][0], own[q][1]);
+        }
+        auto add = [&](int n_a, int n_b, int lnl_index) {
+            // optimizeOneBranch(n_a, n_b): theta of (node = n_b side, dad = n_a side)
+            int nd_ = n_b, dd = n_a;
+            if (nd_ < tree_.ntaxa) std::swap(nd_, dd);
+            std::vector<plk_op> ops;
+            std::vector<int> edges;
+            plan_rec(nd_, dd, ops, &edges);
+            plan_rec(dd, nd_, ops, &edges);
+            plk_sweep_branch B{};
+            B.op_begin = (int)sops.size();
+            B.op_count = (int)ops.size();
+            B.node = plk_end{0, slot(nd_)};
+            B.dad = dd < tree_.ntaxa ? plk_end{1, dd} : plk_end{0, slot(dd)};
+            B.edge = eid(n_a, n_b);
+            B.lnl_index = lnl_index;
+            for (size_t o = 0; o < ops.size(); o++) {
+                plk_sweep_op so{};
+                so.dad_slot = ops[o].dad_slot;
+                so.nchildren = ops[o].nchildren;
+                for (int k = 0; k < ops[o].nchildren; k++) {
+                    so.child[k].is_tip = ops[o].child[k].is_tip;
+                    so.child[k].id = ops[o].child[k].id;
+                    so.child[k].edge = edges[2 * o + k];
+                }
+                sops.push_back(so);
+            }
+            br.push_back(B);
+        };
+        for (int cnt = 0; cnt < 2; cnt++) {
+            const int a = n1[0], c = n2[cnt];
+            nni_swap(node1, a, node2, c);
+            clear_partial(node2, node1);
+            clear_partial(node1, node2);
+            for (int x : others(tree_, node1, node2)) {
+                clear_partial(node1, x);
+                add(node1, x, -1);
+            }
+            clear_partial(node1, node2);
+            add(node1, node2, -1);
+            const std::vector<int> ys = others(tree_, node2, node1);
+            for (size_t q = 0; q < ys.size(); q++) {
+                clear_partial(node2, ys[q]);
+                add(node2, ys[q], q + 1 == ys.size() ? 2 * j + cnt : -1);
+            }
+            clear_partial(node2, node1);
+            nni_swap(node1, c, node2, a);
+        }
+        nni_end();
+        for (int q = 0; q < 5; q++) edge_map_[tree_.edge_id(own[q][0], own[q][1])] = -1;
+    }
+    edge_map_.clear();
+    if (st != PLK_OK) {
+        clear_all_partial();
+        return st;
+    }
+    std::vector<int> ndv(br.size());
+    std::vector<double> brlen(br.size());
+    lnl.assign(2 * (size_t)m, 0.0);
+    double last = 0.0;
+    st = plk_optimize_sweep_ex(ctx_, inst_, sops.data(), (int)sops.size(), br.data(), (int)br.size(),
+                               L.data(), (int)L.size(), min_len, max_len, max_nr, ndv.data(),
+                               brlen.data(), lnl.data(), 2 * m, &last);
+    if (st != PLK_OK) {
+        clear_all_partial();
+        return st;
+    }
+    // per inner branch and move, the five branches in the order of
+    // nni_evaluate's output: central, node1's, node2's
+    lens.assign(10 * (size_t)m, 0.0);
+    nd.assign(10 * (size_t)m, 0);
+    for (int j = 0; j < m; j++)
+        for (int cnt = 0; cnt < 2; cnt++) {
+            const int b0 = 10 * j + 5 * cnt;   // sweep order: node1 x2, central, node2 x2
+            const int order[5] = {2, 0, 1, 3, 4};
+            for (int q = 0; q < 5; q++) {
+                lens[b0 + q] = brlen[b0 + order[q]];
+                nd[b0 + q] = ndv[b0 + order[q]];
+            }
+        }
+    theta_a_ = theta_b_ = -1;
     return PLK_OK;
 }
 

@@ -18,8 +18,16 @@ public:
     int instance() const { return inst_; }
     const Tree& tree() const { return tree_; }
 
-    int slot(int v) const { return v - tree_.ntaxa; }
-    int num_slots() const { return tree_.nnodes - tree_.ntaxa; }
+    // LM_PER_NODE: one buffer per internal node, then the two NNI buffers
+    // (tree/phylotree.cpp:1070-1076); during an NNI evaluation its two nodes
+    // live in the NNI buffers (4127-4136)
+    int slot(int v) const {
+        if (v == nni_node_[0]) return num_central_slots() + nni_slot_[0];
+        if (v == nni_node_[1]) return num_central_slots() + nni_slot_[1];
+        return v - tree_.ntaxa;
+    }
+    int num_central_slots() const { return tree_.nnodes - tree_.ntaxa; }
+    int num_slots() const { return num_central_slots() + 2; }
 
     // computeBestTraversal (tree/phylotree.cpp:2689-2696)
     void best_traversal(std::vector<int>& nodes, std::vector<int>& nodes2);
@@ -33,6 +41,8 @@ public:
                                    int* n_derv);
     bool device_newton = true;
     void set_length(int a, int b, double len);
+    // the length alone, no partial invalidated (optimizeOneBranch with clearLH = false)
+    void set_length_keep(int a, int b, double len) { tree_.set_length(a, b, len); }
     void clear_all_partial();
     void reset();
     // PhyloNode::clearReversePartialLh (tree/phylonode.cpp:22-33)
@@ -41,9 +51,37 @@ public:
     int compute_likelihood(double* lnl, double* pattern_lh);
     int likelihood_branch(int a, int b, double* lnl, double* pattern_lh);
     int likelihood_derv(int a, int b, double* df, double* ddf);
-    int optimize_branch(int a, int b, int max_nr, double* new_len, int* n_derv);
+    // optimizeOneBranch(a, b, clearLH, max_nr)
+    int optimize_branch(int a, int b, int max_nr, double* new_len, int* n_derv, bool clear = true);
     int optimize_all_branches(int iterations, double tolerance, int max_nr, double* lnl, int* n_derv);
     int likelihood_from_buffer(double* lnl);
+    // derivative evaluations per branch of the last optimizeAllBranches pass,
+    // in best_traversal order
+    const std::vector<int>& pass_counts() const { return pass_nd_; }
+
+    // ---- NNI evaluation (getBestNNIForBran, tree/phylotree.cpp:4069-4300) ----
+    // node1 and node2 (adjacent internal nodes) move to the NNI buffers, with
+    // every orientation of them stale
+    int nni_begin(int node1, int node2);
+    // exchange neighbour x of node1 with neighbour y of node2; the edges keep
+    // their lengths and ids (Node::updateNeighbor)
+    int nni_swap(int node1, int x, int node2, int y);
+    // PhyloNeighbor::clearPartialLh of the partial of `node` seen from `from`
+    void clear_partial(int node, int from);
+    // restores the topology, the lengths and node1 and node2's buffers
+    void nni_end();
+    bool nni_active() const { return nni_node_[0] >= 0; }
+    // the two moves of getBestNNIForBran (nni5, nni5_num_eval 1): per move the
+    // tree lnL from the buffer and the five optimised lengths (central branch
+    // first, then node1's and node2's other branches); one engine call per
+    // derivative evaluation
+    int nni_evaluate(int node1, int node2, int max_nr, double lnl[2], double lens[10], int nd[10]);
+    // inner branches (both ends internal), by edge id
+    void inner_branches(std::vector<int>& a, std::vector<int>& b);
+    // nni_evaluate for every inner branch in one device-resident sweep;
+    // lnl [2m], lens [10m], nd [10m] for the m inner branches
+    int nni_evaluate_all(int max_nr, std::vector<double>& lnl, std::vector<double>& lens,
+                         std::vector<int>& nd);
 
     double min_len = 1e-6, max_len = 10.0;
 
@@ -66,6 +104,18 @@ private:
     std::vector<int> size_;        // [nnodes*3] leaves below v away from adj[v][k]; 0 = unknown
     int theta_a_ = -1, theta_b_ = -1;   // branch whose theta buffer is current
     int cur_a_ = -1, cur_b_ = -1;       // current_it of PhyloTree
+    std::vector<int> pass_nd_;
+    // NNI evaluation in progress
+    int nni_node_[2] = {-1, -1}, nni_slot_[2] = {0, 1};
+    Tree nni_saved_tree_;
+    char nni_saved_comp_[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int nni_saved_owner_[2] = {-1, -1};
+    // sweep planning: edge id -> the id the sweep's ops use (-1: itself)
+    std::vector<int> edge_map_;
+    int eid(int u, int v) const {
+        const int e = tree_.edge_id(u, v);
+        return edge_map_.empty() || edge_map_[e] < 0 ? e : edge_map_[e];
+    }
 };
 
 }  // namespace plk

@@ -30,8 +30,9 @@ struct Req {
     int inst = -1;
     enum Phase { TRAVERSE, THETA, NEWTON, FINAL, DONE } phase = TRAVERSE;
     plk::NewtonRun nr;
-    double optx = 0.0, lnl_opt = 0.0, lnl_orig = 0.0;
-    bool need_orig = false;
+    double optx = 0.0, lnl_opt = 0.0, lnl_last = 0.0, lnl_orig = 0.0;
+    bool need_orig = false;   // divergence guard: lnL at the last evaluated and the old length
+    bool need_last = false;   // the last evaluated length differs from optx
     plks_result res{};
     std::vector<double> rell;
 };
@@ -183,7 +184,9 @@ void step(plks_server* s) {
     // one derivative evaluation for every optimisation in progress, and the
     // final lnL evaluations of those whose Newton iteration has ended
     std::vector<Req*> dv;
-    std::vector<int> ids, which;   // which: 0 Newton step, 1 lnL at optx, 2 lnL at the old length
+    // which: 0 Newton step, 1 lnL at optx, 2 lnL at the old length, 3 lnL at
+    // the last evaluated length
+    std::vector<int> ids, which;
     std::vector<double> xs;
     for (Req* q : s->inflight) {
         if (q->phase == Req::NEWTON) {
@@ -201,6 +204,12 @@ void step(plks_server* s) {
                 ids.push_back(q->inst);
                 xs.push_back(q->r.length);
                 which.push_back(2);
+            }
+            if (q->need_last) {
+                dv.push_back(q);
+                ids.push_back(q->inst);
+                xs.push_back(q->nr.last_x());
+                which.push_back(3);
             }
         }
     }
@@ -226,7 +235,10 @@ void step(plks_server* s) {
                 Req* q = dv[k];
                 if (q->phase == Req::DONE) continue;
                 if (which[k] == 0) {
-                    if (std::isnan(lnl[k])) {   // non-finite derivative, as plk_optimize_branch
+                    // a non-finite derivative (marked by lnl = NaN, df = ddf = 0):
+                    // an error of the normal kernel, df = ddf = 0 for the safe
+                    // one (tree/phylokernelnew.h:2577-2579, 2629-2632)
+                    if (std::isnan(lnl[k]) && !q->r.desc.safe_numeric) {
                         q->res.status = PLK_ERR_NONFINITE;
                         q->phase = Req::DONE;
                         continue;
@@ -242,16 +254,35 @@ void step(plks_server* s) {
                     // at the old length (tree/phylotree.cpp:2616-2625)
                     q->optx = q->nr.result();
                     q->need_orig = q->optx > q->r.max_len * 0.95;
+                    q->need_last = q->need_orig && q->nr.last_x() != q->optx;
                     q->phase = Req::FINAL;
                 } else if (which[k] == 1) {
                     q->lnl_opt = lnl[k];
                     finals.push_back(q);
-                } else {
+                } else if (which[k] == 2) {
                     q->lnl_orig = lnl[k];
+                } else {
+                    q->lnl_last = lnl[k];
                 }
             }
             for (Req* q : finals) {
-                if (q->need_orig && q->lnl_orig > q->lnl_opt) {
+                // lnL values the batch could not form: computeLikelihoodFromBuffer
+                // with its per-pattern fix-up, one request at a time
+                int fst = PLK_OK;
+                auto fixed = [&](double x, double& v) {
+                    if (fst == PLK_OK && !std::isfinite(v))
+                        fst = plk_likelihood_from_buffer(ctx, q->inst, x, &v, nullptr);
+                };
+                fixed(q->optx, q->lnl_opt);
+                if (q->need_orig) fixed(q->r.length, q->lnl_orig);
+                if (q->need_last) fixed(q->nr.last_x(), q->lnl_last);
+                if (fst != PLK_OK) {
+                    q->res.status = fst;
+                    q->phase = Req::DONE;
+                    continue;
+                }
+                const double guard_lh = q->need_last ? q->lnl_last : q->lnl_opt;
+                if (q->need_orig && q->lnl_orig > guard_lh) {
                     q->res.length = q->r.length;
                     q->res.lnl = q->lnl_orig;
                 } else {

@@ -10,13 +10,14 @@ PhyloTree does (tree/phylotree.cpp:2594-2773, 5918-6028); `Workload` and
 `Patterns` expose the host data formats (site patterns, model eigensystem).
 """
 import ctypes
+import weakref
 
 import numpy as np
 
 from . import _capi
 from ._capi import (OP_DTYPE, PREC_F32, PREC_F64, PlkConfig, PlkEnd, PlkError,
                     PlkhWorkloadSpec, PlkInstanceDesc, check, dptr, f32ptr, i32ptr, lib,
-                    u8ptr)
+                    shutting_down, u8ptr)
 
 __all__ = ["Context", "Instance", "PhyloTree", "Workload", "Patterns", "end", "ops_array",
            "likelihood_branch_batch",
@@ -64,10 +65,14 @@ class Workload:
         (self.ntaxa, self.nstates, self.ncat, self.npatterns, self.state_unknown,
          self.nsite) = [d.value for d in dims]
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "_h", None):
             lib().plkh_workload_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        if not shutting_down():
+            self.close()
 
     @property
     def handle(self):
@@ -134,19 +139,25 @@ class Patterns:
         return out
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and not shutting_down():
             lib().plkh_patterns_destroy(self._h)
             self._h = None
 
 
 class Context:
-    """plk_context: one device, one stream, an HBM budget."""
+    """plk_context: one device, one stream, an HBM budget. close() first
+    closes every instance, tree and server created on the context."""
 
     def __init__(self, device=0, stream=None, hbm_budget=0):
         cfg = PlkConfig(device, stream, hbm_budget)
         h = ctypes.c_void_p()
+        self._children = weakref.WeakSet()
         check(lib().plk_create(ctypes.byref(cfg), ctypes.byref(h)), "plk_create")
         self._h = h
+
+    def adopt(self, child):
+        """Registers an object holding device state on this context."""
+        self._children.add(child)
 
     @property
     def handle(self):
@@ -177,11 +188,14 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
+            for child in list(self._children):
+                child.close()
             lib().plk_destroy(self._h)
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not shutting_down():
+            self.close()
 
 
 class Instance:
@@ -196,6 +210,7 @@ class Instance:
         check(lib().plk_instance_create(ctx.handle, ctypes.byref(self.desc), ctypes.byref(iid)),
               "plk_instance_create")
         self.id = iid.value
+        ctx.adopt(self)
         self.P = npatterns
         self.block = nstates * ncat
         self.safe = bool(safe_numeric)
@@ -210,7 +225,9 @@ class Instance:
 
     @classmethod
     def from_workload(cls, ctx, w, precision=PREC_F64, safe_numeric=False):
-        inst = cls(ctx, w.ntaxa, w.nstates, w.ncat, w.npatterns, w.ntaxa - 2, w.state_unknown,
+        """Instance with the planner's slots: one per internal node (ntaxa - 2)
+        and the two NNI buffers."""
+        inst = cls(ctx, w.ntaxa, w.nstates, w.ncat, w.npatterns, w.ntaxa, w.state_unknown,
                    precision, safe_numeric)
         inst.set_patterns(w.get("states"), w.get("freq"), w.get("invar"))
         m = w.model_arrays()
@@ -218,9 +235,13 @@ class Instance:
         return inst
 
     def close(self):
-        if self.id >= 0 and self.ctx.handle:
+        if getattr(self, "id", -1) >= 0 and self.ctx.handle:
             lib().plk_instance_destroy(self.ctx.handle, self.id)
         self.id = -1
+
+    def __del__(self):
+        if not shutting_down():
+            self.close()
 
     def set_patterns(self, states, freq, invar):
         s = np.ascontiguousarray(states, np.uint8)
@@ -284,11 +305,13 @@ class Instance:
               "plk_rell")
         return out
 
-    def read_partial(self, slot):
-        part = np.zeros((self.P, self.block))
+    def read_partial(self, slot, partial=True):
+        """(partial [P][block] FP64 or None, scale counters) of one slot."""
+        part = np.zeros((self.P, self.block)) if partial else None
         scale = np.zeros((self.P, self.ncat) if self.safe else (self.P,), np.int32)
-        check(lib().plk_read_partial(self.ctx.handle, self.id, int(slot), dptr(part),
-                                     i32ptr(scale)), "plk_read_partial")
+        check(lib().plk_read_partial(self.ctx.handle, self.id, int(slot),
+                                     dptr(part) if partial else None, i32ptr(scale)),
+              "plk_read_partial")
         return part, scale
 
 
@@ -358,6 +381,8 @@ class PhyloTree:
         self.ctx = ctx
         self.workload = workload
         self.P = workload.npatterns
+        if ctx is not None:
+            ctx.adopt(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -365,7 +390,8 @@ class PhyloTree:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not shutting_down():
+            self.close()
 
     @property
     def instance_id(self):
@@ -434,6 +460,63 @@ class PhyloTree:
                                                     ctypes.byref(lnl), ctypes.byref(nd)),
               "plkh_tree_optimize_all_branches")
         return lnl.value, nd.value
+
+    def pass_counts(self):
+        """Derivative evaluations per branch of the last optimize_all_branches
+        pass, in branches() order."""
+        n = ctypes.c_int()
+        check(lib().plkh_tree_pass_counts(self._h, None, 0, ctypes.byref(n)), "plkh_tree_pass_counts")
+        out = np.zeros(n.value, np.int32)
+        check(lib().plkh_tree_pass_counts(self._h, i32ptr(out), n.value, ctypes.byref(n)),
+              "plkh_tree_pass_counts")
+        return out
+
+    def neighbors(self, node):
+        nb, deg = np.zeros(3, np.int32), ctypes.c_int()
+        check(lib().plkh_tree_neighbors(self._h, int(node), i32ptr(nb), ctypes.byref(deg)),
+              "plkh_tree_neighbors")
+        return nb[:deg.value].tolist()
+
+    def set_length_keep(self, a, b, length):
+        check(lib().plkh_tree_set_length_keep(self._h, a, b, float(length)),
+              "plkh_tree_set_length_keep")
+
+    def nni_begin(self, node1, node2):
+        check(lib().plkh_tree_nni_begin(self._h, node1, node2), "plkh_tree_nni_begin")
+
+    def nni_swap(self, node1, x, node2, y):
+        check(lib().plkh_tree_nni_swap(self._h, node1, x, node2, y), "plkh_tree_nni_swap")
+
+    def clear_partial(self, node, frm):
+        check(lib().plkh_tree_clear_partial(self._h, node, frm), "plkh_tree_clear_partial")
+
+    def nni_end(self):
+        check(lib().plkh_tree_nni_end(self._h), "plkh_tree_nni_end")
+
+    def inner_branches(self):
+        n = ctypes.c_int()
+        check(lib().plkh_tree_inner_branches(self._h, None, None, 0, ctypes.byref(n)),
+              "plkh_tree_inner_branches")
+        a, b = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        check(lib().plkh_tree_inner_branches(self._h, i32ptr(a), i32ptr(b), n.value,
+                                             ctypes.byref(n)), "plkh_tree_inner_branches")
+        return list(zip(a.tolist(), b.tolist()))
+
+    def nni_evaluate(self, node1, node2, max_nr=10):
+        """getBestNNIForBran's two moves: (lnl[2], lens[2][5], n_derv[2][5])."""
+        lnl, lens, nd = np.zeros(2), np.zeros(10), np.zeros(10, np.int32)
+        check(lib().plkh_tree_nni_evaluate(self._h, node1, node2, max_nr, dptr(lnl), dptr(lens),
+                                           i32ptr(nd)), "plkh_tree_nni_evaluate")
+        return lnl, lens.reshape(2, 5), nd.reshape(2, 5)
+
+    def nni_evaluate_all(self, max_nr=10):
+        """Every inner branch's NNI evaluation in one device sweep:
+        (lnl[m][2], lens[m][2][5], n_derv[m][2][5])."""
+        m = len(self.inner_branches())
+        lnl, lens, nd = np.zeros(2 * m), np.zeros(10 * m), np.zeros(10 * m, np.int32)
+        check(lib().plkh_tree_nni_evaluate_all(self._h, max_nr, dptr(lnl), dptr(lens), i32ptr(nd),
+                                               m), "plkh_tree_nni_evaluate_all")
+        return lnl.reshape(m, 2), lens.reshape(m, 2, 5), nd.reshape(m, 2, 5)
 
     def set_device_newton(self, on):
         check(lib().plkh_tree_set_device_newton(self._h, int(bool(on))),

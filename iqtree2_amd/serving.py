@@ -23,7 +23,7 @@ class Request:
     """The inputs of one evaluation on branch `branch` of a workload's tree."""
 
     def __init__(self, workload, kind, branch=0, weights=None, precision=PREC_F64,
-                 min_len=1e-6, max_len=10.0, max_nr=100, length_scale=1.0):
+                 min_len=1e-6, max_len=10.0, max_nr=100, length_scale=1.0, safe_numeric=None):
         w = workload
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
         planner = PhyloTree(None, w)
@@ -35,7 +35,7 @@ class Request:
         self.length = float(length) * length_scale
         # IQ-TREE runs every state count other than 4 and 20 with per-category
         # scaling (tree/phylotreesse.cpp:94-95)
-        safe = int(w.nstates not in (4, 20))
+        safe = int(w.nstates not in (4, 20) or bool(safe_numeric))
         self.desc = PlkInstanceDesc(w.ntaxa, w.nstates, w.ncat, w.npatterns, w.ntaxa - 2,
                                     w.state_unknown, precision, safe)
         self.states = np.ascontiguousarray(w.get("states"), np.uint8)
@@ -88,6 +88,7 @@ class Server:
         check(lib().plks_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "plks_create")
         self._h = h
         self._nboot = {}
+        ctx.adopt(self)
 
     def submit(self, req):
         t = ctypes.c_uint64()
@@ -132,7 +133,5 @@ class Server:
         self.close()
 
     def __del__(self):
-        try:
+        if not _capi.shutting_down():
             self.close()
-        except Exception:
-            pass

@@ -86,7 +86,8 @@ class OracleTree:
     @classmethod
     def from_arrays(cls, ntaxa, nstates, ncat, state_unknown, states, freq, invar, model,
                     safe_numeric=False, nthreads=1):
-        t = cls(ntaxa, nstates, ncat, len(freq), ntaxa - 2, state_unknown, safe_numeric, nthreads)
+        # one slot per internal node and the two NNI buffers, as the planner numbers them
+        t = cls(ntaxa, nstates, ncat, len(freq), ntaxa, state_unknown, safe_numeric, nthreads)
         t.set_patterns(states, freq, invar)
         t.set_model(model["eval"], model["evec"], model["inv_evec"], model["tip"], model["rates"],
                     model["props"])
@@ -151,9 +152,10 @@ class OracleTree:
                                       ctypes.byref(lnl), ctypes.byref(nd), ctypes.byref(st))
         return x, lnl.value, nd.value, st.value
 
-    def read_partial(self, slot):
-        part = np.zeros((self.P, self.block))
+    def read_partial(self, slot, partial=True):
+        part = np.zeros((self.P, self.block)) if partial else None
         scale = np.zeros((self.P, self.ncat) if self.safe else (self.P,), np.int32)
-        lib().orc_read_partial(self._h, int(slot), part.ctypes.data_as(_DP),
+        lib().orc_read_partial(self._h, int(slot),
+                               part.ctypes.data_as(_DP) if partial else None,
                                scale.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         return part, scale

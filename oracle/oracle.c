@@ -36,6 +36,7 @@ struct orc_tree {
     double* buffer_scale;  /* [Ppad] */
     double* pattern_lh;    /* [Ppad] */
     int theta_ready;
+    double last_x;         /* length of the last derivative evaluation (Newton) */
     double* info;          /* per-op echild + leaf tables */
     size_t info_cap;
     double* tipbuf;        /* per-packet tip staging [npackets][2*block*VS] */
@@ -902,7 +903,13 @@ double orc_minimize_newton(double x1, double xguess, double x2, double xacc, dou
 static void tree_fderv(void* user, double x, double* f, double* df) {
     orc_tree* t = (orc_tree*)user;
     double d1, d2;
-    orc_likelihood_derv(t, x, &d1, &d2);
+    t->last_x = x;   /* the length computeFuncDerv leaves in the tree */
+    if (orc_likelihood_derv(t, x, &d1, &d2) != 0 && !t->safe) {
+        /* the normal kernel stops with outError (tree/phylokernelnew.h:2577-2579);
+         * the safe one continues with df = ddf = 0 (2629-2632) */
+        *f = *df = NAN;
+        return;
+    }
     *f = -d1;
     *df = -d2;
 }
@@ -916,8 +923,10 @@ double orc_optimize_branch(orc_tree* t, const orc_op* ops, int nops, orc_end nod
     int st = 0, evals = 0;
     double optx = orc_minimize_newton(min_len, length, max_len, min_len, &d2l, max_nr, tree_fderv, t,
                                       &evals, &st);
-    if (optx > max_len * 0.95) {
-        const double opt_lh = orc_likelihood_from_buffer(t, optx, NULL);
+    if (st == 0 && optx > max_len * 0.95) {
+        /* computeLikelihoodFromBuffer at the length the last derivative
+         * evaluation left in the tree, then at the old length (2616-2625) */
+        const double opt_lh = orc_likelihood_from_buffer(t, t->last_x, NULL);
         const double orig_lh = orc_likelihood_from_buffer(t, length, NULL);
         if (orig_lh > opt_lh) optx = length;
     }

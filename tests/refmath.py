@@ -19,10 +19,20 @@ def adjacency(u, v, t):
     return adj
 
 
-class Pruner:
-    """Real-basis Felsenstein pruning with expm transition matrices."""
+def invar_by_definition(states, indicators, pi, p_inv):
+    """+I mass of every pattern from its definition: the probability that the
+    column arose from an invariable site, p_inv * sum_x pi_x * prod_taxa
+    [state x is in the taxon's observed set] (a gap or unknown character
+    admits every state, an ambiguity code its set)."""
+    compat = np.prod(indicators[states], axis=0)      # (P, n): x compatible with every taxon
+    return p_inv * (compat @ np.asarray(pi))
 
-    def __init__(self, ntaxa, edges, states, Q, pi, rates, props, invar, freq, indicators):
+
+class Pruner:
+    """Real-basis Felsenstein pruning with expm transition matrices. The +I
+    mass is formed from p_inv and the tip states (invar_by_definition)."""
+
+    def __init__(self, ntaxa, edges, states, Q, pi, rates, props, p_inv, freq, indicators):
         self.N = ntaxa
         self.adj = adjacency(*edges)
         self.states = states
@@ -30,7 +40,7 @@ class Pruner:
         self.pi = pi
         self.rates = np.asarray(rates)
         self.props = np.asarray(props)
-        self.invar = invar
+        self.invar = invar_by_definition(states, indicators, pi, p_inv)
         self.freq = freq
         self.ind = indicators
 
@@ -88,8 +98,10 @@ class Pruner:
         return float(np.dot(self.freq, log_lh)), df, ddf
 
 
-def brute_force_lnl(ntaxa, edges, states, Q, pi, rates, props, invar, freq, indicators):
-    """Sum over every assignment of internal-node states (tiny trees only)."""
+def brute_force_lnl(ntaxa, edges, states, Q, pi, rates, props, p_inv, freq, indicators):
+    """Sum over every assignment of internal-node states (tiny trees only),
+    plus the +I term formed from p_inv and the tip states."""
+    invar = invar_by_definition(states, indicators, pi, p_inv)
     adj = adjacency(*edges)
     nodes = sorted(adj)
     internal = [x for x in nodes if x >= ntaxa]
@@ -124,7 +136,9 @@ def brute_force_lnl(ntaxa, edges, states, Q, pi, rates, props, invar, freq, indi
             lh_c += site
         total += pc * lh_c
     lh = total + invar
-    return float(np.dot(freq, np.log(lh))), np.log(lh)
+    with np.errstate(divide="ignore"):
+        log_lh = np.log(lh)
+    return float(np.dot(freq, log_lh)), log_lh
 
 
 def indicators(seq_type, n, state_unknown):

@@ -1,5 +1,6 @@
-"""Full-size workloads on the GPU, checked through properties that do not need
-the oracle (which would take minutes at these sizes):
+"""Full-size workloads on the GPU, checked through properties of the
+likelihood (the comparisons with the oracle at these sizes are in
+test_full_size_oracle.py):
 
 - pulley principle: for a reversible model the tree lnL is the same at every
   branch (tree/phylotree.cpp computeLikelihoodBranch on any branch);
@@ -17,13 +18,13 @@ import numpy as np
 import pytest
 
 import iqtree2_amd as ia
+from iqtree2_amd import workloads as wl
 
 pytestmark = pytest.mark.gpu
 
 
 def dna_200x100k():
-    return ia.Workload(seq_type=ia.DNA, ntaxa=200, nsite=100_000, ncat=4, mean_len=0.1, alpha=0.5,
-                       p_inv=0.2, gap_frac=0.02, seed=20261202 + 1)
+    return wl.dna_200x100k()
 
 
 def check_tree_invariants(ctx, w, nbranch_checks, pulley_rtol):
@@ -85,8 +86,7 @@ def test_dna_200x100k_invariants_and_sweep(ctx):
 
 
 def test_protein_100x20k_invariants(ctx):
-    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=100, nsite=20_000, ncat=4, mean_len=0.1, alpha=0.8,
-                    seed=20261202 + 2)
+    w = wl.protein_100x20k()
     tree, lnl0 = check_tree_invariants(ctx, w, nbranch_checks=6, pulley_rtol=1e-10)
     tree.close()
     t32 = ia.PhyloTree(ctx, w, precision=ia.PREC_F32)
@@ -120,21 +120,8 @@ def test_codon_serving_requests_deterministic(ctx):
                 inst.close()
 
 
-def partitions_500(seed=20261202 + 3):
-    rng = np.random.default_rng(seed)
-    base = ia.Workload(seq_type=ia.DNA, ntaxa=64, nsite=10, mean_len=0.1, seed=seed)
-    edges = base.edges()
-    parts = []
-    for k in range(500):
-        nsite = int(np.clip(round(rng.lognormal(np.log(800.0), 0.8)), 100, 5000))
-        st = ia.PROTEIN if rng.random() < 0.3 else ia.DNA
-        parts.append(ia.Workload(seq_type=st, ntaxa=64, nsite=nsite, ncat=4, alpha=-1.0,
-                                 seed=seed * 1000 + k, edges=edges))
-    return parts
-
-
 def test_partitions_500_batched_equals_per_partition(ctx):
-    parts = partitions_500()
+    parts = wl.partitions_500()
     plans = []
     for w in parts:
         planner = ia.PhyloTree(None, w)

@@ -164,3 +164,41 @@ def test_planner_invariants():
     far = branches[-1]
     ops, _, _, _ = planner.plan(*far)
     assert len(ops) > 0
+
+
+def test_nni_planner_buffers_and_restore():
+    """getBestNNIForBran's buffer handling in the planner: during an NNI
+    evaluation node1 and node2 live in the two NNI slots (after the
+    internal-node slots), their exchanged topology is planned there, and
+    nni_end restores topology, lengths and the central partials (nothing
+    to recompute afterwards)."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=14, nsite=60, seed=12)
+    pl = ia.PhyloTree(None, w)
+    central = w.ntaxa - 2
+    nnodes = 2 * w.ntaxa - 2
+    inner = pl.inner_branches()
+    assert len(inner) == w.ntaxa - 3
+    nb0 = {v: pl.neighbors(v) for v in range(nnodes)}
+    br = pl.branches()
+    len0 = {b: pl.get_length(*b) for b in br}
+    pl.plan(*br[0])                               # every partial computed once
+    for node1, node2 in inner[:4]:
+        pl.nni_begin(node1, node2)
+        n1 = [x for x in pl.neighbors(node1) if x != node2]
+        n2 = [y for y in pl.neighbors(node2) if y != node1]
+        pl.nni_swap(node1, n1[0], node2, n2[1])
+        assert n2[1] in pl.neighbors(node1) and n1[0] in pl.neighbors(node2)
+        assert pl.get_length(node1, n2[1]) == len0.get((node2, n2[1]), len0.get((n2[1], node2)))
+        ops, node, dad, _ = pl.plan(n1[1], node1)
+        dads = [int(o["dad_slot"]) for o in ops]
+        # node1's partial seen from n1[1] is built in an NNI slot, from
+        # node2's (the other NNI slot) and the swapped-in subtree
+        assert dads[-1] in (central, central + 1)
+        assert set(dads) & {central, central + 1}
+        pl.set_length_keep(node1, n1[1], 0.33)
+        pl.nni_swap(node1, n2[1], node2, n1[0])
+        pl.nni_end()
+        assert {v: pl.neighbors(v) for v in range(nnodes)} == nb0
+        assert {b: pl.get_length(*b) for b in br} == len0
+    # central partials were left intact: no op needed for the first branch
+    assert len(pl.plan(*br[0])[0]) <= 2 * 4

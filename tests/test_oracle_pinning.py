@@ -8,12 +8,18 @@ from scipy.special import gammainc, gammaincinv
 
 import iqtree2_amd as ia
 from oracle.binding import OracleTree
-from refmath import Pruner, brute_force_lnl, indicators
+from refmath import Pruner, brute_force_lnl, indicators, invar_by_definition
+
+
+def p_inv_of(w):
+    return float(w.get("params")[1])
 
 
 def pruner_for(w):
+    """Independent pruning; +I from p_inv and the tip states, not the
+    product's ptn_invar."""
     return Pruner(w.ntaxa, w.edges(), w.get("states"), w.get("Q"), w.get("pi"), w.get("rates"),
-                  w.get("props"), w.get("invar"), w.get("freq"),
+                  w.get("props"), p_inv_of(w), w.get("freq"),
                   indicators(w.seq_type, w.nstates, w.state_unknown))
 
 
@@ -35,7 +41,7 @@ def test_brute_force_small_trees(seq_type, ntaxa, nsite, p_inv, gap):
     # codon models run with per-category scaling, as IQ-TREE forces for them
     orc = OracleTree.from_workload(w, safe_numeric=seq_type == ia.CODON)
     ref, ref_p = brute_force_lnl(w.ntaxa, w.edges(), w.get("states"), w.get("Q"), w.get("pi"),
-                                 w.get("rates"), w.get("props"), w.get("invar"), w.get("freq"),
+                                 w.get("rates"), w.get("props"), p_inv, w.get("freq"),
                                  indicators(seq_type, w.nstates, w.state_unknown))
     for a, b in planner.branches():
         lnl, plh = oracle_branch(w, planner, orc, a, b)
@@ -113,7 +119,7 @@ def test_codon_derivatives_and_odd_state_quirk(ncat):
     orc = OracleTree.from_workload(w, safe_numeric=True)
     pr = pruner_for(w)
     ev, inv = w.get("eval"), w.get("inv_evec")
-    rates, props, freq, invar = w.get("rates"), w.get("props"), w.get("freq"), w.get("invar")
+    rates, props, freq, invar = w.get("rates"), w.get("props"), w.get("freq"), pr.invar
     block_even = (w.nstates * ncat) % 2 == 0
     for a, b in planner.branches()[::3]:
         ops, node, dad, length = planner.plan(a, b)
@@ -228,7 +234,7 @@ def test_planted_mistakes_are_caught():
                     p_inv=0.2, gap_frac=0.05, seed=16)
     planner = ia.PhyloTree(None, w)
     ref, _ = brute_force_lnl(w.ntaxa, w.edges(), w.get("states"), w.get("Q"), w.get("pi"),
-                             w.get("rates"), w.get("props"), w.get("invar"), w.get("freq"),
+                             w.get("rates"), w.get("props"), 0.2, w.get("freq"),
                              indicators(ia.DNA, 4, 18))
     a, b = planner.branches()[0]
     ops, node, dad, length = planner.plan(a, b)

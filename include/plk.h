@@ -161,9 +161,10 @@ typedef struct {
 
 /* PhyloSuperTree::computeLikelihood's partition loop
  * (tree/phylosupertree.cpp:701-735) as one persistent launch: every item's
- * traversal and branch lnL, instances of 4 and 20 states with 4 categories
- * and either precision together, pattern tiles handed out largest-partition
- * first; items of other shapes (61 states, other category counts) run one
+ * traversal and branch lnL, instances of 4, 20 and 61 states with 4
+ * categories and either precision, pattern tiles handed out largest-partition
+ * first (4- and 20-state tiles share one launch, 61-state tiles take a launch
+ * of their own in the same call); items of other category counts run one
  * traversal each in the same call. lnl[i] gets item i's tree lnL, equal to
  * plk_likelihood_branch on that item alone to rounding (1e-12 relative). */
 plk_status plk_likelihood_branch_batch(plk_context* ctx, const plk_batch_item* items, int nitems,
@@ -192,6 +193,7 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
  * valid, and the call returns PLK_ERR_NONFINITE. */
 plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
                                      int n, double* df, double* ddf, double* lnl);
+
 
 /* ---- device-resident branch optimisation ------------------------------- */
 /* A traversal step whose children carry edge ids instead of lengths: the
@@ -279,6 +281,13 @@ typedef struct {
     double bytes;
     double flops;
 } plk_kernel_stat;
+
+/* HBM bandwidth of a plain device copy of `bytes` (read + written per launch,
+ * reps launches after one warm-up), timed with HIP events on the context's
+ * stream: the measured roofline denominator beside the 8 TB/s peak, and the
+ * known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.
+ * Measurement support; no counterpart in the reference. */
+plk_status plk_copy_bandwidth(plk_context* ctx, size_t bytes, int reps, double* gbps, double* avg_ms);
 
 plk_status plk_profile_start(plk_context* ctx);
 plk_status plk_profile_stop(plk_context* ctx, plk_kernel_stat* stats, int cap, int* nstats);

@@ -117,6 +117,16 @@ int plkh_tree_optimize_branch(plkh_tree* t, int a, int b, int max_nr, double* ne
  * lnl = final tree lnL, n_derv = derivative evaluations */
 int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double tolerance, int max_nr,
                                     double* lnl, int* n_derv);
+/* PhyloSuperTree::optimizeAllBranches (tree/phylosupertree.cpp:773-792):
+ * optimizeAllBranches(iterations, tolerance / min(ntrees, 10), max_nr) on every
+ * tree (all on one context), with the host Newton loop of each partition
+ * advanced together: one batched traversal launch (plk_batch_evaluate) and
+ * one batched derivative launch (plk_likelihood_derv_batch) per round for all
+ * partitions. Each tree takes the Newton steps it takes alone. lnl, n_derv and
+ * status (may be NULL) per tree; returns the first failing status. */
+int plkh_supertree_optimize_all_branches(plkh_tree* const* trees, int ntrees, int iterations,
+                                         double tolerance, int max_nr, double* lnl,
+                                         int32_t* n_derv, int32_t* status);
 /* derivative evaluations of each branch in the last optimize_all_branches
  * pass, in plkh_tree_branches order; *count = number of branches (values
  * beyond cap are not written) */

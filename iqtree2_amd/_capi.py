@@ -127,6 +127,7 @@ _PROTOS = {
     "plk_rell": (c_int, [c_void_p, c_int, _F32, c_int, _DP]),
     "plk_likelihood_branch_batch": (c_int, [c_void_p, P(PlkBatchItem), c_int, _DP]),
     "plk_likelihood_derv_batch": (c_int, [c_void_p, _I32, _DP, c_int, _DP, _DP, _DP]),
+    "plk_copy_bandwidth": (c_int, [c_void_p, c_size_t, c_int, _DP, _DP]),
     "plk_batch_evaluate": (c_int, [c_void_p, P(PlkBatchItem), _I32, c_int, _DP]),
     "plk_optimize_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
                                    c_double, c_double, c_int, _I32, _DP]),
@@ -178,6 +179,8 @@ _PROTOS = {
     "plkh_tree_set_branch_bounds": (c_int, [c_void_p, c_double, c_double]),
     "plkh_tree_set_device_newton": (c_int, [c_void_p, c_int]),
     "plkh_tree_pass_counts": (c_int, [c_void_p, _I32, c_int, P(c_int)]),
+    "plkh_supertree_optimize_all_branches": (c_int, [P(c_void_p), c_int, c_int, c_double, c_int, _DP,
+                                                     _I32, _I32]),
     "plkh_tree_neighbors": (c_int, [c_void_p, c_int, _I32, P(c_int)]),
     "plkh_tree_set_length_keep": (c_int, [c_void_p, c_int, c_int, c_double]),
     "plkh_tree_nni_begin": (c_int, [c_void_p, c_int, c_int]),

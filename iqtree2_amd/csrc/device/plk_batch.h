@@ -42,6 +42,16 @@ __device__ inline void prep_child_table(const BatchItem& it, const plk_child ch,
                                         const double* tp_sh, const double* ex, T* out) {
     const int n = it.n, C = it.C, SU = it.I.SU, nn = n * n;
     if (!ch.is_tip) {
+        if (n == kProtN) {
+            // the matrix-core tile's fragment order
+            for (int t = threadIdx.x; t < C * kProtFrag; t += kThreads) {
+                const int c = t / kProtFrag;
+                int x, i;
+                prot_frag_index(t - c * kProtFrag, x, i);
+                out[t] = x < 0 ? (T)0 : (T)(ev_sh[x * n + i] * ex[c * n + i]);
+            }
+            return;
+        }
         for (int t = threadIdx.x; t < C * nn; t += kThreads) {
             const int c = t / nn, r = t - c * nn, x = r / n, i = r - x * n;
             out[t] = (T)(ev_sh[x * n + i] * ex[c * n + i]);
@@ -178,12 +188,13 @@ constexpr int kBatchSub = 4;
 
 // Dynamic LDS bytes batch_tile uses for one item shape: U^-1, the two child
 // tables of a step and the root tables (which reuse the table area), or for
-// 20 states the MFMA layout of mfma_ops_tile (U^-1 and tables with row
-// stride N + 1, kBatchSub x C x 16 maxima slots).
+// 20 states the maxima of prot_ops_tile (2 x kBatchSub x C x 16) and the
+// root tables.
 __host__ __device__ inline size_t batch_smem_bytes(int n, int C, int SU, int table_size, size_t elem) {
     const size_t root = (size_t)(SU + 2) * n * C * 8;
-    if (n == 20) {
-        const size_t mf = ((size_t)n * (n + 1) * (1 + 2 * C) + (size_t)kBatchSub * C * 16) * elem;
+    if (n == kCodonN) return (size_t)2 * n * kCodonStride * elem;   // E | U^-1; root rows stay in HBM
+    if (n == kProtN) {
+        const size_t mf = (size_t)2 * kBatchSub * C * 16 * elem;
         const size_t rt = (size_t)n * n * elem + root;
         return mf > rt ? mf : rt;
     }
@@ -197,8 +208,8 @@ __host__ __device__ inline size_t batch_smem_bytes(int n, int C, int SU, int tab
 constexpr int kDnaPatternsPerThread = 2;
 
 // patterns per work-list tile: 4 states, kDnaPatternsPerThread patterns per
-// C threads (128 with four categories); 20 states, one pattern per C threads
-// (four 16-pattern matrix-core tiles)
+// C threads (128 with four categories); 20 and 61 states, one pattern per C
+// threads (four 16-pattern protein matrix-core tiles, or one codon tile)
 __host__ __device__ constexpr int batch_tile_patterns(int n, int C) {
     return (n == 4 ? kDnaPatternsPerThread : 1) * kThreads / C;
 }
@@ -217,18 +228,23 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
     const int p = tile * PB + (int)threadIdx.x / C;
     const T* tables = reinterpret_cast<const T*>(it.tables);
     __syncthreads();
-    if constexpr (N == 20) {
-        // protein: the matrix-core tile of the lone traversal kernel, four
-        // 16-pattern sub-tiles per batch tile
+    if constexpr (N == kCodonN) {
+        // codon: the lone kernel's matrix-core tile, one pass per rate
+        // category (each category's scaling is its own)
+        constexpr int S = kCodonStride;
+        static_assert(PB == kCodonTile, "batch tile must be one codon tile");
+        T* e_m = reinterpret_cast<T*>(smem);
+        T* inv_m = e_m + N * S;
+        stage_lds<N * N>(inv_m, it.m.inv_evec, N * N, [](int t) { return (t / N) * S + t % N; });
+        __syncthreads();
+        for (int cc = 0; cc < C; cc++)
+            codon_ops_tile<T>(I, it.ops, it.nops, tables, it.table_size, cc, tile * PB, e_m, inv_m);
+    } else if constexpr (N == 20) {
+        // protein: the matrix-core tile of the lone traversal kernel, every
+        // wave one category of four 16-pattern sub-tiles
         static_assert(PB == kBatchSub * kMfmaTile, "batch tile must be kBatchSub MFMA tiles");
-        using L = MfmaSmall<T, N, C>;
-        T* inv_m = reinterpret_cast<T*>(smem);
-        T* tab_m = inv_m + L::kInv;
-        T* red_m = tab_m + 2 * L::kTab;
-        stage_inv<T, N>(it.m.inv_evec, inv_m);
-        // prefetched table staging costs FP64 an occupancy step here
-        mfma_ops_tile<T, N, C, kBatchSub, sizeof(T) == 4>(I, it.ops, it.nops, tables, it.table_size, inv_m,
-                                                         tab_m, red_m, tile * PB);
+        prot_ops_tile<T, C, kBatchSub>(I, it.m.inv_evec, it.ops, it.nops, tables, it.table_size,
+                                       reinterpret_cast<T*>(smem), tile * PB);
     } else {
     for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)it.m.inv_evec[t];
     for (int o = 0; o < it.nops; o++) {
@@ -261,27 +277,33 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
     __syncthreads();
     if (it.mode == BATCH_THETA) {
         // computeLikelihoodBufferSIMD's theta: tip dads read the tip vectors
-        if (dad.is_tip)
-            for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = it.m.tip[t];
-        __syncthreads();
-        if constexpr (N == 20) {
-            branch_end_stream<T, N, C>(I, node, dad, MODE_THETA, root_sh, p, c, it.m.tip);
-        } else {
+        // (the streamed form reads them from HBM)
+        if constexpr (N == 4) {
+            if (dad.is_tip)
+                for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) root_sh[t] = it.m.tip[t];
+            __syncthreads();
 #pragma unroll
             for (int j = 0; j < kPP; j++) theta_apply<T, N, C>(I, node, dad, root_sh, p + j * (kThreads / C), c);
+        } else {
+            branch_end_stream<T, N, C>(I, node, dad, MODE_THETA, root_sh, p, c, it.m.tip);
         }
         return 0.0;
     }
-    const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
-    for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = it.root[t];
-    __syncthreads();
+    if constexpr (N == kCodonN) {
+        // 61-state lookup rows (121 KB for a tip dad) stay in HBM, as k_root_stream reads them
+        root_sh = const_cast<double*>(it.root);
+    } else {
+        const int root_size = block + (dad.is_tip ? (I.SU + 1) * block : 0);
+        for (int t = threadIdx.x; t < root_size; t += kThreads) root_sh[t] = it.root[t];
+        __syncthreads();
+    }
     double contrib = 0.0;
 #pragma unroll
     for (int j = 0; j < kPP; j++) {
         const int pj = p + j * (kThreads / C);
         double lh0;
         int minsc;
-        if constexpr (N == 20) {
+        if constexpr (N != 4) {
             // streamed (no per-thread state arrays), same arithmetic as below
             lh0 = branch_end_stream<T, N, C>(I, node, dad, MODE_BRANCH, root_sh, pj, c, it.m.tip, &minsc);
         } else {
@@ -316,11 +338,19 @@ __device__ __forceinline__ double batch_tile(const BatchItem& it, int tile, unsi
     return v[0];
 }
 
-// Instance classes a batch kernel is built for (bit mask)
-enum BatchClass { BC_DNA64 = 1, BC_PROT64 = 2, BC_DNA32 = 4, BC_PROT32 = 8, BC_ALL = 15 };
+// Instance classes a batch kernel is built for (bit mask). BC_ALL is the
+// fused DNA + protein launch; codon tiles (63 KB of LDS) always run in a
+// launch of their own class, so they do not lower the others' occupancy.
+enum BatchClass {
+    BC_DNA64 = 1, BC_PROT64 = 2, BC_DNA32 = 4, BC_PROT32 = 8, BC_ALL = 15,
+    BC_COD64 = 16, BC_COD32 = 32
+};
+constexpr int kBatchClasses = 6;
 
 __host__ __device__ inline int batch_class(int n, int f32) {
-    return n == 4 ? (f32 ? BC_DNA32 : BC_DNA64) : (f32 ? BC_PROT32 : BC_PROT64);
+    if (n == 4) return f32 ? BC_DNA32 : BC_DNA64;
+    if (n == 20) return f32 ? BC_PROT32 : BC_PROT64;
+    return f32 ? BC_COD32 : BC_COD64;
 }
 
 // Persistent workgroups over tiles [tile_begin, tile_end) of the list. MASK
@@ -349,6 +379,8 @@ k_batch(const BatchItem* __restrict__ items, const int* __restrict__ tile_item, 
         else if ((MASK & BC_PROT64) && cls == BC_PROT64) s = batch_tile<double, 20, 4>(it, tile, smem);
         else if ((MASK & BC_DNA32) && cls == BC_DNA32) s = batch_tile<float, 4, 4>(it, tile, smem);
         else if ((MASK & BC_PROT32) && cls == BC_PROT32) s = batch_tile<float, 20, 4>(it, tile, smem);
+        else if ((MASK & BC_COD64) && cls == BC_COD64) s = batch_tile<double, kCodonN, 4>(it, tile, smem);
+        else if ((MASK & BC_COD32) && cls == BC_COD32) s = batch_tile<float, kCodonN, 4>(it, tile, smem);
         if (threadIdx.x == 0) tile_sums[g] = s;
     }
 }

@@ -75,30 +75,27 @@ constexpr int kCodonN = 61;        // sense codons of the standard code
 constexpr int kCodonStride = 65;   // LDS row stride in elements
 constexpr int kCodonTile = 64;     // patterns per workgroup
 
+// All steps of an op list for the kCodonTile patterns from p0 in rate
+// category c (per-category scaling makes categories independent). inv_sh
+// holds U^-1 (row i, column x, row stride kCodonStride); e_sh receives one
+// internal child's E[c] (row x, column k) during phase A and the dad tile
+// [64 patterns][61] while it is written out. Ends with a barrier.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
-                 const T* __restrict__ tables, int table_size) {
+__device__ inline void codon_ops_tile(const InstDev& I, const plk_op* __restrict__ ops, int nops,
+                                      const T* __restrict__ tables, int table_size, int c, int p0,
+                                      T* e_sh, const T* inv_sh) {
     using ST = typename Prec<T>::scale_t;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     constexpr int N = kCodonN, S = kCodonStride;
-    // one child's E[c] (row x, column k) during phase A; the dad tile
-    // [64 patterns][61] while it is written out
-    __shared__ T e_sh[N * S];
-    __shared__ T inv_sh[N * S];   // U^-1, row i, column x
     const int C = I.C;
     const int block = N * C;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, g = lane >> 4;
-    const int p0 = blockIdx.x * kCodonTile;
     const int pl = wave * 16 + col;   // this lane's pattern within the tile
     const int pw = p0 + pl;
     T* partial = reinterpret_cast<T*>(I.partial);
     ST* scale = reinterpret_cast<ST*>(I.scale);
-
-    stage_lds<N * N>(inv_sh, m.inv_evec, N * N, [](int t) { return (t / N) * S + t % N; });
-    __syncthreads();
 
     for (int o = 0; o < nops; o++) {
         const plk_op op = ops[o];
@@ -110,8 +107,8 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
         const bool tiptip = ch[0].is_tip && ch[1].is_tip;
         T* dst = partial + (size_t)op.dad_slot * I.slot_stride;
         {
-            const int c = blockIdx.y;   // per-category scaling makes categories independent
             acc_t prod[4];   // tmp[x = 16*xb + row(g, r)][pattern pl] in prod[xb][r]
+#pragma unroll
             for (int k = 0; k < 2; k++) {
                 acc_t a[4];
                 if (ch[k].is_tip) {
@@ -219,6 +216,18 @@ k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops
             __syncthreads();
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_traverse_codon(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
+                 const T* __restrict__ tables, int table_size) {
+    constexpr int N = kCodonN, S = kCodonStride;
+    __shared__ T e_sh[N * S];
+    __shared__ T inv_sh[N * S];
+    stage_lds<N * N>(inv_sh, m.inv_evec, N * N, [](int t) { return (t / N) * S + t % N; });
+    __syncthreads();
+    codon_ops_tile<T>(I, ops, nops, tables, table_size, blockIdx.y, blockIdx.x * kCodonTile, e_sh, inv_sh);
 }
 
 // Branch ends for the matrix-core kernels, one thread per (pattern,

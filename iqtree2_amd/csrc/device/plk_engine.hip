@@ -51,6 +51,7 @@ constexpr int kSweepMaxBranches = PLK_SWEEP_MAX_BRANCHES;
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+
 struct Instance {
     plk_instance_desc d{};
     int Ppad = 0;
@@ -58,6 +59,7 @@ struct Instance {
     size_t elem = 8;            // bytes of one stored value
     size_t scale_elem = 2;      // bytes of one scale counter
     int table_size = 0;         // elements per (op, child) table
+    int sweep_table_size = 0;   // elements per child table in the sweep's LDS layout
     size_t bytes = 0;           // HBM held
     // device buffers
     uint8_t* states = nullptr;
@@ -130,11 +132,12 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
                 K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH,
-                K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_NKIND };
+                K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_BATCH_CODON, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
                                           "reduce", "rell", "newton_sweep", "batch", "prep_batch",
-                                          "traverse_mfma", "root", "batch_dna", "batch_protein"};
+                                          "traverse_mfma", "root", "batch_dna", "batch_protein",
+                                          "batch_codon"};
 
 struct ProfRec {
     int kind;
@@ -452,9 +455,9 @@ plk_status launch_traverse_n(plk_context* ctx, Instance* in, int nops, int mode,
 template <typename T>
 plk_status launch_traverse_p(plk_context* ctx, Instance* in, int nops, int mode, plk_end node,
                              plk_end dad) {
+    // 20 and 61 states run on the matrix cores (launch_traverse)
     switch (in->d.nstates) {
     case 4: return launch_traverse_n<T, 4>(ctx, in, nops, mode, node, dad);
-    case 20: return launch_traverse_n<T, 20>(ctx, in, nops, mode, node, dad);
     }
     return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
 }
@@ -827,7 +830,8 @@ plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* d, int
     in->block = (size_t)d->nstates * d->ncat;
     in->elem = d->precision == PLK_PREC_F32 ? 4 : 8;
     in->scale_elem = d->precision == PLK_PREC_F32 ? 4 : 2;
-    in->table_size = (int)std::max(in->block * d->nstates, (size_t)(d->state_unknown + 1) * in->block);
+    in->sweep_table_size = (int)std::max(in->block * d->nstates, (size_t)(d->state_unknown + 1) * in->block);
+    in->table_size = d->nstates == kProtN ? prot_table_size(d->ncat, d->state_unknown) : in->sweep_table_size;
     const int n = d->nstates;
     const size_t ncnt = d->safe_numeric ? d->ncat : 1;
     const size_t model_doubles = n + 2 * (size_t)n * n + (size_t)(d->state_unknown + 1) * n + 2 * d->ncat;
@@ -1130,7 +1134,7 @@ plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need)
 template <typename T, int N, int C>
 plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& grid_out) {
     constexpr int kLdsMax = 64 * 1024;
-    const SweepSmem L0(N, C, in->d.state_unknown, in->table_size, a.nedges, (int)sizeof(T), 0);
+    const SweepSmem L0(N, C, in->d.state_unknown, in->sweep_table_size, a.nedges, (int)sizeof(T), 0);
     if (L0.total > kLdsMax) return fail(PLK_ERR_UNSUPPORTED, "sweep LDS need %d bytes", L0.total);
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, N, C>, kThreads, L0.total));
@@ -1145,7 +1149,7 @@ plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& gri
     int lds_tiles = 0, smem = L0.total;
     if (ctx->sweep_lds) {
         const int tiles = (ntiles + grid - 1) / grid;
-        const SweepSmem L1(N, C, in->d.state_unknown, in->table_size, a.nedges, (int)sizeof(T), tiles);
+        const SweepSmem L1(N, C, in->d.state_unknown, in->sweep_table_size, a.nedges, (int)sizeof(T), tiles);
         int nb1 = 0;
         if (L1.total <= kLdsMax)
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb1, k_sweep<T, N, C>, kThreads, L1.total));
@@ -1207,25 +1211,33 @@ plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid)
 namespace {
 
 // k_derv_batch over tiles [begin, end) whose items all have this shape
+// the device buffers of one derivative batch
+struct DervBufs {
+    DervItem* items;
+    int* tile_item;
+    double* tile_sums;
+    double* out;
+};
+
 template <typename T, int N>
-plk_status launch_derv_batch_n(plk_context* ctx, int C, int begin, int end) {
+plk_status launch_derv_batch_n(plk_context* ctx, const DervBufs& b, int C, int begin, int end) {
     const dim3 grid(end - begin), blk(kThreads);
     switch (C) {
-    case 1: hipLaunchKernelGGL((k_derv_batch<T, N, 1>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
-    case 2: hipLaunchKernelGGL((k_derv_batch<T, N, 2>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
-    case 4: hipLaunchKernelGGL((k_derv_batch<T, N, 4>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
-    case 8: hipLaunchKernelGGL((k_derv_batch<T, N, 8>), grid, blk, 0, ctx->stream, ctx->db_items, ctx->db_tile_item, begin, ctx->db_tile_sums); break;
+    case 1: hipLaunchKernelGGL((k_derv_batch<T, N, 1>), grid, blk, 0, ctx->stream, b.items, b.tile_item, begin, b.tile_sums); break;
+    case 2: hipLaunchKernelGGL((k_derv_batch<T, N, 2>), grid, blk, 0, ctx->stream, b.items, b.tile_item, begin, b.tile_sums); break;
+    case 4: hipLaunchKernelGGL((k_derv_batch<T, N, 4>), grid, blk, 0, ctx->stream, b.items, b.tile_item, begin, b.tile_sums); break;
+    case 8: hipLaunchKernelGGL((k_derv_batch<T, N, 8>), grid, blk, 0, ctx->stream, b.items, b.tile_item, begin, b.tile_sums); break;
     default: return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", C);
     }
     HIP_TRY(hipGetLastError());
     return PLK_OK;
 }
 
-plk_status launch_derv_batch(plk_context* ctx, int f32, int n, int C, int begin, int end) {
+plk_status launch_derv_batch(plk_context* ctx, const DervBufs& b, int f32, int n, int C, int begin, int end) {
     switch (n) {
-    case 4: return f32 ? launch_derv_batch_n<float, 4>(ctx, C, begin, end) : launch_derv_batch_n<double, 4>(ctx, C, begin, end);
-    case 20: return f32 ? launch_derv_batch_n<float, 20>(ctx, C, begin, end) : launch_derv_batch_n<double, 20>(ctx, C, begin, end);
-    case kCodonN: return f32 ? launch_derv_batch_n<float, kCodonN>(ctx, C, begin, end) : launch_derv_batch_n<double, kCodonN>(ctx, C, begin, end);
+    case 4: return f32 ? launch_derv_batch_n<float, 4>(ctx, b, C, begin, end) : launch_derv_batch_n<double, 4>(ctx, b, C, begin, end);
+    case 20: return f32 ? launch_derv_batch_n<float, 20>(ctx, b, C, begin, end) : launch_derv_batch_n<double, 20>(ctx, b, C, begin, end);
+    case kCodonN: return f32 ? launch_derv_batch_n<float, kCodonN>(ctx, b, C, begin, end) : launch_derv_batch_n<double, kCodonN>(ctx, b, C, begin, end);
     }
     return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", n);
 }
@@ -1249,18 +1261,21 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     std::vector<int> cls(nitems);
     for (int i = 0; i < nitems; i++)
         cls[i] = batch_class(ins[i]->d.nstates, ins[i]->d.precision == PLK_PREC_F32);
+    // (codon classes last: the fused DNA + protein launch takes a prefix)
+    auto rank = [](int c) { return (c & BC_ALL) ? c + kBatchClasses * 64 : c; };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        if (cls[a] != cls[b]) return cls[a] > cls[b];
+        if (cls[a] != cls[b]) return rank(cls[a]) > rank(cls[b]);
         return cost[a] > cost[b];
     });
     // one pinned upload: items | ops | tile list | root lengths
     std::vector<BatchItem> hb(nitems);
     std::vector<size_t> op_off(nitems);
-    size_t tab_bytes = 0, root_doubles = 0, smem = 0, smem_prep = 0, nops_total = 0;
+    size_t tab_bytes = 0, root_doubles = 0, smem_prep = 0, nops_total = 0;
     int total_tiles = 0, max_nops = 0;
     // algorithmic bytes and flops of the batch, per instance class (index:
     // the BatchClass bit)
-    double cls_bytes[4] = {0, 0, 0, 0}, cls_flops[4] = {0, 0, 0, 0};
+    double cls_bytes[kBatchClasses] = {}, cls_flops[kBatchClasses] = {};
+    size_t cls_smem[kBatchClasses] = {};   // dynamic LDS of each class's tiles
     for (int r = 0; r < nitems; r++) {
         const int i = order[r];
         const Instance* in = ins[i];
@@ -1286,7 +1301,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         tab_bytes += round_up((size_t)items[i].nops * 2 * in->table_size * in->elem, 256);
         b.root = reinterpret_cast<const double*>(root_doubles);
         root_doubles += (size_t)(in->d.state_unknown + 2) * in->block;
-        smem = std::max(smem, batch_smem_bytes(b.n, b.C, in->d.state_unknown, in->table_size, in->elem));
+        const size_t item_smem = batch_smem_bytes(b.n, b.C, in->d.state_unknown, in->table_size, in->elem);
         smem_prep = std::max(smem_prep, (size_t)(b.n * b.n + in->d.state_unknown * b.n + 2 * in->block) * 8);
         double ob = 0.0, of = 0.0;
         for (int o = 0; o < items[i].nops; o++) {
@@ -1302,6 +1317,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         while (!((batch_class(b.n, b.f32) >> cb) & 1)) cb++;
         cls_bytes[cb] += ob * in->d.npatterns;
         cls_flops[cb] += of * in->d.npatterns;
+        cls_smem[cb] = std::max(cls_smem[cb], item_smem);
     }
     const size_t off_ops = round_up(sizeof(BatchItem) * nitems, 256);
     const size_t off_tiles = off_ops + round_up(sizeof(plk_op) * std::max<size_t>(1, nops_total), 256);
@@ -1358,13 +1374,14 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     auto prof_mask = [&](int pe2, int mask) {
         if (pe2 < 0) return;
         double b2 = 0.0, f2 = 0.0;
-        for (int cb = 0; cb < 4; cb++)
+        for (int cb = 0; cb < kBatchClasses; cb++)
             if ((mask >> cb) & 1) {
                 b2 += cls_bytes[cb];
                 f2 += cls_flops[cb];
             }
         const int kind = mask == BC_DNA64 || mask == BC_DNA32 ? K_BATCH_DNA
-                         : mask == BC_PROT64 || mask == BC_PROT32 ? K_BATCH_PROT : K_BATCH;
+                         : mask == BC_PROT64 || mask == BC_PROT32 ? K_BATCH_PROT
+                         : mask == BC_COD64 || mask == BC_COD32 ? K_BATCH_CODON : K_BATCH;
         prof_end(ctx, pe2, kind, b2, f2);
     };
     auto launch = [&](int split, int bpc) -> plk_status {
@@ -1378,8 +1395,13 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
             case BC_PROT64: fn = (const void*)k_batch<BC_PROT64>; break;
             case BC_DNA32: fn = (const void*)k_batch<BC_DNA32>; break;
             case BC_PROT32: fn = (const void*)k_batch<BC_PROT32>; break;
+            case BC_COD64: fn = (const void*)k_batch<BC_COD64>; break;
+            case BC_COD32: fn = (const void*)k_batch<BC_COD32>; break;
             default: fn = (const void*)k_batch<BC_ALL>; break;
             }
+            size_t smem = 0;
+            for (int cb = 0; cb < kBatchClasses; cb++)
+                if (((mask == BC_ALL ? mix & BC_ALL : mask) >> cb) & 1) smem = std::max(smem, cls_smem[cb]);
             int nb = 0;
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, smem));
             const int grid = std::max(1, std::min(end - begin, ctx->num_cu * std::max(1, std::min(nb, bpc))));
@@ -1389,12 +1411,27 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
             double* a5 = ctx->bt_tile_sums;
             void* args[] = {(void*)&a0, (void*)&a1, (void*)&begin, (void*)&end, (void*)&a4, (void*)&a5};
             HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, smem, ctx->stream));
-            prof_mask(pe2, mask == BC_ALL ? mix : mask);
+            prof_mask(pe2, mask == BC_ALL ? mix & BC_ALL : mask);
             return PLK_OK;
         };
-        if (!split || ranges.size() == 1) {
-            const int mask = ranges.size() == 1 ? ranges[0].first : BC_ALL;
-            return one(mask, 0, total_tiles, 0);
+        if (ranges.size() == 1) return one(ranges[0].first, 0, total_tiles, 0);
+        if (!split) {
+            // DNA and protein tiles fused (they lead the list), codon classes after
+            int begin = 0, k = 0, fused_end = 0;
+            for (auto& rg : ranges)
+                if (rg.first & BC_ALL) fused_end = rg.second;
+            if (fused_end > 0) {
+                plk_status s2 = one(BC_ALL, 0, fused_end, k++);
+                if (s2 != PLK_OK) return s2;
+            }
+            begin = fused_end;
+            for (auto& rg : ranges) {
+                if (rg.first & BC_ALL) continue;
+                plk_status s2 = one(rg.first, begin, rg.second, k++);
+                if (s2 != PLK_OK) return s2;
+                begin = rg.second;
+            }
+            return PLK_OK;
         }
         int begin = 0, k = 0;
         for (auto& rg : ranges) {
@@ -1517,7 +1554,7 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
     bool nonfinite = false;
     for (int i = 0; i < nitems; i++) {
         const Instance* in = ins[i];
-        if (in->d.ncat == 4 && (in->d.nstates == 4 || in->d.nstates == 20)) {
+        if (in->d.ncat == 4 && (in->d.nstates == 4 || in->d.nstates == 20 || in->d.nstates == kCodonN)) {
             fused.push_back(items[i]);
             fused_ins.push_back(ins[i]);
             fused_at.push_back(i);
@@ -1547,8 +1584,14 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
     return PLK_OK;
 }
 
-plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
-                                     int n, double* df, double* ddf, double* lnl) {
+}  // extern "C"
+
+namespace {
+
+// plk_likelihood_derv_batch: items grouped by shape, one launch per shape,
+// buffers from the context's caching allocator
+plk_status derv_batch_run(plk_context* ctx, const int* instances, const double* lengths, int n,
+                          double* df, double* ddf, double* lnl) {
     if (!ctx || n < 0 || (n > 0 && (!instances || !lengths || !df || !ddf)))
         return fail(PLK_ERR_ARG, "invalid derivative batch");
     if (n == 0) return PLK_OK;
@@ -1588,24 +1631,24 @@ plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, con
     if ((st = grow(ctx, nullptr, &ctx->db_tile_item, ctx->db_tile_cap, (size_t)total)) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->db_tile_sums, ctx->db_tile_sums_cap, 3 * (size_t)total)) != PLK_OK) return st;
     if ((st = grow(ctx, nullptr, &ctx->db_out, ctx->db_out_cap, 3 * (size_t)n)) != PLK_OK) return st;
-    HIP_TRY(hipMemcpyAsync(ctx->db_items, hd.data(), sizeof(DervItem) * n, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->db_tile_item, tile_item.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
+    const DervBufs b{ctx->db_items, ctx->db_tile_item, ctx->db_tile_sums, ctx->db_out};
+    HIP_TRY(hipMemcpyAsync(b.items, hd.data(), sizeof(DervItem) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(b.tile_item, tile_item.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
     const int pe = prof_begin(ctx);
     for (int r = 0; r < n;) {
         int e = r;
         while (e < n && shape(order[e]) == shape(order[r])) e++;
         const Instance* in = ins[order[r]];
-        st = launch_derv_batch(ctx, in->d.precision == PLK_PREC_F32, in->d.nstates, in->d.ncat,
+        st = launch_derv_batch(ctx, b, in->d.precision == PLK_PREC_F32, in->d.nstates, in->d.ncat,
                                hd[r].tile_begin, hd[e - 1].tile_begin + hd[e - 1].ntiles);
         if (st != PLK_OK) return st;
         r = e;
     }
     prof_end(ctx, pe, K_DERV, bytes, 0.0);
-    hipLaunchKernelGGL(k_derv_reduce, dim3(n), dim3(kThreads), 0, ctx->stream, ctx->db_items,
-                       ctx->db_tile_sums, ctx->db_out);
+    hipLaunchKernelGGL(k_derv_reduce, dim3(n), dim3(kThreads), 0, ctx->stream, b.items, b.tile_sums, b.out);
     HIP_TRY(hipGetLastError());
     std::vector<double> out(3 * (size_t)n);
-    HIP_TRY(hipMemcpyAsync(out.data(), ctx->db_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out.data(), b.out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     bool nonfinite = false;
     for (int r = 0; r < n; r++) {
@@ -1622,6 +1665,15 @@ plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, con
     }
     if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-derivative in a batch item");
     return PLK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
+                                     int n, double* df, double* ddf, double* lnl) {
+    return derv_batch_run(ctx, instances, lengths, n, df, ddf, lnl);
 }
 
 
@@ -1697,7 +1749,7 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     a.min_len = min_len;
     a.max_len = max_len;
     a.max_nr = max_nr;
-    a.table_size = in->table_size;
+    a.table_size = in->sweep_table_size;
     a.part = nullptr;
     a.release = nullptr;
     a.nderv = in->sw_nderv;
@@ -1787,6 +1839,47 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     if (status == PLK_ERR_NONFINITE) return fail(PLK_ERR_NONFINITE, "non-finite derivative in the Newton sweep");
     if (status == PLK_ERR_DEVICE) return fail(PLK_ERR_DEVICE, "grid reduction in the Newton sweep timed out");
     if (status != 0) return fail((plk_status)status, "Newton sweep failed (status %d)", status);
+    return PLK_OK;
+}
+
+plk_status plk_copy_bandwidth(plk_context* ctx, size_t bytes, int reps, double* gbps, double* avg_ms) {
+    if (!ctx || bytes < 4096 || reps < 1 || !gbps) return fail(PLK_ERR_ARG, "invalid copy measurement");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = bytes / 32;   // 32-byte vectors
+    unsigned char *a = nullptr, *b = nullptr;
+    plk_status st = dmalloc(ctx, nullptr, &a, n * 32);
+    if (st != PLK_OK) return st;
+    if ((st = dmalloc(ctx, nullptr, &b, n * 32)) != PLK_OK) {
+        dfree(ctx, nullptr, a, n * 32);
+        return st;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemsetAsync(a, 0, n * 32, ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const dim3 grid(ctx->num_cu * 16), blk(kThreads);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_copy, grid, blk, 0, ctx->stream, reinterpret_cast<const double4*>(a),
+                           reinterpret_cast<double4*>(b), n);   // warm-up
+        e = hipEventRecord(e0, ctx->stream);
+    }
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        hipLaunchKernelGGL(k_copy, grid, blk, 0, ctx->stream, reinterpret_cast<const double4*>(a),
+                           reinterpret_cast<double4*>(b), n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(ctx, nullptr, a, n * 32);
+    dfree(ctx, nullptr, b, n * 32);
+    if (e != hipSuccess) return fail(PLK_ERR_DEVICE, "copy measurement: %s", hipGetErrorString(e));
+    *gbps = 2.0 * (double)n * 32 * reps / (ms * 1e-3) * 1e-9;
+    if (avg_ms) *avg_ms = ms / reps;
     return PLK_OK;
 }
 

@@ -76,6 +76,38 @@ struct InstDev {
     size_t scale_stride;     // Ppad*(safe?C:1)
 };
 
+// Transition tables of protein (20-state) internal children are stored in
+// the operand order of the matrix-core tile (plk_mfma.h).
+constexpr int kProtN = 20;                 // protein states
+constexpr int kProtKSteps = 5;             // 4-wide K steps over 20 states
+constexpr int kProtFrag = 2 * kProtKSteps * 64;   // fragment elements per category
+
+// Elements of one (op, child) table for 20 states: an internal child in
+// fragment order (C categories), a tip child's lookup [SU+1][C][20].
+__host__ __device__ inline int prot_table_size(int C, int SU) {
+    const int frag = C * kProtFrag, lut = (SU + 1) * C * kProtN;
+    return frag > lut ? frag : lut;
+}
+
+// Fragment order of an internal child's table, per category:
+// frag[xb][st][lane] = E[x = 16 xb + lane % 16][k = 5 (lane / 16) + st],
+// zero for x >= 20. Returns the (x, k) of element t, x = -1 for padding.
+__host__ __device__ inline void prot_frag_index(int t, int& x, int& k) {
+    const int xb = t / (kProtKSteps * 64), st = (t / 64) % kProtKSteps, l = t & 63;
+    x = 16 * xb + (l & 15);
+    k = 5 * (l >> 4) + st;
+    if (x >= kProtN) x = -1;
+}
+
+// A plain device copy of a known size (each launch reads and writes n*32
+// bytes): the HBM bandwidth a streaming kernel reaches, and the reference
+// size that calibrates the hardware byte counters.
+__global__ void __launch_bounds__(kThreads)
+k_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 enum TraverseMode { MODE_PARTIALS = 0, MODE_BRANCH = 1, MODE_THETA = 2 };
 enum BufferMode { BUF_DERV = 0, BUF_LNL = 1 };
 
@@ -101,6 +133,16 @@ k_prep_ops(const plk_op* __restrict__ ops, int nops, ModelDev m, int n, int C, i
     T* out = tables + (size_t)(2 * o + k) * table_size;
     if (!ch.is_tip) {
         __syncthreads();
+        if (n == kProtN) {
+            // the matrix-core tile's fragment order
+            T* oc = out + (size_t)c * kProtFrag;
+            for (int t = threadIdx.x; t < kProtFrag; t += blockDim.x) {
+                int x, i;
+                prot_frag_index(t, x, i);
+                oc[t] = x < 0 ? (T)0 : (T)(m.evec[x * n + i] * ex[i]);
+            }
+            return;
+        }
         T* oc = out + (size_t)c * n * n;
         for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
             const int i = t % n, x = t / n;

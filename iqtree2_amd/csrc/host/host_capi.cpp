@@ -330,6 +330,25 @@ int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double toleran
     return t->t->optimize_all_branches(iterations, tolerance, max_nr, lnl, n_derv);
 }
 
+int plkh_supertree_optimize_all_branches(plkh_tree* const* trees, int ntrees, int iterations,
+                                         double tolerance, int max_nr, double* lnl,
+                                         int32_t* n_derv, int32_t* status) {
+    if (ntrees < 0 || (ntrees > 0 && !trees) || iterations < 1 || max_nr < 1) return PLK_ERR_ARG;
+    std::vector<PhyloTreeGPU*> ts(ntrees);
+    for (int i = 0; i < ntrees; i++) {
+        if (!trees[i]) return PLK_ERR_ARG;
+        ts[i] = trees[i]->t;
+    }
+    std::vector<int> nd(ntrees), st(ntrees);
+    const int r = PhyloTreeGPU::optimize_all_branches_batch(ts, iterations, tolerance, max_nr, lnl,
+                                                            nd.data(), st.data());
+    for (int i = 0; i < ntrees; i++) {
+        if (n_derv) n_derv[i] = nd[i];
+        if (status) status[i] = st[i];
+    }
+    return r;
+}
+
 int plkh_tree_pass_counts(const plkh_tree* t, int32_t* n_derv, int cap, int* count) {
     if (!t || !count || cap < 0 || (cap > 0 && !n_derv)) return PLK_ERR_ARG;
     const std::vector<int>& v = t->t->pass_counts();

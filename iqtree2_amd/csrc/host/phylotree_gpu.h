@@ -39,6 +39,13 @@ public:
                     std::vector<plk_sweep_op>& sops, std::vector<plk_sweep_branch>& branches);
     int optimize_all_branches_host(int iterations, double tolerance, int max_nr, double* lnl,
                                    int* n_derv);
+    // PhyloSuperTree::optimizeAllBranches (tree/phylosupertree.cpp:773-792):
+    // optimize_all_branches_host(iterations, tolerance / min(n, 10), max_nr)
+    // on every tree of one context, all advanced together (supertree.cpp);
+    // lnl, n_derv and status per tree; returns the first failing status
+    static int optimize_all_branches_batch(const std::vector<PhyloTreeGPU*>& trees, int iterations,
+                                           double tolerance, int max_nr, double* lnl, int* n_derv,
+                                           int* status);
     bool device_newton = true;
     void set_length(int a, int b, double len);
     // the length alone, no partial invalidated (optimizeOneBranch with clearLH = false)

@@ -368,6 +368,23 @@ def likelihood_derv_batch(ctx, pairs):
     return df, ddf, lnl
 
 
+def supertree_optimize_all_branches(trees, iterations=1, tolerance=0.001, max_nr=100):
+    """PhyloSuperTree::optimizeAllBranches (tree/phylosupertree.cpp:773-792):
+    every tree's optimizeAllBranches(iterations, tolerance / min(n, 10),
+    max_nr) with the host Newton loops advanced together (one batched
+    traversal and one batched derivative launch per round). Returns the
+    per-tree lnL and derivative-evaluation counts."""
+    n = len(trees)
+    hs = (ctypes.c_void_p * max(n, 1))(*[t._h for t in trees])
+    lnl = np.zeros(n)
+    nd = np.zeros(n, np.int32)
+    st = np.zeros(n, np.int32)
+    check(lib().plkh_supertree_optimize_all_branches(hs, n, iterations, tolerance, max_nr, dptr(lnl),
+                                                     i32ptr(nd), i32ptr(st)),
+          "plkh_supertree_optimize_all_branches")
+    return lnl, nd
+
+
 class PhyloTree:
     """PhyloTree-shaped C++ driver (traversal planning + Newton branch optimisation)."""
 

@@ -135,10 +135,15 @@ def test_nni_driver_host_and_device_vs_oracle(ctx, seq_type):
         assert len(inner) == w.ntaxa - 3
         host = [tree.nni_evaluate(a, b, NNI_MAX_NR_STEP) for a, b in inner]
         dev_lnl, dev_lens, dev_nd = tree.nni_evaluate_all(NNI_MAX_NR_STEP)
-        # the tree is unchanged afterwards: its lnL equals a fresh evaluation
+        # the tree is unchanged afterwards: its lnL equals a fresh evaluation.
+        # Partials the sweep computed away from the two NNI nodes stay cached;
+        # the sweep's per-thread arithmetic and the protein traversal's
+        # matrix-core tile round them differently, so the two agree to
+        # rounding, not bit for bit.
         lnl_after = tree.compute_likelihood()
         tree.clear_all_partial()
-        assert tree.compute_likelihood() == lnl_after
+        lnl_fresh = tree.compute_likelihood()
+        assert abs(lnl_fresh - lnl_after) <= 1e-12 * abs(lnl_fresh)
     finally:
         tree.close()
     orc = OracleTree.from_workload(w)

@@ -194,6 +194,21 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
 plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
                                      int n, double* df, double* ddf, double* lnl);
 
+/* plk_likelihood_derv_batch without any device allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The batch's items, tile list, tile sums and results live in a
+ * module-scope device array of 4 MiB per GPU (held until the process ends)
+ * instead of the context's allocator; the call makes no hipMalloc, hipFree or
+ * hipHostMalloc. Results are bit-identical to plk_likelihood_derv_batch. A
+ * batch whose scratch would exceed 4 MiB (28 bytes per 256/C-pattern tile
+ * of every item, plus about 200 bytes per item) is rejected with
+ * PLK_ERR_NOMEM before anything runs. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (from different contexts or threads)
+ * race; calls on one context are serialised by the context. */
+plk_status plk_likelihood_derv_batch_noalloc(plk_context* ctx, const int* instances,
+                                             const double* lengths, int n, double* df, double* ddf,
+                                             double* lnl);
 
 /* ---- device-resident branch optimisation ------------------------------- */
 /* A traversal step whose children carry edge ids instead of lengths: the

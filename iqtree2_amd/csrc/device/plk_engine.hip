@@ -51,6 +51,33 @@ constexpr int kSweepMaxBranches = PLK_SWEEP_MAX_BRANCHES;
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Scratch of plk_likelihood_derv_batch_noalloc (items, tile list, tile sums
+// and results of one call): a module-scope device array, so that entry point
+// allocates no device memory. Each GPU the library runs on holds its own copy
+// until the process ends; in host code the array's name is not a device
+// address, so host code only ever uses the per-device address that
+// hipGetSymbolAddress returns (scratch_address).
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// this GPU's scratch address, looked up once per device with the device current
+hipError_t scratch_address(int dev, unsigned char** out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 struct Instance {
     plk_instance_desc d{};
@@ -187,6 +214,8 @@ struct plk_context {
     size_t db_tile_sums_cap = 0;
     double* db_out = nullptr;
     size_t db_out_cap = 0;
+    // device address of this GPU's copy of the no-allocation scratch array
+    unsigned char* scratch = nullptr;
     int batch_blocks_per_cu = 0;    // 0: autotuned
     // caching device allocator: live blocks by address, free blocks by size
     // class; `used` counts live bytes, `cached` the free blocks kept
@@ -741,6 +770,7 @@ plk_status plk_create(const plk_config* cfg, plk_context** out) {
         if (q == hipSuccess) q = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev);
         if (q == hipSuccess)
             q = hipDeviceGetAttribute(&ctx->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+        if (q == hipSuccess) q = scratch_address(dev, &ctx->scratch);
         if (q != hipSuccess) {
             if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
             delete ctx;
@@ -1588,10 +1618,11 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
 
 namespace {
 
-// plk_likelihood_derv_batch: items grouped by shape, one launch per shape,
-// buffers from the context's caching allocator
+// plk_likelihood_derv_batch and its no-allocation form: the same items,
+// kernels and reduction; the buffers come from the context's caching
+// allocator, or from this GPU's scratch array (noalloc).
 plk_status derv_batch_run(plk_context* ctx, const int* instances, const double* lengths, int n,
-                          double* df, double* ddf, double* lnl) {
+                          double* df, double* ddf, double* lnl, bool noalloc) {
     if (!ctx || n < 0 || (n > 0 && (!instances || !lengths || !df || !ddf)))
         return fail(PLK_ERR_ARG, "invalid derivative batch");
     if (n == 0) return PLK_OK;
@@ -1627,11 +1658,34 @@ plk_status derv_batch_run(plk_context* ctx, const int* instances, const double* 
         bytes += ((double)in->block * in->elem + 24.0) * in->d.npatterns;
     }
     const int total = (int)tile_item.size();
-    if ((st = grow(ctx, nullptr, &ctx->db_items, ctx->db_items_cap, (size_t)n)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->db_tile_item, ctx->db_tile_cap, (size_t)total)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->db_tile_sums, ctx->db_tile_sums_cap, 3 * (size_t)total)) != PLK_OK) return st;
-    if ((st = grow(ctx, nullptr, &ctx->db_out, ctx->db_out_cap, 3 * (size_t)n)) != PLK_OK) return st;
-    const DervBufs b{ctx->db_items, ctx->db_tile_item, ctx->db_tile_sums, ctx->db_out};
+    DervBufs b{};
+    if (noalloc) {
+        // items | tile list | tile sums | results, 256-byte aligned, in the
+        // scratch array; every tile writes its sums before the reduction
+        // reads them, so nothing needs clearing
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            const size_t at = off;
+            off = round_up(off + bytes, 256);
+            return at;
+        };
+        const size_t o_items = carve(sizeof(DervItem) * n), o_tiles = carve(sizeof(int) * (size_t)total);
+        const size_t o_sums = carve(sizeof(double) * 3 * (size_t)total), o_out = carve(sizeof(double) * 3 * (size_t)n);
+        if (off > kNoAllocScratchBytes)
+            return fail(PLK_ERR_NOMEM, "derivative batch needs %zu bytes of scratch; the no-allocation form holds %zu",
+                        off, kNoAllocScratchBytes);
+        if (!ctx->scratch) return fail(PLK_ERR_DEVICE, "no scratch address for device %d", ctx->device);
+        b.items = reinterpret_cast<DervItem*>(ctx->scratch + o_items);
+        b.tile_item = reinterpret_cast<int*>(ctx->scratch + o_tiles);
+        b.tile_sums = reinterpret_cast<double*>(ctx->scratch + o_sums);
+        b.out = reinterpret_cast<double*>(ctx->scratch + o_out);
+    } else {
+        if ((st = grow(ctx, nullptr, &ctx->db_items, ctx->db_items_cap, (size_t)n)) != PLK_OK) return st;
+        if ((st = grow(ctx, nullptr, &ctx->db_tile_item, ctx->db_tile_cap, (size_t)total)) != PLK_OK) return st;
+        if ((st = grow(ctx, nullptr, &ctx->db_tile_sums, ctx->db_tile_sums_cap, 3 * (size_t)total)) != PLK_OK) return st;
+        if ((st = grow(ctx, nullptr, &ctx->db_out, ctx->db_out_cap, 3 * (size_t)n)) != PLK_OK) return st;
+        b = DervBufs{ctx->db_items, ctx->db_tile_item, ctx->db_tile_sums, ctx->db_out};
+    }
     HIP_TRY(hipMemcpyAsync(b.items, hd.data(), sizeof(DervItem) * n, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(b.tile_item, tile_item.data(), sizeof(int) * total, hipMemcpyHostToDevice, ctx->stream));
     const int pe = prof_begin(ctx);
@@ -1673,7 +1727,12 @@ extern "C" {
 
 plk_status plk_likelihood_derv_batch(plk_context* ctx, const int* instances, const double* lengths,
                                      int n, double* df, double* ddf, double* lnl) {
-    return derv_batch_run(ctx, instances, lengths, n, df, ddf, lnl);
+    return derv_batch_run(ctx, instances, lengths, n, df, ddf, lnl, false);
+}
+
+plk_status plk_likelihood_derv_batch_noalloc(plk_context* ctx, const int* instances, const double* lengths,
+                                             int n, double* df, double* ddf, double* lnl) {
+    return derv_batch_run(ctx, instances, lengths, n, df, ddf, lnl, true);
 }
 
 

@@ -92,6 +92,14 @@ def roofline_of(prof, names, traffic_key):
 
 
 # ---------------------------------------------------------------------------
+# measured HBM bandwidth: a plain device copy of a known size
+def leg_copy(ctx, nbytes=256 << 20, reps=10):
+    gbps, ms = ctx.copy_bandwidth(nbytes, reps)
+    return {"bytes_read_and_written_per_launch": 2 * nbytes, "launches": reps, "avg_ms": ms,
+            "gbps": gbps, "frac_of_peak": gbps * 1e9 / HBM_PEAK}
+
+
+# ---------------------------------------------------------------------------
 # headline: 500 partitions
 def partition_jobs(ia, ctx, parts, prec):
     insts, jobs = [], []
@@ -479,7 +487,7 @@ def main():
     ap.add_argument("--no-dna", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
-    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving"),
+    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy"),
                     help="run one leg alone (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -500,10 +508,14 @@ def main():
                 out = leg_dna(ia, ctx, wl.dna_200x100k(), args)
             elif args.only == "mfma":
                 out = leg_mfma(ia, ctx)
+            elif args.only == "copy":
+                out = leg_copy(ctx)
             else:
                 out = leg_serving(ia, ctx, args.requests)
             print(json.dumps({args.only: out}), flush=True)
             return
+        copy = leg_copy(ctx)
+        log(f"copy: {json.dumps(copy)}")
         t0 = time.perf_counter()
         parts = wl.partitions_500(args.partitions)
         log(f"partitions: {len(parts)}, {sum(w.npatterns for w in parts)} patterns "
@@ -567,6 +579,7 @@ def main():
                    "sites": part["sites"], "unit_note": "one PLE = one unique pattern under one "
                    "whole tree (post-order over 62 internal nodes + root lnL)"},
         "roofline": f64["roofline"],
+        "hbm_copy_measured": copy,
         "cpu_baseline": cpu,
         "targets": targets,
         "partition_batch": part,

@@ -175,6 +175,14 @@ class Context:
     def synchronize(self):
         check(lib().plk_synchronize(self._h), "plk_synchronize")
 
+    def copy_bandwidth(self, nbytes=256 << 20, reps=10):
+        """HBM bandwidth of a plain device copy of nbytes (read + written per
+        launch): (GB/s, average ms per launch)."""
+        g, ms = ctypes.c_double(), ctypes.c_double()
+        check(lib().plk_copy_bandwidth(self._h, int(nbytes), int(reps), ctypes.byref(g),
+                                       ctypes.byref(ms)), "plk_copy_bandwidth")
+        return g.value, ms.value
+
     def profile_start(self):
         check(lib().plk_profile_start(self._h), "plk_profile_start")
 

@@ -113,6 +113,18 @@ plk_status plk_set_model(plk_context* ctx, int instance, const double* eval,
                          const double* evec, const double* inv_evec, const double* tip_eigen,
                          const double* cat_rates, const double* cat_props);
 
+/* Lewis's ascertainment bias correction, ASC_VARIANT (+ASC; model/modelfactory.cpp
+ * getASC, tree/phylokernelnew.h:2580-2628, 3175-3221, 3385-3425): the last nasc of
+ * the instance's npatterns patterns are the unobserved constant patterns
+ * (Alignment::getUnobservedConstPatterns, alignment/alignment.cpp:4140-4157),
+ * passed with weight 0 and +I mass 0. Every lnL, pattern lnL and derivative is
+ * then conditioned on variable sites: lnL - nsite * log(1 - P(const)), with
+ * nsite the weight of the observed patterns. nasc = 0 turns it off. The batched
+ * derivative call and the device sweep do not apply it (PLK_ERR_UNSUPPORTED);
+ * batched traversals evaluate such items alone. pattern_lh outputs hold the
+ * conditioned values of the observed patterns, then nasc zeros. */
+plk_status plk_set_asc(plk_context* ctx, int instance, int nasc);
+
 /* ---- kernels ---------------------------------------------------------- */
 /* computePartialLikelihood over a post-order traversal list
  * (tree/phylokernelnew.h:1289-2039, with computePartialInfo 895-1110) */

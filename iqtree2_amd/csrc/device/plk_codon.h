@@ -292,10 +292,13 @@ k_root_stream(InstDev I, ModelDev m, int mode, plk_end node, plk_end dad,
     double contrib = 0.0;
     if (c == 0) {
         const double lh = fabs(lh0) + I.invar[p];
-        const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
-        if (p < I.P) {
+        const int pobs = I.P - I.nasc;
+        if (p < pobs) {
+            const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
             I.ptn_lh[p] = lnp;
             contrib = lnp * I.freq[p];
+        } else if (p < I.P) {
+            I.asc[4 * (p - pobs)] = Prec<T>::asc_scale(lh, minsc);   // +ASC
         }
     }
     double v[1] = {contrib};

@@ -122,10 +122,19 @@ struct Instance {
     void* tables = nullptr;
     bool patterns_set = false, model_set = false, theta_ready = false, ptn_lh_ready = false;
     double theta_length = 0.0;
+    // +ASC (Lewis): the last nasc patterns are the unobserved constant ones
+    int nasc = 0;
+    std::vector<double> hfreq;   // pattern weights (host copy)
+    double nsite = 0.0;          // sites of the observed patterns
     double staged_bytes = 0.0, staged_flops = 0.0;   // per-pattern cost of the staged ops
 };
 
 size_t alloc_class(size_t bytes);
+
+// red_out: 8 reduced scalars, then four values per unobserved constant
+// pattern of +ASC (at most one per state, 61 for codons)
+constexpr int kAscMax = 64;
+constexpr int kRedOutDoubles = 8 + 4 * kAscMax;
 
 // device bytes of an instance, as allocated (size classes included), with
 // the op tables of a full traversal
@@ -145,7 +154,7 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
         (size_t)Ppad * block * elem,                              // theta
         (size_t)Ppad * 8, (size_t)Ppad * 8,                       // buffer_scale, ptn_lh
         (size_t)(Ppad / (kThreads / 8) + 1) * 3 * 8,              // block sums
-        64,                                                       // reduced scalars
+        kRedOutDoubles * 8,                                       // reduced scalars, +ASC values
         (SU + 2) * block * 8,                                     // root tables
         sizeof(plk_op) * (size_t)d->nslots,                       // ops of a full traversal
         (size_t)d->nslots * 2 * table * elem,                     // and their tables
@@ -383,6 +392,8 @@ InstDev inst_dev(const Instance* in) {
     I.ptn_lh = in->ptn_lh;
     I.slot_stride = (size_t)in->Ppad * in->block;
     I.scale_stride = (size_t)in->Ppad * (in->d.safe_numeric ? in->d.ncat : 1);
+    I.nasc = in->nasc;
+    I.asc = in->red_out + 8;
     return I;
 }
 
@@ -695,7 +706,7 @@ plk_status fetch_scalars(plk_context* ctx, Instance* in, int count) {
 
 // per-pattern fix-up of a non-finite tree lnL (tree/phylokernelnew.h:3115-3132)
 plk_status fixup_nonfinite(plk_context* ctx, Instance* in, double* lnl, double* pattern_lh) {
-    const int P = in->d.npatterns;
+    const int P = in->d.npatterns - in->nasc;   // observed patterns
     std::vector<double> plh(P), freq(P);
     HIP_TRY(hipMemcpyAsync(plh.data(), in->ptn_lh, sizeof(double) * P, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(freq.data(), in->freq, sizeof(double) * P, hipMemcpyDeviceToHost, ctx->stream));
@@ -709,6 +720,40 @@ plk_status fixup_nonfinite(plk_context* ctx, Instance* in, double* lnl, double* 
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (pattern_lh) std::memcpy(pattern_lh, plh.data(), sizeof(double) * P);
     *lnl = s;
+    return PLK_OK;
+}
+
+// +ASC: the four per-pattern values of the unobserved constant patterns,
+// each summed in pattern order
+plk_status asc_sums(plk_context* ctx, Instance* in, double sums[4]) {
+    double h[4 * kAscMax];
+    HIP_TRY(hipMemcpyAsync(h, in->red_out + 8, sizeof(double) * 4 * in->nasc, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int j = 0; j < 4; j++) sums[j] = 0.0;
+    for (int k = 0; k < in->nasc; k++)
+        for (int j = 0; j < 4; j++) sums[j] += h[4 * k + j];
+    return PLK_OK;
+}
+
+// Lewis's correction of a tree lnL and its pattern lnL (ASC_VARIANT,
+// tree/phylokernelnew.h:3198-3221, 3405-3424): lnL -= nsite * log(1 - P(const)),
+// the device pattern lnL shifted the same way (the unobserved patterns' own
+// entries zeroed), so RELL and later reads see the conditioned values
+plk_status asc_correct_lnl(plk_context* ctx, Instance* in, double prob_const, double* lnl, double* pattern_lh) {
+    if (!(prob_const >= 0.0 && prob_const < 1.0))
+        return fail(PLK_ERR_NONFINITE, "probability of the unobserved constant patterns %g outside [0, 1)",
+                    prob_const);
+    const double corr = std::log(1.0 - prob_const);
+    *lnl -= in->nsite * corr;
+    const int P = in->d.npatterns, pobs = P - in->nasc;
+    hipLaunchKernelGGL(k_asc_shift, dim3((P + kThreads - 1) / kThreads), dim3(kThreads), 0, ctx->stream,
+                       in->ptn_lh, pobs, P, corr);
+    HIP_TRY(hipGetLastError());
+    if (pattern_lh) {
+        HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * P, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PLK_OK;
 }
 
@@ -880,7 +925,7 @@ plk_status plk_instance_create(plk_context* ctx, const plk_instance_desc* d, int
     A(&in->buffer_scale, (size_t)Ppad * 8);
     A(&in->ptn_lh, (size_t)Ppad * 8);
     A(&in->block_sums, (size_t)max_grid * 3 * 8);
-    A(&in->red_out, 8 * 8);
+    A(&in->red_out, kRedOutDoubles * 8);
     A(&in->root_tab, (size_t)(d->state_unknown + 2) * in->block * 8);
     if (st == PLK_OK) {
         // padding patterns: unknown tips, zero weight
@@ -944,8 +989,24 @@ plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, con
     HIP_TRY(hipMemcpyAsync(in->invar, hv, sizeof(double) * Ppad, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->h_stage_inflight = false;
+    in->hfreq.assign(freq, freq + P);
+    in->nsite = 0.0;
+    for (int p = 0; p < P - in->nasc; p++) in->nsite += freq[p];
     in->patterns_set = true;
     in->theta_ready = false;
+    return PLK_OK;
+}
+
+plk_status plk_set_asc(plk_context* ctx, int id, int nasc) {
+    Instance* in = get_instance(ctx, id);
+    if (!in) return fail(PLK_ERR_ARG, "invalid instance");
+    if (nasc < 0 || nasc > std::min(kAscMax, in->d.nstates) || nasc >= in->d.npatterns)
+        return fail(PLK_ERR_ARG, "%d unobserved constant patterns for %d states", nasc, in->d.nstates);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    in->nasc = nasc;
+    in->nsite = 0.0;
+    for (int p = 0; p < (int)in->hfreq.size() - nasc; p++) in->nsite += in->hfreq[p];
+    in->ptn_lh_ready = false;
     return PLK_OK;
 }
 
@@ -1018,12 +1079,15 @@ plk_status plk_likelihood_branch(plk_context* ctx, int id, const plk_op* ops, in
     if ((st = fetch_scalars(ctx, in, 1)) != PLK_OK) return st;
     *lnl = ctx->h_out[0];
     in->ptn_lh_ready = true;
-    if (!std::isfinite(*lnl)) {
-        st = fixup_nonfinite(ctx, in, lnl, pattern_lh);
-        if (st != PLK_OK) return st;
-        return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch; per-pattern fix-up applied");
+    const bool nonfinite = !std::isfinite(*lnl);
+    if (nonfinite && (st = fixup_nonfinite(ctx, in, lnl, pattern_lh)) != PLK_OK) return st;
+    if (in->nasc) {
+        double a[4];
+        if ((st = asc_sums(ctx, in, a)) != PLK_OK) return st;
+        if ((st = asc_correct_lnl(ctx, in, a[0], lnl, pattern_lh)) != PLK_OK) return st;
     }
-    if (pattern_lh) {
+    if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch; per-pattern fix-up applied");
+    if (pattern_lh && !in->nasc) {
         HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * in->d.npatterns,
                                hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1063,6 +1127,18 @@ plk_status plk_likelihood_derv(plk_context* ctx, int id, double length, double* 
     *df = ctx->h_out[0];
     *ddf = ctx->h_out[1];
     if (lnl) *lnl = ctx->h_out[2];
+    if (in->nasc) {
+        // ASC_Lewis (tree/phylokernelnew.h:2620-2627): the derivative sums count
+        // the unobserved patterns' lh as the derivative kernel does, the lnL
+        // beside them the plain lh
+        double a[4];
+        if ((st = asc_sums(ctx, in, a)) != PLK_OK) return st;
+        const double pv = 1.0 - a[1];
+        const double df_frac = a[2] / pv, ddf_frac = a[3] / pv;
+        *df += in->nsite * df_frac;
+        *ddf += in->nsite * (ddf_frac + df_frac * df_frac);
+        if (lnl) *lnl -= in->nsite * std::log(1.0 - a[0]);
+    }
     if (!std::isfinite(*df)) {
         *df = *ddf = 0.0;
         return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-derivative");
@@ -1083,14 +1159,19 @@ plk_status plk_likelihood_from_buffer(plk_context* ctx, int id, double length, d
     if ((st = fetch_scalars(ctx, in, 3)) != PLK_OK) return st;
     *lnl = ctx->h_out[2];
     in->ptn_lh_ready = true;
-    if (!std::isfinite(*lnl)) {
-        st = fixup_nonfinite(ctx, in, lnl, pattern_lh);
-        if (st != PLK_OK) return st;
+    const bool nonfinite = !std::isfinite(*lnl);
+    if (nonfinite && (st = fixup_nonfinite(ctx, in, lnl, pattern_lh)) != PLK_OK) return st;
+    if (in->nasc) {
+        double a[4];
+        if ((st = asc_sums(ctx, in, a)) != PLK_OK) return st;
+        if ((st = asc_correct_lnl(ctx, in, a[0], lnl, pattern_lh)) != PLK_OK) return st;
+    }
+    if (nonfinite) {
         if (!in->d.safe_numeric)
             return fail(PLK_ERR_NONFINITE, "numerical underflow (lh-from-buffer); use the safe kernel");
         return PLK_OK;
     }
-    if (pattern_lh) {
+    if (pattern_lh && !in->nasc) {
         HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * in->d.npatterns,
                                hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1584,7 +1665,8 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
     bool nonfinite = false;
     for (int i = 0; i < nitems; i++) {
         const Instance* in = ins[i];
-        if (in->d.ncat == 4 && (in->d.nstates == 4 || in->d.nstates == 20 || in->d.nstates == kCodonN)) {
+        if (in->d.ncat == 4 && in->nasc == 0 &&
+            (in->d.nstates == 4 || in->d.nstates == 20 || in->d.nstates == kCodonN)) {
             fused.push_back(items[i]);
             fused_ins.push_back(ins[i]);
             fused_at.push_back(i);
@@ -1632,6 +1714,7 @@ plk_status derv_batch_run(plk_context* ctx, const int* instances, const double* 
         ins[i] = get_instance(ctx, instances[i]);
         if ((st = ready(ins[i])) != PLK_OK) return st;
         if (!ins[i]->theta_ready) return fail(PLK_ERR_STATE, "item %d: theta buffer not computed", i);
+        if (ins[i]->nasc) return fail(PLK_ERR_UNSUPPORTED, "item %d: +ASC instances take plk_likelihood_derv", i);
         if (!std::isfinite(lengths[i])) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1752,6 +1835,7 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     Instance* in = get_instance(ctx, id);
     plk_status st = ready(in);
     if (st != PLK_OK) return st;
+    if (in->nasc) return fail(PLK_ERR_UNSUPPORTED, "the device sweep does not apply +ASC");
     if (nops < 0 || (nops > 0 && !ops) || nbranches < 1 || !branches || !lengths || nedges < 1 ||
         max_nr < 1 || !(min_len > 0.0) || !(max_len > min_len))
         return fail(PLK_ERR_ARG, "invalid sweep arguments");

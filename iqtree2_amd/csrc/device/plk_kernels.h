@@ -41,6 +41,10 @@ template <> struct Prec<double> {
         return d == 0 ? 1.0 : (d == 1 ? kScaleThreshold64 : 0.0);
     }
     static constexpr double log_unit = kLogScaleThreshold;   // ln(2^-256)
+    // an unobserved constant pattern (+ASC): a rescaled one counts once at the
+    // scaling threshold, whatever its counter (tree/phylokernelnew.h:2940-2947)
+    __device__ static double asc_scale(double v, int minsc) { return minsc != 0 ? v * kScaleThreshold64 : v; }
+    __device__ static double asc_scale_log(double v, double bs) { return bs != 0.0 ? v * kScaleThreshold64 : v; }
 };
 template <> struct Prec<float> {
     using scale_t = int32_t;
@@ -51,6 +55,9 @@ template <> struct Prec<float> {
     // counters are exponents, so categories differ by an exact power of two
     __device__ static double safe_adjust(int d) { return ldexp(1.0, -d); }
     static constexpr double log_unit = -kLn2;
+    // counters are exponents: the exact scale back
+    __device__ static double asc_scale(double v, int minsc) { return ldexp(v, -minsc); }
+    __device__ static double asc_scale_log(double v, double bs) { return v * exp(bs); }
 };
 
 struct ModelDev {
@@ -74,6 +81,11 @@ struct InstDev {
     double* ptn_lh;          // [Ppad]
     size_t slot_stride;      // Ppad*block
     size_t scale_stride;     // Ppad*(safe?C:1)
+    // +ASC (Lewis): the last nasc of the P patterns are the unobserved
+    // constant patterns; pattern k's lh goes to asc[4k], and a derivative
+    // evaluation adds lh as the derivative sum counts it, d1 and d2 at 4k+1..3
+    int nasc;
+    double* asc;
 };
 
 // Transition tables of protein (20-state) internal children are stored in
@@ -587,10 +599,13 @@ k_traverse(InstDev I, ModelDev m, const plk_op* __restrict__ ops, int nops,
             const double lh0 = I.safe ? group_sum_sumvec<C>(lhc * adjust) : group_sum_ordered<C>(lhc);
             if (c == 0) {
                 const double lh = fabs(lh0) + I.invar[p];
-                const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
-                if (p < I.P) {
+                const int pobs = I.P - I.nasc;
+                if (p < pobs) {
+                    const double lnp = log(lh) + (double)minsc * Prec<T>::log_unit;
                     I.ptn_lh[p] = lnp;
                     contrib = lnp * I.freq[p];
+                } else if (p < I.P) {
+                    I.asc[4 * (p - pobs)] = Prec<T>::asc_scale(lh, minsc);
                 }
             }
         }
@@ -663,7 +678,20 @@ __device__ inline void buffer_block(const InstDev& I, const ModelDev& m, double 
         lhq = kTwiceLast ? group_sum_ordered<C>(lhq) : lh;
     }
     double acc[3] = {0.0, 0.0, 0.0};
-    if (c == 0 && p < I.P) {
+    const int pobs = I.P - I.nasc;
+    if (c == 0 && p >= pobs && p < I.P) {
+        // an unobserved constant pattern (+ASC): its probability and
+        // derivatives, no weight of its own (tree/phylokernelnew.h:2497-2530, 3340-3358)
+        const double bs = I.buffer_scale[p];
+        double* a = I.asc + 4 * (p - pobs);
+        a[0] = Prec<T>::asc_scale_log(fabs(lh) + I.invar[p], bs);
+        if (mode == BUF_DERV) {
+            a[1] = Prec<T>::asc_scale_log(fabs(lhq) + I.invar[p], bs);
+            a[2] = Prec<T>::asc_scale_log(d1, bs);
+            a[3] = Prec<T>::asc_scale_log(d2, bs);
+        }
+    }
+    if (c == 0 && p < pobs) {
         lh = fabs(lh) + I.invar[p];
         const double f = I.freq[p];
         const double lnp = log(lh) + I.buffer_scale[p];
@@ -693,6 +721,15 @@ k_buffer(InstDev I, ModelDev m, double length, int mode, double* __restrict__ bl
     __shared__ double val[3 * N * C];
     __shared__ double red[3 * (kThreads / 64)];
     buffer_block<T, N, C>(I, m, length, mode, blockIdx.x, val, red, block_sums + 3 * blockIdx.x);
+}
+
+// +ASC: pattern lnL of the observed patterns shifted by -log(1 - P(const)),
+// the unobserved patterns' entries zeroed
+__global__ void __launch_bounds__(kThreads)
+k_asc_shift(double* __restrict__ ptn_lh, int pobs, int P, double corr) {
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (p < pobs) ptn_lh[p] -= corr;
+    else if (p < P) ptn_lh[p] = 0.0;
 }
 
 // Fixed-order final reduction of per-workgroup partial sums: thread t adds

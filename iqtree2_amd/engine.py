@@ -251,6 +251,11 @@ class Instance:
         if not shutting_down():
             self.close()
 
+    def set_asc(self, nasc):
+        """Lewis's ascertainment bias correction: the last nasc patterns are the
+        unobserved constant patterns (weight 0, +I mass 0)."""
+        check(lib().plk_set_asc(self.ctx.handle, self.id, int(nasc)), "plk_set_asc")
+
     def set_patterns(self, states, freq, invar):
         s = np.ascontiguousarray(states, np.uint8)
         f = np.ascontiguousarray(freq, np.float64)

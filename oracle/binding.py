@@ -36,6 +36,7 @@ _PROTOS = {
                                        c_double, c_double, c_int, _DP, ctypes.POINTER(c_int),
                                        ctypes.POINTER(c_int)]),
     "orc_read_partial": (None, [c_void_p, c_int, _DP, ctypes.POINTER(ctypes.c_int32)]),
+    "orc_set_asc": (None, [c_void_p, c_int]),
 }
 
 _lib = None
@@ -79,6 +80,7 @@ class OracleTree:
         self._h = lib().orc_create(ntaxa, nstates, ncat, npatterns, nslots, state_unknown,
                                    int(bool(safe_numeric)), nthreads)
         self.P = npatterns
+        self.nasc = 0
         self.block = nstates * ncat
         self.ncat = ncat
         self.safe = bool(safe_numeric)
@@ -104,6 +106,12 @@ class OracleTree:
             lib().orc_destroy(self._h)
             self._h = None
 
+    def set_asc(self, nasc):
+        """+ASC (Lewis): the last nasc patterns are the unobserved constant ones;
+        pattern lnL outputs then cover the observed patterns only."""
+        lib().orc_set_asc(self._h, int(nasc))
+        self.nasc = int(nasc)
+
     def set_threads(self, n):
         lib().orc_set_threads(self._h, n)
 
@@ -124,7 +132,7 @@ class OracleTree:
 
     def likelihood_branch(self, ops, node, dad, length, want_pattern_lh=False):
         p, n = _ops(ops)
-        plh = np.zeros(self.P)
+        plh = np.zeros(self.P - self.nasc)
         st = c_int()
         lnl = lib().orc_likelihood_branch(self._h, p, n, _end(node), _end(dad), float(length),
                                           plh.ctypes.data_as(_DP), ctypes.byref(st))
@@ -140,7 +148,7 @@ class OracleTree:
         return df.value, ddf.value
 
     def likelihood_from_buffer(self, length, want_pattern_lh=False):
-        plh = np.zeros(self.P)
+        plh = np.zeros(self.P - self.nasc)
         lnl = lib().orc_likelihood_from_buffer(self._h, float(length), plh.ctypes.data_as(_DP))
         return (lnl, plh) if want_pattern_lh else lnl
 

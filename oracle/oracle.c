@@ -36,6 +36,8 @@ struct orc_tree {
     double* buffer_scale;  /* [Ppad] */
     double* pattern_lh;    /* [Ppad] */
     int theta_ready;
+    int nasc;              /* +ASC (Lewis): the last nasc patterns are the unobserved constant ones */
+    double nsite;          /* sites of the observed patterns (aln->getNSite()) */
     double last_x;         /* length of the last derivative evaluation (Newton) */
     double* info;          /* per-op echild + leaf tables */
     size_t info_cap;
@@ -333,7 +335,22 @@ void orc_set_patterns(orc_tree* t, const uint8_t* states, const double* freq, co
     memcpy(t->freq, freq, (size_t)t->P * 8);
     memcpy(t->invar, invar, (size_t)t->P * 8);
     for (int p = t->P; p < t->Ppad; p++) { t->freq[p] = 0.0; t->invar[p] = 0.0; }
+    orc_set_asc(t, t->nasc);
 }
+
+/* Lewis's ascertainment bias correction (ASC_VARIANT; the unobserved constant
+ * patterns of Alignment::getUnobservedConstPatterns, alignment/alignment.cpp:
+ * 4140-4157, placed after the observed ones as the kernels place them,
+ * tree/phylokernelnew.h:1314-1316) */
+void orc_set_asc(orc_tree* t, int nasc) {
+    t->nasc = nasc;
+    t->nsite = 0.0;
+    for (int p = 0; p < t->P - nasc; p++) t->nsite += t->freq[p];
+}
+
+/* lh of an unobserved constant pattern: rescaled patterns count once at the
+ * scaling threshold (tree/phylokernelnew.h:2940-2947, 2506-2515, 3345-3351) */
+INLINE double asc_lh(double lh, double scale_log) { return scale_log != 0.0 ? lh * SCALING_THRESHOLD : lh; }
 
 void orc_set_model(orc_tree* t, const double* eval, const double* evec, const double* inv_evec,
                    const double* tip_eigen, const double* rates, const double* props) {
@@ -578,13 +595,14 @@ void orc_update_partials(orc_tree* t, const orc_op* ops, int nops) {
 /* computeLikelihoodBranchSIMD (tree/phylokernelnew.h:2641-3223), save_log_value = true */
 INLINE double branch_body(orc_tree* t, const orc_op* ops, int nops, const op_info* infos,
                           orc_end node, orc_end dad, const double* val, const double* tiplut,
-                          const int n) {
+                          const int n, double* prob_const) {
     const int C = t->C, safe = t->safe;
     const size_t block = (size_t)n * C;
+    const size_t nobs = (size_t)(t->P - t->nasc);
     size_t limits[1024];
     compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
-    double all = 0.0;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads) reduction(+ : all)
+    double all = 0.0, all_const = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads) reduction(+ : all, all_const)
     for (int pk = 0; pk < t->npackets; pk++) {
         const size_t lo = limits[pk], hi = limits[pk + 1];
         run_ops_packet(t, ops, nops, infos, lo, hi, pk);
@@ -644,15 +662,18 @@ INLINE double branch_body(orc_tree* t, const orc_op* ops, int nops, const op_inf
             minsc *= vset1(LOG_SCALING_THRESHOLD);
             lh = vabs(lh) + *(const v4d*)(t->invar + ptn);
             for (int x = 0; x < VS; x++) {
-                if (ptn + x < (size_t)t->P) {
+                if (ptn + x < nobs) {
                     const double lp = log(lh[x]) + minsc[x];
                     t->pattern_lh[ptn + x] = lp;
                     tree_lh[x] = lp * t->freq[ptn + x] + tree_lh[x];
+                } else if (ptn + x < (size_t)t->P) {
+                    all_const += asc_lh(lh[x], minsc[x]);
                 }
             }
         }
         all += hadd(tree_lh);
     }
+    *prob_const = all_const;
     return all;
 }
 
@@ -675,22 +696,29 @@ double orc_likelihood_branch(orc_tree* t, const orc_op* ops, int nops, orc_end n
                 for (int i = 0; i < n; i++)
                     tiplut[(size_t)s * block + c * n + i] = val[c * n + i] * t->tip[s * n + i];
     }
-    double lnl;
-    if (n == 4) lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, 4);
-    else if (n == 20) lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, 20);
-    else lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, n);
+    double lnl, prob_const = 0.0;
+    if (n == 4) lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, 4, &prob_const);
+    else if (n == 20) lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, 20, &prob_const);
+    else lnl = branch_body(t, ops, nops, infos, node, dad, val, tiplut, n, &prob_const);
+    const int nobs = t->P - t->nasc;
     int st = 0;
     if (!isfinite(lnl)) {
         /* per-pattern fix-up (tree/phylokernelnew.h:3115-3132) */
         st = 4;
         lnl = 0.0;
-        for (int p = 0; p < t->P; p++) {
+        for (int p = 0; p < nobs; p++) {
             if (!isfinite(t->pattern_lh[p])) t->pattern_lh[p] = LOG_SCALING_THRESHOLD * 4;
             lnl += t->pattern_lh[p] * t->freq[p];
         }
     }
+    if (t->nasc) {
+        /* ASC_Lewis (tree/phylokernelnew.h:3198-3221) */
+        const double corr = log(1.0 - prob_const);
+        for (int p = 0; p < nobs; p++) t->pattern_lh[p] -= corr;
+        lnl -= t->nsite * corr;
+    }
     if (status) *status = st;
-    if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)t->P * 8);
+    if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)nobs * 8);
     free(val);
     free(tiplut);
     free(infos);
@@ -780,8 +808,10 @@ int orc_likelihood_derv(orc_tree* t, double length, double* df, double* ddf) {
     }
     size_t limits[1024];
     compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
-    double all_df = 0.0, all_ddf = 0.0;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads) reduction(+ : all_df, all_ddf)
+    const size_t nobs = (size_t)(t->P - t->nasc);
+    double all_df = 0.0, all_ddf = 0.0, all_pc = 0.0, all_dfc = 0.0, all_ddfc = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads) \
+    reduction(+ : all_df, all_ddf, all_pc, all_dfc, all_ddfc)
     for (int pk = 0; pk < t->npackets; pk++) {
         v4d my_df = vset1(0.0), my_ddf = vset1(0.0);
         for (size_t ptn = limits[pk]; ptn < limits[pk + 1]; ptn += VS) {
@@ -793,7 +823,17 @@ int orc_likelihood_derv(orc_tree* t, double length, double* df, double* ddf) {
             const v4d inv = vset1(1.0) / lh;
             const v4d df_frac = d1 * inv;
             const v4d ddf_frac = d2 * inv;
-            const v4d freq = *(const v4d*)(t->freq + ptn);
+            v4d freq = *(const v4d*)(t->freq + ptn);
+            for (int x = 0; x < VS; x++)
+                if (ptn + x >= nobs && ptn + x < (size_t)t->P) {
+                    /* an unobserved constant pattern: its probability and derivatives
+                     * (tree/phylokernelnew.h:2497-2530) */
+                    freq[x] = 0.0;
+                    const double bs = t->buffer_scale[ptn + x];
+                    all_pc += asc_lh(lh[x], bs);
+                    all_dfc += asc_lh(d1[x], bs);
+                    all_ddfc += asc_lh(d2[x], bs);
+                }
             const v4d tmp1 = df_frac * freq;
             const v4d tmp2 = ddf_frac * freq;
             my_df += tmp1;
@@ -803,6 +843,13 @@ int orc_likelihood_derv(orc_tree* t, double length, double* df, double* ddf) {
         all_ddf += hadd(my_ddf);
     }
     free(val0); free(val1); free(val2);
+    if (t->nasc) {
+        /* ASC_Lewis (tree/phylokernelnew.h:2620-2627) */
+        const double pv = 1.0 - all_pc;
+        const double df_frac = all_dfc / pv, ddf_frac = all_ddfc / pv;
+        all_df += t->nsite * df_frac;
+        all_ddf += t->nsite * (ddf_frac + df_frac * df_frac);
+    }
     *df = all_df;
     *ddf = all_ddf;
     if (!isfinite(*df)) {
@@ -821,18 +868,21 @@ double orc_likelihood_from_buffer(orc_tree* t, double length, double* pattern_lh
         const double len = t->rates[c] * length;
         for (int i = 0; i < n; i++) val0[c * n + i] = exp(t->eval[i] * len) * t->props[c];
     }
-    double all = 0.0;
-#pragma omp parallel for num_threads(t->nthreads) reduction(+ : all)
+    const size_t nobs = (size_t)(t->P - t->nasc);
+    double all = 0.0, all_const = 0.0;
+#pragma omp parallel for num_threads(t->nthreads) reduction(+ : all, all_const)
     for (size_t ptn = 0; ptn < (size_t)t->Ppad; ptn += VS) {
         const v4d* th = (const v4d*)(t->theta + ptn * block);
         v4d lh = dot_vec_s(val0, th, (int)block);
         lh = vabs(lh) + *(const v4d*)(t->invar + ptn);
         v4d acc = vset1(0.0);
         for (int x = 0; x < VS; x++) {
-            if (ptn + x < (size_t)t->P) {
+            if (ptn + x < nobs) {
                 const double lp = log(fabs(lh[x])) + t->buffer_scale[ptn + x];
                 t->pattern_lh[ptn + x] = lp;
                 acc[x] = lp * t->freq[ptn + x];
+            } else if (ptn + x < (size_t)t->P) {
+                all_const += asc_lh(lh[x], t->buffer_scale[ptn + x]);
             }
         }
         all += hadd(acc);
@@ -840,12 +890,18 @@ double orc_likelihood_from_buffer(orc_tree* t, double length, double* pattern_lh
     free(val0);
     if (!isfinite(all)) {
         all = 0.0;
-        for (int p = 0; p < t->P; p++) {
+        for (size_t p = 0; p < nobs; p++) {
             if (!isfinite(t->pattern_lh[p])) t->pattern_lh[p] = LOG_SCALING_THRESHOLD * 4;
             all += t->pattern_lh[p] * t->freq[p];
         }
     }
-    if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)t->P * 8);
+    if (t->nasc) {
+        /* ASC_Lewis (tree/phylokernelnew.h:3405-3424) */
+        const double corr = log(1.0 - all_const);
+        for (size_t p = 0; p < nobs; p++) t->pattern_lh[p] -= corr;
+        all -= t->nsite * corr;
+    }
+    if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, nobs * 8);
     return all;
 }
 

@@ -48,6 +48,11 @@ void orc_set_patterns(orc_tree* t, const uint8_t* states, const double* freq,
                       const double* invar);
 void orc_set_model(orc_tree* t, const double* eval, const double* evec, const double* inv_evec,
                    const double* tip_eigen, const double* rates, const double* props);
+/* Lewis's ascertainment bias correction (ASC_VARIANT): the last nasc of the
+ * npatterns patterns are the unobserved constant patterns (frequency 0);
+ * lnL, pattern lnL and derivatives are conditioned on variable sites.
+ * pattern_lh outputs then hold the npatterns - nasc observed patterns. */
+void orc_set_asc(orc_tree* t, int nasc);
 
 void orc_update_partials(orc_tree* t, const orc_op* ops, int nops);
 /* returns tree lnL; status (nullable) 0 ok, 4 non-finite fixed up */

@@ -322,6 +322,40 @@ def cpu_dna(ia, w, budget_s, threads):
 
 
 # ---------------------------------------------------------------------------
+# NNI evaluation, as tree search runs it (getBestNNIForBran on every inner branch)
+def leg_nni(ia, ctx, w, max_nr=10, reps=3):
+    """Every inner branch's two NNI moves (nni5: after the swap, node1's two
+    other branches, the central branch and node2's two other branches are
+    optimised, then the lnL is read from the buffer; NNI_MAX_NR_STEP 10), all
+    in one device sweep; and the host-driven form (one engine call per
+    derivative evaluation) on a sample of inner branches."""
+    tree = ia.PhyloTree(ctx, w)
+    try:
+        tree.compute_likelihood()
+        inner = tree.inner_branches()
+        m = len(inner)
+        tree.nni_evaluate_all(max_nr)                 # warm-up (tree unchanged afterwards)
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            _, _, nd = tree.nni_evaluate_all(max_nr)
+        dt = (time.perf_counter() - t) / reps
+        sample = inner[::max(1, m // 16)]
+        t = time.perf_counter()
+        for a, b in sample:
+            tree.nni_evaluate(a, b, max_nr)
+        dth = (time.perf_counter() - t) / len(sample)
+        return {"inner_branches": m, "moves": 2 * m, "device_sweep_ms": 1e3 * dt,
+                "moves_per_s": 2 * m / dt, "branch_optimisations": 10 * m,
+                "derivative_evals": int(nd.sum()),
+                "pattern_derivative_evals_per_s": w.npatterns * int(nd.sum()) / dt,
+                "host_driven_ms_per_inner_branch": 1e3 * dth,
+                "host_driven_moves_per_s": 2.0 / dth}
+    finally:
+        tree.close()
+
+
+# ---------------------------------------------------------------------------
 # matrix-core traversals: protein 100 x 20,000 and codon 64 x 5,000
 def leg_mfma(ia, ctx, reps=3):
     from iqtree2_amd import workloads as wl
@@ -487,7 +521,7 @@ def main():
     ap.add_argument("--no-dna", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
-    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy"),
+    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy", "nni"),
                     help="run one leg alone (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -510,6 +544,10 @@ def main():
                 out = leg_mfma(ia, ctx)
             elif args.only == "copy":
                 out = leg_copy(ctx)
+            elif args.only == "nni":
+                w = wl.dna_200x100k()
+                out = leg_nni(ia, ctx, w)
+                w.close()
             else:
                 out = leg_serving(ia, ctx, args.requests)
             print(json.dumps({args.only: out}), flush=True)
@@ -533,6 +571,7 @@ def main():
             dna = leg_dna(ia, ctx, w, args)
             if not args.no_cpu:
                 dna["cpu_baseline"] = cpu_dna(ia, w, args.cpu_seconds / 2, host_threads())
+            dna["nni"] = leg_nni(ia, ctx, w)
             w.close()
             log(f"dna: {json.dumps(dna)}")
         if not args.no_mfma:

@@ -20,7 +20,7 @@
 //     each phase-A A operand is one contiguous 64-lane read;
 //   - in step st, lane (col, g) takes K index 5g + st, so a lane's five B
 //     operands (child values) are five consecutive elements of its pattern;
-//   - U^-1 (phase B's B operand) is loaded into registers once per tile;
+//   - U^-1 (phase B's B operand) is staged in LDS in lane order once per tile;
 //   - tip states are read one step ahead, so a tip child's lookup rows are
 //     read together with the internal child's values.
 // The only exchange between waves is the per-(pattern, category) maximum that
@@ -51,6 +51,18 @@ __device__ inline void prot_ops_tile(const InstDev& I, const double* __restrict_
     constexpr int CPW = (C + CW - 1) / CW;     // categories per wave
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, g = lane >> 4;
+    // phase B's B operand, B[k = g][n = col] = U^-1[16 ib + col][x] for the
+    // K steps (xb, r), x = 16 xb + row(g, r), in LDS in lane order:
+    // ub_sh[((ib * 2 + xb) * 4 + r) * 64 + lane] (in registers it costs the
+    // FP64 tile a wave per SIMD); staged by every thread before the waves
+    // without categories leave
+    __shared__ T ub_sh[2 * 2 * 4 * 64];
+    for (int t = threadIdx.x; t < 2 * 2 * 4 * 64; t += kThreads) {
+        const int tb = t >> 9, txb = (t >> 8) & 1, tr = (t >> 6) & 3, tl = t & 63;
+        const int i = 16 * tb + (tl & 15), x = 16 * txb + M::row(tl >> 4, tr);
+        ub_sh[t] = (i < N && x < N) ? (T)inv_evec[i * N + x] : (T)0;
+    }
+    __syncthreads();
     if (wave >= CW) {
         // no categories: only the per-step barriers
         for (int o = 0; o < nops; o++) __syncthreads();
@@ -64,19 +76,6 @@ __device__ inline void prot_ops_tile(const InstDev& I, const double* __restrict_
     const int pl = p_first + lane;
     const bool no_invar = plane && I.invar[pl] == 0.0;
     const bool counters = safe || wave == 0;   // normal mode: one counter per pattern, wave 0
-
-    // phase B's B operand, B[k = g][n = col] = U^-1[16 ib + col][x] for the
-    // K steps (xb, r) that hold a state, x = 16 xb + row(g, r)
-    T ub[2][2][4];
-#pragma unroll
-    for (int ib = 0; ib < 2; ib++)
-#pragma unroll
-        for (int xb = 0; xb < 2; xb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = 16 * ib + col, x = 16 * xb + M::row(g, r);
-                ub[ib][xb][r] = (i < N && x < N) ? (T)inv_evec[i * N + x] : (T)0;
-            }
 
     // tip states of the next step's children (this lane's pattern column of
     // sub-tile s in byte s)
@@ -193,7 +192,7 @@ __device__ inline void prot_ops_tile(const InstDev& I, const double* __restrict_
                     for (int r = 0; r < 4; r++) {
                         if (16 * xb + M::row(0, r) >= N) continue;   // a K step with no state
 #pragma unroll
-                        for (int ib = 0; ib < 2; ib++) d[ib] = M::mma(prod[xb][r], ub[ib][xb][r], d[ib]);
+                        for (int ib = 0; ib < 2; ib++) d[ib] = M::mma(prod[xb][r], ub_sh[((ib * 2 + xb) * 4 + r) * 64 + lane], d[ib]);
                     }
                 // unscaled store and the per-(pattern, category) maxima
 #pragma unroll

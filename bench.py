@@ -356,6 +356,72 @@ def leg_nni(ia, ctx, w, max_nr=10, reps=3):
 
 
 # ---------------------------------------------------------------------------
+# Fitch parsimony (computePartialParsimonyFast + computeParsimonyBranchFast)
+def leg_parsimony(ia, ctx, w, reps=20):
+    """The whole tree from scratch (every internal node's sets, then the
+    branch), timed per call and per kernel launch; then every branch in turn
+    through the tree driver, which recomputes only the stale slots (the
+    reference's partial_lh_computed & 2), as a stepwise-addition or SPR scan
+    evaluates branches."""
+    inst = ia.Instance.from_workload(ctx, w)
+    tree = ia.PhyloTree(ctx, w)
+    try:
+        ops, node, dad = tree.plan_full(*tree.branches()[0])
+        score = inst.parsimony_branch(ops, node, dad)        # builds the tip planes
+        ctx.synchronize()
+        ctx.profile_start()
+        t = time.perf_counter()
+        for _ in range(reps):
+            assert inst.parsimony_branch(ops, node, dad) == score
+        dt = (time.perf_counter() - t) / reps
+        prof = ctx.profile_stop()
+        k = prof["parsimony"]
+        avg_s = k["total_ms"] * 1e-3 / k["launches"]
+        bytes_per = k["bytes"] / k["launches"]
+        branches = tree.branches()
+        tree.parsimony_branch(*branches[0])
+        t = time.perf_counter()
+        nb = 0
+        for a, b in branches + branches[::-1]:
+            assert tree.parsimony_branch(a, b)[0] == score[0]
+            nb += 1
+        dtb = (time.perf_counter() - t) / nb
+        return {"score": score[0], "ops": len(ops), "patterns": w.npatterns,
+                "full_tree_ms": 1e3 * dt, "full_tree_kernel_us": 1e6 * avg_s,
+                "pattern_node_evals_per_s": w.npatterns * len(ops) / avg_s,
+                "roofline": {"bound": "hbm", "achieved": bytes_per / avg_s / 1e9,
+                             "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                             "frac": bytes_per / avg_s / HBM_PEAK, "bytes_per_launch": bytes_per},
+                "branch_scan_ms_per_branch": 1e3 * dtb, "branch_scan_branches": nb}
+    finally:
+        inst.close()
+        tree.close()
+
+
+def cpu_parsimony(ia, w, budget_s, threads):
+    """The reference's bit-parallel Fitch (sites expanded by frequency,
+    32-site planes, popcount, OpenMP over 256-site blocks) restated in the
+    oracle, on the same whole-tree op list."""
+    from oracle.binding import OracleTree
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w, nthreads=threads)
+    try:
+        ops, node, dad = planner.plan_full(*planner.branches()[0])
+        score = orc.parsimony_fast(ops, node, dad)           # builds the tip planes
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            assert orc.parsimony_fast(ops, node, dad) == score
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+    finally:
+        planner.close()
+    return {"full_tree_ms": 1e3 * dt, "pattern_node_evals_per_s": w.npatterns * len(ops) / dt,
+            "score": score[0], "cores": threads, "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"{n} whole-tree evaluations, {n * dt:.1f} s"}
+
+
+# ---------------------------------------------------------------------------
 # matrix-core traversals: protein 100 x 20,000 and codon 64 x 5,000
 def leg_mfma(ia, ctx, reps=3):
     from iqtree2_amd import workloads as wl
@@ -521,7 +587,8 @@ def main():
     ap.add_argument("--no-dna", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
-    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy", "nni"),
+    ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy", "nni",
+                                       "parsimony"),
                     help="run one leg alone (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -548,6 +615,12 @@ def main():
                 w = wl.dna_200x100k()
                 out = leg_nni(ia, ctx, w)
                 w.close()
+            elif args.only == "parsimony":
+                w = wl.dna_200x100k()
+                out = leg_parsimony(ia, ctx, w)
+                if not args.no_cpu:
+                    out["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
+                w.close()
             else:
                 out = leg_serving(ia, ctx, args.requests)
             print(json.dumps({args.only: out}), flush=True)
@@ -572,6 +645,9 @@ def main():
             if not args.no_cpu:
                 dna["cpu_baseline"] = cpu_dna(ia, w, args.cpu_seconds / 2, host_threads())
             dna["nni"] = leg_nni(ia, ctx, w)
+            dna["parsimony"] = leg_parsimony(ia, ctx, w)
+            if not args.no_cpu:
+                dna["parsimony"]["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
             w.close()
             log(f"dna: {json.dumps(dna)}")
         if not args.no_mfma:

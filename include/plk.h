@@ -309,6 +309,20 @@ typedef struct {
     double flops;
 } plk_kernel_stat;
 
+/* Fitch parsimony (PhyloTree::computePartialParsimonyFast and
+ * computeParsimonyBranchFast, tree/phylotreepars.cpp:32-300; the SIMD kernel
+ * tree/phylokernel.h:1465-2094): ops compute the Fitch state sets and subtree
+ * change counts of their dad slots (kept apart from the likelihood partials,
+ * same slot numbers; a full post-order list for a first evaluation, then only
+ * the slots whose direction changed), then *score = sum_p freq_p * changes_p
+ * of the tree seen across the branch (node, dad) -- the reference's integer
+ * score for integer frequencies -- and *branch_subst (nullable) the changes
+ * on that branch alone (the reference's branch_subst). Tip sets: the observed
+ * state, the ambiguity set (DNA IUPAC, protein B/Z/J) or every state for
+ * STATE_UNKNOWN. DNA, protein and codon instances. */
+plk_status plk_parsimony_branch(plk_context* ctx, int instance, const plk_op* ops, int nops,
+                                plk_end node, plk_end dad, double* score, double* branch_subst);
+
 /* HBM bandwidth of a plain device copy of `bytes` (read + written per launch,
  * reps launches after one warm-up), timed with HIP events on the context's
  * stream: the measured roofline denominator beside the 8 TB/s peak, and the

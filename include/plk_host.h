@@ -117,6 +117,16 @@ int plkh_tree_optimize_branch(plkh_tree* t, int a, int b, int max_nr, double* ne
  * lnl = final tree lnL, n_derv = derivative evaluations */
 int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double tolerance, int max_nr,
                                     double* lnl, int* n_derv);
+/* the full post-order op list for branch (a, b), every internal node
+ * included, no likelihood partial marked (for parsimony) */
+int plkh_tree_plan_full(plkh_tree* t, int a, int b, plk_op* ops, int cap, int* nops, plk_end* node,
+                        plk_end* dad);
+/* computeParsimonyBranchFast at branch (a, b) (tree/phylotreepars.cpp:240-300):
+ * the tree's frequency-weighted Fitch changes; branch_subst (nullable) the
+ * changes on the branch itself. Only slots not holding the sets for the
+ * needed direction are recomputed (the reference's partial_lh_computed & 2);
+ * an NNI swap drops them all. */
+int plkh_tree_parsimony_branch(plkh_tree* t, int a, int b, double* score, double* branch_subst);
 /* PhyloSuperTree::optimizeAllBranches (tree/phylosupertree.cpp:773-792):
  * optimizeAllBranches(iterations, tolerance / min(ntrees, 10), max_nr) on every
  * tree (all on one context), with the host Newton loop of each partition

@@ -130,6 +130,10 @@ _PROTOS = {
     "plk_likelihood_derv_batch_noalloc": (c_int, [c_void_p, _I32, _DP, c_int, _DP, _DP, _DP]),
     "plk_copy_bandwidth": (c_int, [c_void_p, c_size_t, c_int, _DP, _DP]),
     "plk_set_asc": (c_int, [c_void_p, c_int, c_int]),
+    "plk_parsimony_branch": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd, _DP, _DP]),
+    "plkh_tree_plan_full": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, P(c_int), P(PlkEnd),
+                                    P(PlkEnd)]),
+    "plkh_tree_parsimony_branch": (c_int, [c_void_p, c_int, c_int, _DP, _DP]),
     "plk_batch_evaluate": (c_int, [c_void_p, P(PlkBatchItem), _I32, c_int, _DP]),
     "plk_optimize_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, _DP, c_int,
                                    c_double, c_double, c_int, _I32, _DP]),

@@ -353,23 +353,55 @@ __host__ __device__ inline int batch_class(int n, int f32) {
     return f32 ? BC_COD32 : BC_COD64;
 }
 
-// Persistent workgroups over tiles [tile_begin, tile_end) of the list. MASK
+// The tile list of one launch in kXcds segments, one per XCD of the device:
+// the host places all of an item's tiles in one segment (its transition
+// tables then sit in one XCD's L2 instead of every XCD fetching them),
+// items bin-packed by cost. Segment q holds tiles [b[q], b[q + 1]).
+constexpr int kXcds = 8;
+struct XcdSplit {
+    int b[kXcds + 1];
+};
+
+// Persistent workgroups over the tiles of `xs`. A workgroup takes tiles from
+// its own XCD's segment (workgroups are dispatched to the XCDs round-robin,
+// so blockIdx.x % kXcds), then from the others' in turn until every segment
+// is drained; every workgroup ends after trying each segment's counter. MASK
 // names the instance classes the launch may meet; a tile's code path is
 // uniform across its workgroup.
+// The protein FP64 class is held to 3 waves per SIMD (<= 168 registers,
+// which it fits without spilling); the compiler's own choice left it at 2.
 template <int MASK>
-__global__ void __launch_bounds__(kThreads)
-k_batch(const BatchItem* __restrict__ items, const int* __restrict__ tile_item, int tile_begin,
-        int tile_end, unsigned* __restrict__ counter, double* __restrict__ tile_sums) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MASK == BC_PROT64 ? 3 : 1)))
+k_batch(const BatchItem* __restrict__ items, const int* __restrict__ tile_item, XcdSplit xs,
+        unsigned* __restrict__ counters, double* __restrict__ tile_sums) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int next_sh;
+    // thread 0's scheduling state lives in LDS, so no register stays live
+    // across the tiles
+    __shared__ int next_sh, tried_sh, bound_sh[kXcds + 1];
+    if (threadIdx.x <= kXcds) bound_sh[threadIdx.x] = xs.b[threadIdx.x];
+    if (threadIdx.x == 0) tried_sh = 0;   // segments found empty
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) next_sh = tile_begin + (int)atomicAdd(counter, 1u);
+        if (threadIdx.x == 0) {
+            const int home = blockIdx.x % kXcds;
+            int g = -1, tried = tried_sh;
+            while (tried < kXcds) {
+                const int q = (home + tried) % kXcds;
+                const int t = bound_sh[q] + (int)atomicAdd(&counters[q], 1u);
+                if (t < bound_sh[q + 1]) {
+                    g = t;
+                    break;
+                }
+                tried++;
+            }
+            tried_sh = tried;
+            next_sh = g;
+        }
         __syncthreads();
         // wave-uniform by construction; readfirstlane lets the item's fields
         // live in scalar registers
         const int g = __builtin_amdgcn_readfirstlane(next_sh);
-        if (g >= tile_end) break;
+        if (g < 0) break;
         const int i = __builtin_amdgcn_readfirstlane(tile_item[g]);
         const BatchItem& it = items[i];
         const int tile = g - it.tile_begin;

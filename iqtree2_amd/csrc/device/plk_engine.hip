@@ -21,6 +21,7 @@
 #include "plk_batch.h"
 #include "plk_codon.h"
 #include "plk_mfma.h"
+#include "plk_pars.h"
 #include "newton.h"
 
 using namespace plkdev;
@@ -122,6 +123,15 @@ struct Instance {
     void* tables = nullptr;
     bool patterns_set = false, model_set = false, theta_ready = false, ptn_lh_ready = false;
     double theta_length = 0.0;
+    // Fitch parsimony state (allocated by the first parsimony call)
+    uint32_t* pars_tip_planes = nullptr;   // [ntaxa][n][Ppad/32] (DNA, protein)
+    uint32_t* pars_planes = nullptr;       // [nslots][n][Ppad/32] (DNA, protein)
+    double* pars_counts = nullptr;         // [nslots][Ppad/32] (DNA, protein)
+    uint64_t* pars_sets = nullptr;         // [nslots][Ppad] (codon)
+    uint16_t* pars_changes = nullptr;      // [nslots][Ppad] (codon)
+    bool pars_tips_ready = false;          // tip planes of the current patterns
+    plk_op* pars_ops = nullptr;            // ops sorted by level, then level starts (int)
+    int pars_ops_cap = 0;
     // +ASC (Lewis): the last nasc patterns are the unobserved constant ones
     int nasc = 0;
     std::vector<double> hfreq;   // pattern weights (host copy)
@@ -168,12 +178,13 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
                 K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH,
-                K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_BATCH_CODON, K_NKIND };
+                K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_BATCH_CODON, K_PARSIMONY,
+                K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
                                           "reduce", "rell", "newton_sweep", "batch", "prep_batch",
                                           "traverse_mfma", "root", "batch_dna", "batch_protein",
-                                          "batch_codon"};
+                                          "batch_codon", "parsimony"};
 
 struct ProfRec {
     int kind;
@@ -329,7 +340,9 @@ plk_status destroy_instance(plk_context* ctx, int id) {
     void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
                     in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
                     in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
-                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res};
+                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res,
+                    in->pars_tip_planes, in->pars_planes, in->pars_counts, in->pars_sets,
+                    in->pars_changes, in->pars_ops};
     for (void* b : bufs) dfree(ctx, nullptr, b, 0);
     delete in;
     ctx->inst[id] = nullptr;
@@ -993,6 +1006,7 @@ plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, con
     in->nsite = 0.0;
     for (int p = 0; p < P - in->nasc; p++) in->nsite += freq[p];
     in->patterns_set = true;
+    in->pars_tips_ready = false;
     in->theta_ready = false;
     return PLK_OK;
 }
@@ -1360,29 +1374,101 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     plk_status st;
     std::lock_guard<std::mutex> lock(ctx->mu);
     // largest partitions first (PhyloSuperTree::computePartitionOrder)
-    std::vector<int> order(nitems);
     std::vector<double> cost(nitems);
+    std::vector<int> cls(nitems), ntiles_of(nitems);
+    int total_tiles = 0, mix = 0, nclasses = 0;
     for (int i = 0; i < nitems; i++) {
-        order[i] = i;
-        const double n = ins[i]->d.nstates;
-        cost[i] = (double)ins[i]->d.npatterns * (items[i].nops + 1) * n * n;
+        const Instance* in = ins[i];
+        const double n = in->d.nstates;
+        cost[i] = (double)in->d.npatterns * (items[i].nops + 1) * n * n;
+        cls[i] = batch_class(in->d.nstates, in->d.precision == PLK_PREC_F32);
+        const int PB = batch_tile_patterns(in->d.nstates, in->d.ncat);
+        ntiles_of[i] = (in->d.npatterns + PB - 1) / PB;
+        total_tiles += ntiles_of[i];
+        if (!(mix & cls[i])) nclasses++;
+        mix |= cls[i];
     }
-    // grouped by instance class (so each class's tiles are contiguous for a
-    // per-class launch), largest partitions first inside a class
-    std::vector<int> cls(nitems);
-    for (int i = 0; i < nitems; i++)
-        cls[i] = batch_class(ins[i]->d.nstates, ins[i]->d.precision == PLK_PREC_F32);
-    // (codon classes last: the fused DNA + protein launch takes a prefix)
-    auto rank = [](int c) { return (c & BC_ALL) ? c + kBatchClasses * 64 : c; };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        if (cls[a] != cls[b]) return rank(cls[a]) > rank(cls[b]);
+    // launch shape: the tuned one for a recurring batch shape (class mix,
+    // log2 tiles), else the default. Online tuning: a recurring shape's
+    // first calls each run one candidate (fused or per-class launches x 1,
+    // 2, 4, 8 workgroups per CU) and time it; results do not depend on the
+    // shape, so no call repeats work. Batches of fewer tiles than a few per
+    // CU keep the default. Likely-good shapes first, so exploring costs little.
+    int lg = 0;
+    while ((1 << lg) < total_tiles) lg++;
+    const long key = (long)mix * 64 + lg;
+    int split = 1, bpc = ctx->batch_blocks_per_cu > 0 ? ctx->batch_blocks_per_cu : 2;
+    static const std::pair<int, int> kOrder[8] = {{1, 2}, {1, 4}, {0, 2}, {0, 4}, {1, 8}, {0, 8}, {1, 1}, {0, 1}};
+    std::pair<int, int> cands[8];
+    int ncand = 0;
+    for (const auto& cd : kOrder)
+        if (cd.first == 0 || nclasses > 1) cands[ncand++] = cd;
+    plk_context::BatchTune* tune = nullptr;
+    if (ctx->batch_blocks_per_cu == 0 && ctx->autotune && total_tiles >= 4 * ctx->num_cu) {
+        plk_context::BatchTune& bt = ctx->batch_tune[key];
+        if (bt.done) {
+            split = bt.split;
+            bpc = bt.bpc;
+        } else {
+            split = cands[bt.next].first;
+            bpc = cands[bt.next].second;
+            tune = &bt;
+        }
+    }
+    // launch groups: one per class (split), or the DNA and protein classes
+    // fused and each codon class alone; inside a group every item's tiles go
+    // to one XCD segment, items bin-packed over the segments by cost (largest
+    // first onto the least loaded), so an item's tables are fetched into one
+    // XCD's L2
+    auto group_of = [&](int c) { return (split || !(c & BC_ALL)) ? c : BC_ALL; };
+    auto rank = [](int g) { return (g & BC_ALL) ? g + kBatchClasses * 64 : g; };   // fused / DNA / protein first
+    std::vector<int> sorted(nitems);
+    for (int i = 0; i < nitems; i++) sorted[i] = i;
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
+        const int ga = group_of(cls[a]), gb = group_of(cls[b]);
+        if (ga != gb) return rank(ga) > rank(gb);
         return cost[a] > cost[b];
     });
+    std::vector<int> order;
+    order.reserve(nitems);
+    struct Group {
+        int mask;
+        XcdSplit xs;
+    };
+    std::vector<Group> groups;
+    int t = 0;
+    for (int r0 = 0; r0 < nitems;) {
+        const int g = group_of(cls[sorted[r0]]);
+        int r1 = r0;
+        while (r1 < nitems && group_of(cls[sorted[r1]]) == g) r1++;
+        double load[kXcds] = {};
+        std::vector<int> seg[kXcds];
+        for (int r = r0; r < r1; r++) {
+            int q = 0;
+            for (int k = 1; k < kXcds; k++)
+                if (load[k] < load[q]) q = k;
+            load[q] += cost[sorted[r]];
+            seg[q].push_back(sorted[r]);
+        }
+        Group G;
+        G.mask = 0;
+        for (int q = 0; q < kXcds; q++) {
+            G.xs.b[q] = t;
+            for (int i : seg[q]) {
+                order.push_back(i);
+                t += ntiles_of[i];
+                G.mask |= cls[i];
+            }
+        }
+        G.xs.b[kXcds] = t;
+        groups.push_back(G);
+        r0 = r1;
+    }
     // one pinned upload: items | ops | tile list | root lengths
     std::vector<BatchItem> hb(nitems);
     std::vector<size_t> op_off(nitems);
     size_t tab_bytes = 0, root_doubles = 0, smem_prep = 0, nops_total = 0;
-    int total_tiles = 0, max_nops = 0;
+    int tiles_so_far = 0, max_nops = 0;
     // algorithmic bytes and flops of the batch, per instance class (index:
     // the BatchClass bit)
     double cls_bytes[kBatchClasses] = {}, cls_flops[kBatchClasses] = {};
@@ -1399,7 +1485,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         b.m = model_dev(in);
         b.nops = items[i].nops;
         b.table_size = in->table_size;
-        b.tile_begin = total_tiles;
+        b.tile_begin = tiles_so_far;
         b.ntiles = (in->d.npatterns + PB - 1) / PB;
         b.node = items[i].node;
         b.dad = items[i].dad;
@@ -1407,7 +1493,7 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
         b.C = in->d.ncat;
         b.f32 = in->d.precision == PLK_PREC_F32;
         b.mode = modes ? modes[i] : BATCH_LNL;
-        total_tiles += b.ntiles;
+        tiles_so_far += b.ntiles;
         b.tables = reinterpret_cast<const void*>(tab_bytes);   // offsets until the buffers exist
         tab_bytes += round_up((size_t)items[i].nops * 2 * in->table_size * in->elem, 256);
         b.root = reinterpret_cast<const double*>(root_doubles);
@@ -1467,20 +1553,9 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
     const int rows = std::max(1, std::min(max_nops, std::max(4, (4096 + nitems - 1) / nitems)));
     int pe = prof_begin(ctx);
     hipLaunchKernelGGL(k_prep_batch, dim3(nitems, rows), dim3(kThreads), smem_prep, ctx->stream,
-                       d_items, d_lens, ctx->bt_roots, ctx->bt_counter, 16);
+                       d_items, d_lens, ctx->bt_roots, ctx->bt_counter, kXcds * kBatchClasses);
     HIP_TRY(hipGetLastError());
     prof_end(ctx, pe, K_PREP_BATCH, (double)tab_bytes, 0.0);
-    // class ranges of the tile list
-    int mix = 0;
-    std::vector<std::pair<int, int>> ranges;   // (class, end tile)
-    for (int r = 0; r < nitems; r++) {
-        const int c = batch_class(hb[r].n, hb[r].f32);
-        mix |= c;
-        const int end = hb[r].tile_begin + hb[r].ntiles;
-        if (ranges.empty() || ranges.back().first != c) ranges.push_back({c, end});
-        else ranges.back().second = end;
-    }
-    bool counters_clear = true;   // k_prep_batch cleared them
     // profiling record of one launch over the classes in mask
     auto prof_mask = [&](int pe2, int mask) {
         if (pe2 < 0) return;
@@ -1495,13 +1570,14 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
                          : mask == BC_COD64 || mask == BC_COD32 ? K_BATCH_CODON : K_BATCH;
         prof_end(ctx, pe2, kind, b2, f2);
     };
-    auto launch = [&](int split, int bpc) -> plk_status {
-        if (!counters_clear) HIP_TRY(hipMemsetAsync(ctx->bt_counter, 0, sizeof(unsigned) * 16, ctx->stream));
-        counters_clear = false;
-        auto one = [&](int mask, int begin, int end, int k) -> plk_status {
+    // one launch per group, each with its own kXcds work counters
+    auto launch = [&]() -> plk_status {
+        for (size_t k = 0; k < groups.size(); k++) {
+            const Group& G = groups[k];
             const int pe2 = prof_begin(ctx);
+            const int fmask = __builtin_popcount(G.mask) > 1 ? BC_ALL : G.mask;   // fused DNA + protein
             const void* fn;
-            switch (mask) {
+            switch (fmask) {
             case BC_DNA64: fn = (const void*)k_batch<BC_DNA64>; break;
             case BC_PROT64: fn = (const void*)k_batch<BC_PROT64>; break;
             case BC_DNA32: fn = (const void*)k_batch<BC_DNA32>; break;
@@ -1512,78 +1588,30 @@ plk_status batch_fused(plk_context* ctx, const plk_batch_item* items, const int*
             }
             size_t smem = 0;
             for (int cb = 0; cb < kBatchClasses; cb++)
-                if (((mask == BC_ALL ? mix & BC_ALL : mask) >> cb) & 1) smem = std::max(smem, cls_smem[cb]);
+                if ((G.mask >> cb) & 1) smem = std::max(smem, cls_smem[cb]);
             int nb = 0;
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kThreads, smem));
-            const int grid = std::max(1, std::min(end - begin, ctx->num_cu * std::max(1, std::min(nb, bpc))));
+            const int ntl = G.xs.b[kXcds] - G.xs.b[0];
+            const int grid = std::max(1, std::min(ntl, ctx->num_cu * std::max(1, std::min(nb, bpc))));
             const BatchItem* a0 = d_items;
             const int* a1 = d_tiles;
-            unsigned* a4 = ctx->bt_counter + k;
-            double* a5 = ctx->bt_tile_sums;
-            void* args[] = {(void*)&a0, (void*)&a1, (void*)&begin, (void*)&end, (void*)&a4, (void*)&a5};
+            XcdSplit xs = G.xs;
+            unsigned* a3 = ctx->bt_counter + kXcds * k;
+            double* a4 = ctx->bt_tile_sums;
+            void* args[] = {(void*)&a0, (void*)&a1, (void*)&xs, (void*)&a3, (void*)&a4};
             HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, smem, ctx->stream));
-            prof_mask(pe2, mask == BC_ALL ? mix & BC_ALL : mask);
-            return PLK_OK;
-        };
-        if (ranges.size() == 1) return one(ranges[0].first, 0, total_tiles, 0);
-        if (!split) {
-            // DNA and protein tiles fused (they lead the list), codon classes after
-            int begin = 0, k = 0, fused_end = 0;
-            for (auto& rg : ranges)
-                if (rg.first & BC_ALL) fused_end = rg.second;
-            if (fused_end > 0) {
-                plk_status s2 = one(BC_ALL, 0, fused_end, k++);
-                if (s2 != PLK_OK) return s2;
-            }
-            begin = fused_end;
-            for (auto& rg : ranges) {
-                if (rg.first & BC_ALL) continue;
-                plk_status s2 = one(rg.first, begin, rg.second, k++);
-                if (s2 != PLK_OK) return s2;
-                begin = rg.second;
-            }
-            return PLK_OK;
-        }
-        int begin = 0, k = 0;
-        for (auto& rg : ranges) {
-            plk_status s2 = one(rg.first, begin, rg.second, k++);
-            if (s2 != PLK_OK) return s2;
-            begin = rg.second;
+            prof_mask(pe2, G.mask);
         }
         return PLK_OK;
     };
-    int lg = 0;
-    while ((1 << lg) < total_tiles) lg++;
-    const long key = (long)mix * 64 + lg;
-    int split = 1, bpc = ctx->batch_blocks_per_cu > 0 ? ctx->batch_blocks_per_cu : 2;
-    // Online tuning: a recurring shape's first calls each run one candidate
-    // (fused or per-class launches x 1, 2, 4, 8 workgroups per CU) and time
-    // it; results do not depend on the shape, so no call repeats work.
-    // Batches of fewer tiles than a few per CU keep the default.
-    // likely-good shapes first, so exploring costs little
-    static const std::pair<int, int> kOrder[8] = {{1, 2}, {1, 4}, {0, 2}, {0, 4}, {1, 8}, {0, 8}, {1, 1}, {0, 1}};
-    std::pair<int, int> cands[8];
-    int ncand = 0;
-    for (const auto& cd : kOrder)
-        if (cd.first == 0 || ranges.size() > 1) cands[ncand++] = cd;
-    plk_context::BatchTune* tune = nullptr;
-    if (ctx->batch_blocks_per_cu == 0 && ctx->autotune && total_tiles >= 4 * ctx->num_cu) {
-        plk_context::BatchTune& bt = ctx->batch_tune[key];
-        if (bt.done) {
-            split = bt.split;
-            bpc = bt.bpc;
-        } else {
-            split = cands[bt.next].first;
-            bpc = cands[bt.next].second;
-            tune = &bt;
-            if (!ctx->tune_ev[0]) {
-                HIP_TRY(hipEventCreate(&ctx->tune_ev[0]));
-                HIP_TRY(hipEventCreate(&ctx->tune_ev[1]));
-            }
-            HIP_TRY(hipEventRecord(ctx->tune_ev[0], ctx->stream));
+    if (tune) {
+        if (!ctx->tune_ev[0]) {
+            HIP_TRY(hipEventCreate(&ctx->tune_ev[0]));
+            HIP_TRY(hipEventCreate(&ctx->tune_ev[1]));
         }
+        HIP_TRY(hipEventRecord(ctx->tune_ev[0], ctx->stream));
     }
-    if ((st = launch(split, bpc)) != PLK_OK) return st;
+    if ((st = launch()) != PLK_OK) return st;
     if (tune) HIP_TRY(hipEventRecord(ctx->tune_ev[1], ctx->stream));
     hipLaunchKernelGGL(k_batch_reduce, dim3((nitems + kThreads - 1) / kThreads), dim3(kThreads), 0,
                        ctx->stream, d_items, nitems, ctx->bt_tile_sums, ctx->bt_vals);
@@ -1982,6 +2010,127 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     if (status == PLK_ERR_NONFINITE) return fail(PLK_ERR_NONFINITE, "non-finite derivative in the Newton sweep");
     if (status == PLK_ERR_DEVICE) return fail(PLK_ERR_DEVICE, "grid reduction in the Newton sweep timed out");
     if (status != 0) return fail((plk_status)status, "Newton sweep failed (status %d)", status);
+    return PLK_OK;
+}
+
+plk_status plk_parsimony_branch(plk_context* ctx, int id, const plk_op* ops, int nops, plk_end node,
+                                plk_end dad, double* score, double* branch_subst) {
+    Instance* in = get_instance(ctx, id);
+    if (!in) return fail(PLK_ERR_ARG, "invalid instance");
+    if (!in->patterns_set) return fail(PLK_ERR_STATE, "patterns not set");
+    if (!score) return fail(PLK_ERR_ARG, "null score");
+    plk_status st = check_ops(in, ops, nops);
+    if (st != PLK_OK) return st;
+    if ((st = check_end(in, node, true)) != PLK_OK) return st;
+    if ((st = check_end(in, dad, true)) != PLK_OK) return st;
+    if (node.is_tip && dad.is_tip && in->d.ntaxa > 2) return fail(PLK_ERR_ARG, "branch between two tips");
+    if (in->d.nstates == kCodonN && in->d.ntaxa > 65535)
+        return fail(PLK_ERR_UNSUPPORTED, "%d taxa: codon change counts are 16-bit", in->d.ntaxa);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int n = in->d.nstates;
+    const bool planes = n != kCodonN;
+    const size_t Ppad = in->Ppad, nw = Ppad / 32, nslots = in->d.nslots;
+    if (planes && !in->pars_planes) {
+        if ((st = dmalloc(ctx, in, &in->pars_tip_planes, (size_t)in->d.ntaxa * n * nw * 4)) != PLK_OK) return st;
+        if ((st = dmalloc(ctx, in, &in->pars_planes, nslots * n * nw * 4)) != PLK_OK) return st;
+        if ((st = dmalloc(ctx, in, &in->pars_counts, nslots * nw * 8)) != PLK_OK) return st;
+    }
+    if (!planes && !in->pars_sets) {
+        if ((st = dmalloc(ctx, in, &in->pars_sets, nslots * Ppad * 8)) != PLK_OK) return st;
+        if ((st = dmalloc(ctx, in, &in->pars_changes, nslots * Ppad * 2)) != PLK_OK) return st;
+    }
+    // levels: an op's level is one more than the level of the ops in this
+    // list that write its children's slots; ops of one level are independent.
+    // A list that writes a slot twice, or reads a slot a later op writes,
+    // keeps its order, one op per level.
+    std::vector<int> level(nops, 0), writer(nslots, -1);
+    bool ordered = true;
+    for (int o = 0; o < nops && ordered; o++) {
+        for (int k = 0; k < ops[o].nchildren; k++)
+            if (!ops[o].child[k].is_tip && writer[ops[o].child[k].id] >= 0)
+                level[o] = std::max(level[o], level[writer[ops[o].child[k].id]] + 1);
+        if (writer[ops[o].dad_slot] >= 0) ordered = false;
+        writer[ops[o].dad_slot] = o;
+    }
+    for (int o = 0; o < nops && ordered; o++)
+        for (int k = 0; k < ops[o].nchildren; k++)
+            if (!ops[o].child[k].is_tip && writer[ops[o].child[k].id] > o) ordered = false;
+    if (!ordered)
+        for (int o = 0; o < nops; o++) level[o] = o;
+    const int nlevels = nops ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+    std::vector<plk_op> sorted(nops);
+    std::vector<int> level_start(nlevels + 1, 0);
+    for (int o = 0; o < nops; o++) level_start[level[o] + 1]++;
+    for (int l = 0; l < nlevels; l++) level_start[l + 1] += level_start[l];
+    {
+        std::vector<int> fill(level_start.begin(), level_start.end() - 1);
+        for (int o = 0; o < nops; o++) sorted[fill[level[o]]++] = ops[o];
+    }
+    const size_t ops_bytes = round_up(sizeof(plk_op) * nops, 16), need = ops_bytes + sizeof(int) * (nlevels + 1);
+    if ((size_t)in->pars_ops_cap < need) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        dfree(ctx, in, in->pars_ops, in->pars_ops_cap);
+        in->pars_ops = nullptr;
+        in->pars_ops_cap = 0;
+        const size_t cap = std::max(need, round_up(sizeof(plk_op) * nslots, 16) + sizeof(int) * (nslots + 1));
+        if ((st = dmalloc(ctx, in, &in->pars_ops, cap)) != PLK_OK) return st;
+        in->pars_ops_cap = (int)cap;
+    }
+    int* d_level_start = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(in->pars_ops) + ops_bytes);
+    if (nops) HIP_TRY(hipMemcpyAsync(in->pars_ops, sorted.data(), sizeof(plk_op) * nops, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_level_start, level_start.data(), sizeof(int) * (nlevels + 1), hipMemcpyHostToDevice,
+                           ctx->stream));
+    ParsDev D;
+    D.P = in->d.npatterns;
+    D.Ppad = (int)Ppad;
+    D.nwords = (int)nw;
+    D.n = n;
+    D.SU = in->d.state_unknown;
+    D.seq = n == 4 ? 0 : (n == 20 ? 1 : 2);
+    D.states = in->states;
+    D.freq = in->freq;
+    D.tip_planes = in->pars_tip_planes;
+    D.planes = in->pars_planes;
+    D.sets = in->pars_sets;
+    D.counts = in->pars_counts;
+    D.changes = in->pars_changes;
+    if (planes && !in->pars_tips_ready) {
+        const int g = (int)(((size_t)in->d.ntaxa * nw + kThreads - 1) / kThreads);
+        if (n == 4) hipLaunchKernelGGL(k_pars_tip_planes<4>, dim3(g), dim3(kThreads), 0, ctx->stream, D, in->d.ntaxa);
+        else hipLaunchKernelGGL(k_pars_tip_planes<20>, dim3(g), dim3(kThreads), 0, ctx->stream, D, in->d.ntaxa);
+        HIP_TRY(hipGetLastError());
+        in->pars_tips_ready = true;
+    }
+    const int grid = planes ? (int)((nw + kParsWords - 1) / kParsWords) : (int)((Ppad + kThreads - 1) / kThreads);
+    const int pe = prof_begin(ctx);
+    if (n == 4)
+        hipLaunchKernelGGL(k_pars_levels<4>, dim3(grid), dim3(kThreads), 0, ctx->stream, D, in->pars_ops,
+                           d_level_start, nlevels, node, dad, in->block_sums);
+    else if (n == 20)
+        hipLaunchKernelGGL(k_pars_levels<20>, dim3(grid), dim3(kThreads), 0, ctx->stream, D, in->pars_ops,
+                           d_level_start, nlevels, node, dad, in->block_sums);
+    else
+        hipLaunchKernelGGL(k_pars_sets, dim3(grid), dim3(kThreads), 0, ctx->stream, D, in->pars_ops, nops, node,
+                           dad, in->block_sums);
+    HIP_TRY(hipGetLastError());
+    {
+        // algorithmic bytes per pattern: an internal end is read as its sets
+        // and count, a tip as its planes (DNA, protein) or state byte (codon);
+        // every op writes one set and count
+        const double set_b = planes ? n / 8.0 : 8.0, cnt_b = planes ? 8.0 / 32 : 2.0;
+        const double tip_b = planes ? n / 8.0 : 1.0;
+        double b = 0.0;
+        for (int o = 0; o < nops; o++)
+            for (int k = 0; k < 2; k++) b += ops[o].child[k].is_tip ? tip_b : set_b + cnt_b;
+        b += nops * (set_b + cnt_b);
+        for (const plk_end& e : {node, dad}) b += e.is_tip ? tip_b : set_b + cnt_b;
+        prof_end(ctx, pe, K_PARSIMONY, b * in->d.npatterns, 0.0);
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->block_sums, grid, 2, in->red_out);
+    HIP_TRY(hipGetLastError());
+    if ((st = fetch_scalars(ctx, in, 2)) != PLK_OK) return st;
+    *score = ctx->h_out[0];
+    if (branch_subst) *branch_subst = ctx->h_out[1];
     return PLK_OK;
 }
 

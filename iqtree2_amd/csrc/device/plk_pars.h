@@ -1,0 +1,220 @@
+// Fitch parsimony (tree/phylotreepars.cpp:32-300, computePartialParsimonyFast
+// and computeParsimonyBranchFast; the SIMD form tree/phylokernel.h:1465-2094).
+//
+// Fitch's step at a node whose two children hold state sets X and Y:
+//   Z = X & Y;  where Z is empty, Z = X | Y and the site costs one change.
+// The score of a branch is the changes below both ends plus the sites whose
+// two end sets are disjoint.
+//
+// DNA and protein keep the reference's bit-plane layout: per node, one 32-bit
+// word per state for every 32 patterns (bit b of plane s of word w: pattern
+// 32w+b admits state s), the n planes of a word adjacent ([slot][word][n], as
+// the reference's partial_pars). The reference expands each pattern into
+// `frequency` sites so that a popcount gives the score; here patterns stay
+// compressed and the empty-intersection mask is weighted by the pattern
+// frequencies, so the per-node cache is a weighted change count per word
+// ([slot][word], double) in place of the reference's one subtree score per
+// node. Both give the same integer score for integer frequencies. Tip planes
+// are built once per pattern set (the reference's computeTipPartialParsimony,
+// tree/phylotreepars.cpp:561-633).
+//
+// One thread per word walking the whole op list would chain one memory round
+// trip per op; instead the host groups the ops into levels (an op's level is
+// one more than its children's), and a workgroup owns kParsWords words and
+// runs each level's ops side by side (kThreads / kParsWords ops at a time),
+// with a barrier between levels. A wave then reads 8 ops x 8 adjacent words.
+//
+// Codons (61 states) keep one 64-bit set per pattern: 61 planes per 32
+// patterns hold the same 61 bits, so the layout costs nothing there, and a
+// thread per pattern keeps a wave's loads contiguous.
+#pragma once
+#include "plk_kernels.h"
+
+namespace plkdev {
+
+struct ParsDev {
+    int P, Ppad, nwords, n, SU, seq;   // seq: 0 DNA, 1 protein, 2 codon
+    const uint8_t* states;             // [ntaxa][Ppad]
+    const double* freq;                // [Ppad], 0 on padding
+    uint32_t* tip_planes;              // [ntaxa][nwords][n] (DNA, protein)
+    uint32_t* planes;                  // [nslots][nwords][n] (DNA, protein)
+    uint64_t* sets;                    // [nslots][Ppad] (codon)
+    double* counts;                    // [nslots][nwords] weighted changes in the subtree (DNA, protein)
+    uint16_t* changes;                 // [nslots][Ppad] changes in the subtree (codon)
+};
+
+// the Fitch set of an observed tip state: the state, the ambiguity set, or
+// every state for STATE_UNKNOWN (tree/phylotreepars.cpp:60-160)
+__device__ inline uint64_t tip_set(int n, int SU, int seq, int s) {
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    if (s < n) return 1ull << s;
+    if (s == SU) return all;
+    if (seq == 0) return (uint64_t)(s - 3) & all;   // DNA ambiguity: bit mask + 3
+    if (seq == 1) {                                  // protein B = N|D, Z = Q|E, J = I|L
+        const int lo = s == 20 ? 2 : (s == 21 ? 5 : 9);
+        return (1ull << lo) | (1ull << (lo + 1));
+    }
+    return all;
+}
+
+// weighted count of the patterns of word w whose bit is set in e
+__device__ inline double weigh(uint32_t e, const double* __restrict__ freq, int w) {
+    double s = 0.0;
+    while (e) {
+        s += freq[32 * w + __ffs(e) - 1];
+        e &= e - 1;
+    }
+    return s;
+}
+
+constexpr int kParsWords = 8;   // words per workgroup in k_pars_levels
+
+// N planes of one word as 16-byte vectors (N = 4: one, N = 20: five)
+template <int N>
+struct Planes {
+    uint4 v[N / 4];
+};
+
+template <int N>
+__device__ inline Planes<N> load_planes(const uint32_t* p) {
+    Planes<N> x;
+#pragma unroll
+    for (int k = 0; k < N / 4; k++) x.v[k] = reinterpret_cast<const uint4*>(p)[k];
+    return x;
+}
+
+template <int N>
+__device__ inline void store_planes(uint32_t* p, const Planes<N>& x) {
+#pragma unroll
+    for (int k = 0; k < N / 4; k++) reinterpret_cast<uint4*>(p)[k] = x.v[k];
+}
+
+// bits of the patterns whose two sets are disjoint
+template <int N>
+__device__ inline uint32_t disjoint(const Planes<N>& x, const Planes<N>& y) {
+    uint32_t any = 0u;
+#pragma unroll
+    for (int k = 0; k < N / 4; k++)
+        any |= (x.v[k].x & y.v[k].x) | (x.v[k].y & y.v[k].y) | (x.v[k].z & y.v[k].z) | (x.v[k].w & y.v[k].w);
+    return ~any;
+}
+
+// Tip planes: one thread per (tip, word) gathers its 32 patterns' sets.
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_pars_tip_planes(ParsDev D, int ntaxa) {
+    const size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= (size_t)ntaxa * D.nwords) return;
+    const int tip = (int)(t / D.nwords), w = (int)(t % D.nwords);
+    uint32_t pl[N];
+#pragma unroll
+    for (int s = 0; s < N; s++) pl[s] = 0u;
+    const uint8_t* st = D.states + (size_t)tip * D.Ppad + 32 * w;
+    for (int b = 0; b < 32; b++) {
+        const uint32_t set = (uint32_t)tip_set(N, D.SU, D.seq, st[b]);
+#pragma unroll
+        for (int s = 0; s < N; s++) pl[s] |= ((set >> s) & 1u) << b;
+    }
+    uint32_t* out = D.tip_planes + t * N;
+#pragma unroll
+    for (int s = 0; s < N; s++) out[s] = pl[s];
+}
+
+// The ops level by level, then the branch (node, dad). level_start[l] is the
+// first op of level l (ops sorted by level on the host). block_sums[2b]: the
+// tree score, [2b+1]: the changes on the branch itself.
+template <int N>
+__global__ void __launch_bounds__(kThreads)
+k_pars_levels(ParsDev D, const plk_op* __restrict__ ops, const int* __restrict__ level_start, int nlevels,
+              plk_end node, plk_end dad, double* __restrict__ block_sums) {
+    __shared__ double red[2 * kThreads / 64];
+    constexpr int kLanes = kThreads / kParsWords;   // ops side by side
+    const int w = blockIdx.x * kParsWords + (int)threadIdx.x % kParsWords;
+    const int lane = (int)threadIdx.x / kParsWords;
+    const bool live = w < D.nwords;
+    const size_t nw = D.nwords;
+    auto end_planes = [&](bool is_tip, int id) {
+        return load_planes<N>((is_tip ? D.tip_planes : D.planes) + ((size_t)id * nw + w) * N);
+    };
+    auto end_count = [&](bool is_tip, int id) { return is_tip ? 0.0 : D.counts[(size_t)id * nw + w]; };
+    for (int l = 0; l < nlevels; l++) {
+        if (live)
+            for (int o = level_start[l] + lane; o < level_start[l + 1]; o += kLanes) {
+                const plk_op op = ops[o];
+                const Planes<N> x = end_planes(op.child[0].is_tip, op.child[0].id);
+                const Planes<N> y = end_planes(op.child[1].is_tip, op.child[1].id);
+                const double c = end_count(op.child[0].is_tip, op.child[0].id) +
+                                 end_count(op.child[1].is_tip, op.child[1].id);
+                const uint32_t e = disjoint<N>(x, y);
+                Planes<N> z;
+#pragma unroll
+                for (int k = 0; k < N / 4; k++) {
+                    const uint4 a = x.v[k], b = y.v[k];
+                    z.v[k].x = (a.x & b.x) | (e & (a.x | b.x));
+                    z.v[k].y = (a.y & b.y) | (e & (a.y | b.y));
+                    z.v[k].z = (a.z & b.z) | (e & (a.z | b.z));
+                    z.v[k].w = (a.w & b.w) | (e & (a.w | b.w));
+                }
+                store_planes<N>(D.planes + ((size_t)op.dad_slot * nw + w) * N, z);
+                D.counts[(size_t)op.dad_slot * nw + w] = c + weigh(e, D.freq, w);
+            }
+        __syncthreads();   // this level's sets visible to the next
+    }
+    double score = 0.0, bsub = 0.0;
+    if (live && lane == 0) {
+        bsub = weigh(disjoint<N>(end_planes(node.is_tip, node.id), end_planes(dad.is_tip, dad.id)), D.freq, w);
+        score = end_count(node.is_tip, node.id) + end_count(dad.is_tip, dad.id) + bsub;
+    }
+    double v[2] = {score, bsub};
+    block_sum<2, false>(v, red);
+    if (threadIdx.x == 0) {
+        block_sums[2 * blockIdx.x] = v[0];
+        block_sums[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// Codons: one thread per pattern, one 64-bit set and the number of changes
+// in the subtree per pattern, weighted by the frequency at the branch.
+__device__ inline void pars_end64(const ParsDev& D, plk_end e, int p, uint64_t& set, int& c) {
+    if (e.is_tip) {
+        set = tip_set(D.n, D.SU, D.seq, D.states[(size_t)e.id * D.Ppad + p]);
+        c = 0;
+    } else {
+        set = D.sets[(size_t)e.id * D.Ppad + p];
+        c = D.changes[(size_t)e.id * D.Ppad + p];
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_pars_sets(ParsDev D, const plk_op* __restrict__ ops, int nops, plk_end node, plk_end dad,
+            double* __restrict__ block_sums) {
+    __shared__ double red[2 * kThreads / 64];
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    double score = 0.0, bsub = 0.0;
+    if (p < D.Ppad) {
+        const double f = D.freq[p];
+        for (int o = 0; o < nops; o++) {
+            const plk_op op = ops[o];
+            uint64_t a, b;
+            int ca, cb;
+            pars_end64(D, plk_end{op.child[0].is_tip, op.child[0].id}, p, a, ca);
+            pars_end64(D, plk_end{op.child[1].is_tip, op.child[1].id}, p, b, cb);
+            const uint64_t both = a & b;
+            D.sets[(size_t)op.dad_slot * D.Ppad + p] = both ? both : (a | b);
+            D.changes[(size_t)op.dad_slot * D.Ppad + p] = (uint16_t)(ca + cb + (both ? 0 : 1));
+        }
+        uint64_t a, b;
+        int ca, cb;
+        pars_end64(D, node, p, a, ca);
+        pars_end64(D, dad, p, b, cb);
+        bsub = (a & b) ? 0.0 : f;
+        score = (double)(ca + cb) * f + bsub;
+    }
+    double v[2] = {score, bsub};
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) {
+        block_sums[2 * blockIdx.x] = v[0];
+        block_sums[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+}  // namespace plkdev

@@ -330,6 +330,23 @@ int plkh_tree_optimize_all_branches(plkh_tree* t, int iterations, double toleran
     return t->t->optimize_all_branches(iterations, tolerance, max_nr, lnl, n_derv);
 }
 
+int plkh_tree_plan_full(plkh_tree* t, int a, int b, plk_op* ops, int cap, int* nops, plk_end* node,
+                        plk_end* dad) {
+    if (!t || !nops || !node || !dad || cap < 0 || (cap > 0 && !ops) || !is_branch(t->t, a, b))
+        return PLK_ERR_ARG;
+    std::vector<plk_op> v;
+    t->t->plan_full(a, b, v, *node, *dad);
+    *nops = (int)v.size();
+    if ((int)v.size() > cap) return PLK_ERR_ARG;
+    if (!v.empty()) std::memcpy(ops, v.data(), sizeof(plk_op) * v.size());
+    return PLK_OK;
+}
+
+int plkh_tree_parsimony_branch(plkh_tree* t, int a, int b, double* score, double* branch_subst) {
+    if (!t || !score || !is_branch(t->t, a, b)) return PLK_ERR_ARG;
+    return t->t->parsimony_branch(a, b, score, branch_subst);
+}
+
 int plkh_supertree_optimize_all_branches(plkh_tree* const* trees, int ntrees, int iterations,
                                          double tolerance, int max_nr, double* lnl,
                                          int32_t* n_derv, int32_t* status) {

@@ -9,6 +9,7 @@ PhyloTreeGPU::PhyloTreeGPU(plk_context* ctx, const Workload& w, int precision, i
     : tree_(w.tree), initial_len_(w.tree.len), ctx_(ctx) {
     computed_.assign((size_t)tree_.nnodes * 3, 0);
     owner_.assign(tree_.nnodes, -1);
+    pars_dir_.assign(tree_.nnodes, -1);
     size_.assign((size_t)tree_.nnodes * 3, 0);
     const ModelParams& m = w.model;
     nstates_ = m.nstates;
@@ -84,6 +85,54 @@ void PhyloTreeGPU::plan_rec(int node, int dad, std::vector<plk_op>& ops, std::ve
         if (edges) edges->push_back(eid(node, ch));
     }
     ops.push_back(op);
+}
+
+void PhyloTreeGPU::plan_full_rec(int node, int dad, std::vector<plk_op>& ops, bool only_invalid) {
+    if (node < tree_.ntaxa || (only_invalid && pars_dir_[node] == dad)) return;
+    int children[2], nc = 0;
+    for (int j = 0; j < tree_.deg[node]; j++) {
+        const int ch = tree_.adj[3 * node + j];
+        if (ch != dad && nc < 2) children[nc++] = ch;
+    }
+    if (nc == 2 && subtree_size(children[0], node) < subtree_size(children[1], node))
+        std::swap(children[0], children[1]);
+    for (int i = 0; i < nc; i++) plan_full_rec(children[i], node, ops, only_invalid);
+    plk_op op{};
+    op.dad_slot = slot(node);
+    op.nchildren = nc;
+    for (int i = 0; i < nc; i++) {
+        const int ch = children[i];
+        op.child[i].is_tip = ch < tree_.ntaxa;
+        op.child[i].id = ch < tree_.ntaxa ? ch : slot(ch);
+        op.child[i].length = tree_.length(node, ch);
+    }
+    ops.push_back(op);
+    if (only_invalid) pars_dir_[node] = dad;   // the op about to run fills it
+}
+
+void PhyloTreeGPU::plan_full(int a, int b, std::vector<plk_op>& ops, plk_end& node, plk_end& dad) {
+    int nd = a, dd = b;
+    if (nd < tree_.ntaxa) std::swap(nd, dd);
+    ops.clear();
+    plan_full_rec(nd, dd, ops);
+    plan_full_rec(dd, nd, ops);
+    node = nd < tree_.ntaxa ? plk_end{1, nd} : plk_end{0, slot(nd)};
+    dad = dd < tree_.ntaxa ? plk_end{1, dd} : plk_end{0, slot(dd)};
+}
+
+int PhyloTreeGPU::parsimony_branch(int a, int b, double* score, double* branch_subst) {
+    if (!ctx_) return PLK_ERR_STATE;
+    int nd = a, dd = b;
+    if (nd < tree_.ntaxa) std::swap(nd, dd);
+    std::vector<plk_op> ops;
+    plan_full_rec(nd, dd, ops, true);
+    plan_full_rec(dd, nd, ops, true);
+    const plk_end node = nd < tree_.ntaxa ? plk_end{1, nd} : plk_end{0, slot(nd)};
+    const plk_end dad = dd < tree_.ntaxa ? plk_end{1, dd} : plk_end{0, slot(dd)};
+    const int st = plk_parsimony_branch(ctx_, inst_, ops.data(), (int)ops.size(), node, dad, score,
+                                        branch_subst);
+    if (st != PLK_OK) std::fill(pars_dir_.begin(), pars_dir_.end(), -1);
+    return st;
 }
 
 void PhyloTreeGPU::plan_sweep(const std::vector<int>& nodes, const std::vector<int>& nodes2,
@@ -400,6 +449,7 @@ int PhyloTreeGPU::nni_begin(int node1, int node2) {
         t.deg[node1] != 3 || t.deg[node2] != 3 || t.neighbor_index(node1, node2) < 0)
         return PLK_ERR_ARG;
     nni_saved_tree_ = tree_;
+    std::fill(pars_dir_.begin(), pars_dir_.end(), -1);   // slot mapping changes
     for (int k = 0; k < 3; k++) {
         nni_saved_comp_[0][k] = computed_[(size_t)node1 * 3 + k];
         nni_saved_comp_[1][k] = computed_[(size_t)node2 * 3 + k];
@@ -429,6 +479,7 @@ int PhyloTreeGPU::nni_swap(int node1, int x, int node2, int y) {
     Tree& t = tree_;
     const int k1 = t.neighbor_index(node1, x), k2 = t.neighbor_index(node2, y);
     if (!nni_active() || k1 < 0 || k2 < 0 || x == node2 || y == node1) return PLK_ERR_ARG;
+    std::fill(pars_dir_.begin(), pars_dir_.end(), -1);   // topology changes
     const int kx = t.neighbor_index(x, node1), ky = t.neighbor_index(y, node2);
     const int e1 = t.edge[3 * node1 + k1], e2 = t.edge[3 * node2 + k2];
     const double l1 = t.len[3 * node1 + k1], l2 = t.len[3 * node2 + k2];
@@ -471,6 +522,7 @@ void PhyloTreeGPU::nni_end() {
     const int node1 = nni_node_[0], node2 = nni_node_[1];
     tree_ = nni_saved_tree_;
     std::fill(size_.begin(), size_.end(), 0);
+    std::fill(pars_dir_.begin(), pars_dir_.end(), -1);
     for (int k = 0; k < 3; k++) {
         computed_[(size_t)node1 * 3 + k] = nni_saved_comp_[0][k];
         computed_[(size_t)node2 * 3 + k] = nni_saved_comp_[1][k];

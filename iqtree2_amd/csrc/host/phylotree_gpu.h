@@ -90,9 +90,19 @@ public:
     int nni_evaluate_all(int max_nr, std::vector<double>& lnl, std::vector<double>& lens,
                          std::vector<int>& nd);
 
+    // ---- Fitch parsimony (tree/phylotreepars.cpp:32-300) ----
+    // the full post-order op list for branch (a, b), every partial included
+    // and no likelihood partial marked (parsimony sets are kept apart)
+    void plan_full(int a, int b, std::vector<plk_op>& ops, plk_end& node, plk_end& dad);
+    // computeParsimonyBranchFast at (a, b): frequency-weighted changes of
+    // the tree and of the branch itself; only the slots not yet holding the
+    // sets for the needed direction are recomputed (partial_lh_computed & 2)
+    int parsimony_branch(int a, int b, double* score, double* branch_subst);
+
     double min_len = 1e-6, max_len = 10.0;
 
 private:
+    void plan_full_rec(int node, int dad, std::vector<plk_op>& ops, bool only_invalid = false);
     void plan_rec(int node, int dad, std::vector<plk_op>& ops, std::vector<int>* edges = nullptr);
     int subtree_size(int node, int dad);
     int farthest_leaf(int node, int dad, std::vector<int>& height);
@@ -108,6 +118,7 @@ private:
     bool safe_ = false;
     std::vector<char> computed_;   // [nnodes*3]: partial of v seen from adj[v][k]
     std::vector<int> owner_;       // [nnodes]: orientation holding v's buffer
+    std::vector<int> pars_dir_;    // [nnodes]: neighbour v's parsimony sets point away from; -1 none
     std::vector<int> size_;        // [nnodes*3] leaves below v away from adj[v][k]; 0 = unknown
     int theta_a_ = -1, theta_b_ = -1;   // branch whose theta buffer is current
     int cur_a_ = -1, cur_b_ = -1;       // current_it of PhyloTree

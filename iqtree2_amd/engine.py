@@ -188,9 +188,9 @@ class Context:
 
     def profile_stop(self):
         """Per-kernel {launches, total_ms, bytes, flops} since profile_start."""
-        stats = (_capi.PlkKernelStat * 16)()
+        stats = (_capi.PlkKernelStat * 32)()
         n = ctypes.c_int()
-        check(lib().plk_profile_stop(self._h, stats, 16, ctypes.byref(n)), "plk_profile_stop")
+        check(lib().plk_profile_stop(self._h, stats, 32, ctypes.byref(n)), "plk_profile_stop")
         return {s.name.decode(): dict(launches=s.launches, total_ms=s.total_ms, bytes=s.bytes,
                                       flops=s.flops) for s in stats[:n.value]}
 
@@ -250,6 +250,16 @@ class Instance:
     def __del__(self):
         if not shutting_down():
             self.close()
+
+    def parsimony_branch(self, ops, node, dad):
+        """Fitch parsimony (plk_parsimony_branch) after ops: (the tree's
+        frequency-weighted changes, the changes on branch (node, dad))."""
+        p, n = _ops_ptr(ops)
+        score, sub = ctypes.c_double(), ctypes.c_double()
+        check(lib().plk_parsimony_branch(self.ctx.handle, self.id, p, n, node, dad, ctypes.byref(score),
+                                         ctypes.byref(sub)),
+              "plk_parsimony_branch")
+        return score.value, sub.value
 
     def set_asc(self, nasc):
         """Lewis's ascertainment bias correction: the last nasc patterns are the
@@ -457,6 +467,26 @@ class PhyloTree:
                                    ctypes.byref(n), ctypes.byref(node), ctypes.byref(dad),
                                    ctypes.byref(length)), "plkh_tree_plan")
         return ops[:n.value].copy(), node, dad, length.value
+
+    def plan_full(self, a, b):
+        """The full post-order op list for branch (a, b), no partial marked
+        (parsimony): (ops, node, dad)."""
+        cap = 2 * self.workload.ntaxa
+        ops = ops_array(cap)
+        n = ctypes.c_int()
+        node, dad = PlkEnd(), PlkEnd()
+        check(lib().plkh_tree_plan_full(self._h, a, b, ops.ctypes.data_as(ctypes.c_void_p), cap,
+                                        ctypes.byref(n), ctypes.byref(node), ctypes.byref(dad)),
+              "plkh_tree_plan_full")
+        return ops[:n.value].copy(), node, dad
+
+    def parsimony_branch(self, a, b):
+        """computeParsimonyBranchFast at branch (a, b) on the device, only the
+        stale slots recomputed: (score, changes on the branch)."""
+        score, sub = ctypes.c_double(), ctypes.c_double()
+        check(lib().plkh_tree_parsimony_branch(self._h, a, b, ctypes.byref(score), ctypes.byref(sub)),
+              "plkh_tree_parsimony_branch")
+        return score.value, sub.value
 
     def compute_likelihood(self, want_pattern_lh=False):
         lnl = ctypes.c_double()

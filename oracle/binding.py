@@ -37,6 +37,9 @@ _PROTOS = {
                                        ctypes.POINTER(c_int)]),
     "orc_read_partial": (None, [c_void_p, c_int, _DP, ctypes.POINTER(ctypes.c_int32)]),
     "orc_set_asc": (None, [c_void_p, c_int]),
+    "orc_parsimony_branch": (c_double, [c_void_p, c_void_p, c_int, OrcEnd, OrcEnd,
+                                        ctypes.POINTER(ctypes.c_int32), _DP]),
+    "orc_parsimony_fast": (c_double, [c_void_p, c_void_p, c_int, OrcEnd, OrcEnd, _DP]),
 }
 
 _lib = None
@@ -159,6 +162,24 @@ class OracleTree:
                                       float(min_len), float(max_len), int(max_nr),
                                       ctypes.byref(lnl), ctypes.byref(nd), ctypes.byref(st))
         return x, lnl.value, nd.value, st.value
+
+    def parsimony_branch(self, ops, node, dad, want_pattern=False):
+        """(score, branch_subst), with each pattern's changes appended when
+        want_pattern."""
+        p, n = _ops(ops)
+        per = np.zeros(self.P, np.int32)
+        sub = ctypes.c_double()
+        score = lib().orc_parsimony_branch(self._h, p, n, _end(node), _end(dad),
+                                           per.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                           ctypes.byref(sub))
+        return (score, sub.value, per) if want_pattern else (score, sub.value)
+
+    def parsimony_fast(self, ops, node, dad):
+        """(score, branch_subst) by the bit-parallel form."""
+        p, n = _ops(ops)
+        sub = ctypes.c_double()
+        score = lib().orc_parsimony_fast(self._h, p, n, _end(node), _end(dad), ctypes.byref(sub))
+        return score, sub.value
 
     def read_partial(self, slot, partial=True):
         part = np.zeros((self.P, self.block)) if partial else None

@@ -42,6 +42,14 @@ struct orc_tree {
     double* info;          /* per-op echild + leaf tables */
     size_t info_cap;
     double* tipbuf;        /* per-packet tip staging [npackets][2*block*VS] */
+    uint64_t* pars_set;    /* Fitch state sets [nslots][Ppad] */
+    uint16_t* pars_changes;/* changes below each slot [nslots][Ppad] */
+    /* bit-plane form (computePartialParsimonyFast): sites expanded by
+     * frequency, blocks of 256 sites, [block][state][8] words */
+    int pf_nblocks;        /* 0: not built */
+    uint32_t* pf_tip;      /* [ntaxa][nblocks][n][8] */
+    uint32_t* pf_node;     /* [nslots][nblocks][n][8] */
+    int64_t* pf_score;     /* [nslots] subtree score */
 };
 
 static void* amalloc(size_t bytes) {
@@ -326,10 +334,16 @@ void orc_destroy(orc_tree* t) {
     free(t->inv_evec); free(t->tip); free(t->rates); free(t->props); free(t->partial);
     free(t->scale); free(t->theta); free(t->buffer_scale); free(t->pattern_lh); free(t->info);
     free(t->tipbuf);
+    free(t->pars_set);
+    free(t->pars_changes);
+    free(t->pf_tip);
+    free(t->pf_node);
+    free(t->pf_score);
     free(t);
 }
 
 void orc_set_patterns(orc_tree* t, const uint8_t* states, const double* freq, const double* invar) {
+    t->pf_nblocks = 0;   /* bit planes rebuilt on the next parsimony_fast */
     for (int s = 0; s < t->ntaxa; s++)
         memcpy(t->states + (size_t)s * t->Ppad, states + (size_t)s * t->P, t->P);
     memcpy(t->freq, freq, (size_t)t->P * 8);
@@ -1007,4 +1021,169 @@ void orc_read_partial(orc_tree* t, int slot, double* partial, int32_t* scale) {
                 scale[p] = sc[p];
         }
     }
+}
+
+/* ---- Fitch parsimony (tree/phylotreepars.cpp:32-300) ---------------------- */
+
+/* the Fitch set of an observed tip state, as computePartialParsimonyFast fills
+ * its bit planes (tree/phylotreepars.cpp:60-160) */
+static uint64_t pars_tip_set(const orc_tree* t, int s) {
+    const int n = t->n;
+    const uint64_t all = n >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << n) - 1);
+    static const int ambi_aa[] = {2, 3, 5, 6, 9, 10};
+    if (s < n) return (uint64_t)1 << s;
+    if (s == t->SU) return all;
+    if (n == 4) return (uint64_t)(s - 3) & all;
+    if (n == 20) return ((uint64_t)1 << ambi_aa[2 * (s - 20)]) | ((uint64_t)1 << ambi_aa[2 * (s - 20) + 1]);
+    return all;
+}
+
+static void pars_end(const orc_tree* t, int is_tip, int id, size_t p, uint64_t* set, int* ch) {
+    if (is_tip) {
+        *set = pars_tip_set(t, t->states[(size_t)id * t->Ppad + p]);
+        *ch = 0;
+    } else {
+        *set = t->pars_set[(size_t)id * t->Ppad + p];
+        *ch = t->pars_changes[(size_t)id * t->Ppad + p];
+    }
+}
+
+/* computePartialParsimonyFast over the ops, then computeParsimonyBranchFast at
+ * (node, dad): the number of changes of every pattern, weighted by its
+ * frequency and summed */
+double orc_parsimony_branch(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                            int32_t* ptn_score, double* branch_subst) {
+    const size_t Pp = (size_t)t->Ppad;
+    if (!t->pars_set) {
+        t->pars_set = (uint64_t*)amalloc((size_t)t->nslots * Pp * 8);
+        t->pars_changes = (uint16_t*)amalloc((size_t)t->nslots * Pp * 2);
+    }
+    double total = 0.0, on_branch = 0.0;
+    for (size_t p = 0; p < (size_t)t->P; p++) {
+        for (int o = 0; o < nops; o++) {
+            const orc_op* op = &ops[o];
+            uint64_t a, b;
+            int ca, cb;
+            pars_end(t, op->child[0].is_tip, op->child[0].id, p, &a, &ca);
+            pars_end(t, op->child[1].is_tip, op->child[1].id, p, &b, &cb);
+            const uint64_t both = a & b;
+            t->pars_set[(size_t)op->dad_slot * Pp + p] = both ? both : (a | b);
+            t->pars_changes[(size_t)op->dad_slot * Pp + p] = (uint16_t)(ca + cb + (both ? 0 : 1));
+        }
+        uint64_t a, b;
+        int ca, cb;
+        pars_end(t, node.is_tip, node.id, p, &a, &ca);
+        pars_end(t, dad.is_tip, dad.id, p, &b, &cb);
+        const int score = ca + cb + ((a & b) ? 0 : 1);
+        if (ptn_score) ptn_score[p] = score;
+        total += score * t->freq[p];
+        if (!(a & b)) on_branch += t->freq[p];
+    }
+    if (branch_subst) *branch_subst = on_branch;
+    return total;
+}
+
+/* The reference's bit-parallel form (computePartialParsimonyFast and
+ * computeParsimonyBranchFast, tree/phylotreepars.cpp:32-300, the 256-bit SIMD
+ * variant tree/phylokernel.h:1465-2094): every pattern with changes possible
+ * (its tips' sets share no state; orderPatternByNumChars drops the invariant
+ * ones, alignment/alignment.cpp:1178-1227) is expanded into `frequency` sites,
+ * one bit per site in nstates planes; a node's planes are X & Y, or X | Y on
+ * the sites where that is empty, counted by popcount; each node keeps its
+ * subtree score. The timed CPU baseline of the device kernel; frequencies
+ * must be integers. Returns -1 when they are not. */
+
+static int pf_build(orc_tree* t) {
+    const int n = t->n;
+    int64_t nsites = 0;
+    uint64_t* common = (uint64_t*)amalloc(sizeof(uint64_t) * t->P);
+    for (int p = 0; p < t->P; p++) {
+        if (t->freq[p] != floor(t->freq[p]) || t->freq[p] < 0) {
+            free(common);
+            return -1;
+        }
+        uint64_t c = ~(uint64_t)0;
+        for (int s = 0; s < t->ntaxa; s++) c &= pars_tip_set(t, t->states[(size_t)s * t->Ppad + p]);
+        common[p] = c;
+        if (!c) nsites += (int64_t)t->freq[p];
+    }
+    const int nb = (int)((nsites + 255) / 256);
+    t->pf_nblocks = nb < 1 ? 1 : nb;
+    const size_t per = (size_t)t->pf_nblocks * n * 8;
+    free(t->pf_tip);
+    free(t->pf_node);
+    free(t->pf_score);
+    t->pf_tip = (uint32_t*)amalloc(sizeof(uint32_t) * per * t->ntaxa);
+    t->pf_node = (uint32_t*)amalloc(sizeof(uint32_t) * per * t->nslots);
+    t->pf_score = (int64_t*)amalloc(sizeof(int64_t) * t->nslots);
+    for (int s = 0; s < t->ntaxa; s++) {
+        uint32_t* x = t->pf_tip + per * s;
+        int64_t site = 0;
+        for (int p = 0; p < t->P; p++) {
+            if (common[p]) continue;
+            const uint64_t set = pars_tip_set(t, t->states[(size_t)s * t->Ppad + p]);
+            for (int64_t j = 0; j < (int64_t)t->freq[p]; j++, site++) {
+                const size_t b = (size_t)(site / 256), lane = (size_t)(site % 256) / 32;
+                const uint32_t bit = 1u << (site % 32);
+                for (int k = 0; k < n; k++)
+                    if ((set >> k) & 1) x[(b * n + k) * 8 + lane] |= bit;
+            }
+        }
+        /* trailing sites of the last block admit every state (no change) */
+        for (; site < (int64_t)t->pf_nblocks * 256; site++) {
+            const size_t b = (size_t)(site / 256), lane = (size_t)(site % 256) / 32;
+            for (int k = 0; k < n; k++) x[(b * n + k) * 8 + lane] |= 1u << (site % 32);
+        }
+    }
+    free(common);
+    return 0;
+}
+
+static const uint32_t* pf_end(const orc_tree* t, int is_tip, int id, int64_t* score) {
+    const size_t per = (size_t)t->pf_nblocks * t->n * 8;
+    *score = is_tip ? 0 : t->pf_score[id];
+    return is_tip ? t->pf_tip + per * id : t->pf_node + per * id;
+}
+
+double orc_parsimony_fast(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                          double* branch_subst) {
+    if (!t->pf_nblocks && pf_build(t) != 0) return -1.0;
+    const int n = t->n, nb = t->pf_nblocks, nth = t->nthreads > 0 ? t->nthreads : 1;
+    const size_t per = (size_t)nb * n * 8;
+    for (int o = 0; o < nops; o++) {
+        int64_t sx, sy, score = 0;
+        const uint32_t* x = pf_end(t, ops[o].child[0].is_tip, ops[o].child[0].id, &sx);
+        const uint32_t* y = pf_end(t, ops[o].child[1].is_tip, ops[o].child[1].id, &sy);
+        uint32_t* z = t->pf_node + per * ops[o].dad_slot;
+#pragma omp parallel for num_threads(nth) reduction(+ : score) if (nb > 25)
+        for (int b = 0; b < nb; b++) {
+            const size_t off = (size_t)b * n * 8;
+            uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < n; k++)
+                for (int l = 0; l < 8; l++) {
+                    z[off + k * 8 + l] = x[off + k * 8 + l] & y[off + k * 8 + l];
+                    w[l] |= z[off + k * 8 + l];
+                }
+            for (int l = 0; l < 8; l++) {
+                w[l] = ~w[l];
+                score += __builtin_popcount(w[l]);
+            }
+            for (int k = 0; k < n; k++)
+                for (int l = 0; l < 8; l++) z[off + k * 8 + l] |= w[l] & (x[off + k * 8 + l] | y[off + k * 8 + l]);
+        }
+        t->pf_score[ops[o].dad_slot] = score + sx + sy;
+    }
+    int64_t sx, sy, on = 0;
+    const uint32_t* x = pf_end(t, node.is_tip, node.id, &sx);
+    const uint32_t* y = pf_end(t, dad.is_tip, dad.id, &sy);
+#pragma omp parallel for num_threads(nth) reduction(+ : on) if (nb > 25)
+    for (int b = 0; b < nb; b++) {
+        const size_t off = (size_t)b * n * 8;
+        uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < n; k++)
+            for (int l = 0; l < 8; l++) w[l] |= x[off + k * 8 + l] & y[off + k * 8 + l];
+        for (int l = 0; l < 8; l++) on += __builtin_popcount(~w[l]);
+    }
+    if (branch_subst) *branch_subst = (double)on;
+    return (double)(sx + sy + on);
 }

@@ -68,6 +68,19 @@ double orc_optimize_branch(orc_tree* t, const orc_op* ops, int nops, orc_end nod
                            double* lnl, int* n_derv, int* status);
 void orc_read_partial(orc_tree* t, int slot, double* partial, int32_t* scale);
 
+/* Fitch parsimony: ops give the state sets and subtree change counts of their
+ * dad slots (kept apart from the partials), then the frequency-weighted
+ * number of changes of the tree at (node, dad); ptn_score (nullable) [P]
+ * each pattern's changes, branch_subst (nullable) the weighted changes on
+ * the branch (node, dad) itself */
+double orc_parsimony_branch(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                            int32_t* ptn_score, double* branch_subst);
+/* the same score by the reference's bit-parallel algorithm (sites expanded by
+ * frequency, planes of 32 sites, popcount; OpenMP over 256-site blocks);
+ * -1 when a frequency is not an integer */
+double orc_parsimony_fast(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                          double* branch_subst);
+
 /* minimizeNewton on a caller-supplied function (utils/optimization.cpp:422-499);
  * fderv(x, &f, &df) returns f and its derivative. status 0 ok, 7 no convergence */
 typedef void (*orc_fderv)(void* user, double x, double* f, double* df);

@@ -160,3 +160,53 @@ def indicators(seq_type, n, state_unknown):
             pair = {20: (2, 3), 21: (5, 6), 22: (9, 10)}[s]
             ind[s, list(pair)] = 1.0
     return ind
+
+
+def brute_force_parsimony(ntaxa, edges, states, indicators, branch=None):
+    """Per pattern, the least number of state changes over every assignment of
+    internal-node states (tiny trees only): an internal edge costs 1 when its
+    ends differ, a tip edge when the internal end's state is not among the
+    states the tip's character admits (its indicator row).
+
+    With branch=(a, b), also whether branch (a, b) itself carries a change
+    (0 or 1): 1 where forcing (a, b) to cost nothing raises the least cost of
+    the other edges, i.e. every optimal labelling changes on (a, b), which
+    happens exactly when the two sides' optimal state sets are disjoint
+    (Fitch's test at a root placed on that branch)."""
+    adj = adjacency(*edges)
+    internal = sorted(x for x in adj if x >= ntaxa)
+    n = indicators.shape[1]
+    P = states.shape[1]
+    admits = indicators[states] > 0                  # (ntaxa, P, n)
+    big = np.iinfo(np.int64).max // 4
+    best = np.full(P, big)
+    free, forced = np.full(P, big), np.full(P, big)
+    iedges = [(x, y) for x in internal for y, _ in adj[x] if y in internal and x < y]
+    tedges = [(x, y) for x in internal for y, _ in adj[x] if y < ntaxa]
+    key = None if branch is None else frozenset(branch)
+    for assign in itertools.product(range(n), repeat=len(internal)):
+        st = dict(zip(internal, assign))
+        cost = np.zeros(P, np.int64)
+        rest = np.zeros(P, np.int64)            # every edge but the branch
+        on = np.zeros(P, bool)                  # the branch's cost
+        for x, y in iedges:
+            c = int(st[x] != st[y])
+            cost += c
+            if frozenset((x, y)) == key:
+                on[:] = bool(c)
+            else:
+                rest += c
+        for x, leaf in tedges:
+            c = ~admits[leaf, :, st[x]]
+            cost += c
+            if frozenset((x, leaf)) == key:
+                on = c
+            else:
+                rest += c
+        best = np.minimum(best, cost)
+        if key is not None:
+            free = np.minimum(free, rest)
+            forced = np.minimum(forced, np.where(on, big, rest))
+    if key is None:
+        return best
+    return best, (forced > free).astype(np.int64)

@@ -2066,20 +2066,66 @@ plk_status plk_parsimony_branch(plk_context* ctx, int id, const plk_op* ops, int
         std::vector<int> fill(level_start.begin(), level_start.end() - 1);
         for (int o = 0; o < nops; o++) sorted[fill[level[o]]++] = ops[o];
     }
-    const size_t ops_bytes = round_up(sizeof(plk_op) * nops, 16), need = ops_bytes + sizeof(int) * (nlevels + 1);
+    // LDS form (DNA, protein, an ordered list): number the branch ends the
+    // call touches, inputs (tips, slots of earlier calls) first, then the
+    // slots the ops write; the widest word count whose working set fits
+    // 60 KB of LDS (two workgroups per CU), else the HBM form
+    std::vector<plk_end> inputs;
+    std::vector<int4> lops;
+    int pars_w = 0, bnode = -1, bdad = -1, nends = 0;
+    if (planes && ordered) {
+        std::vector<int> of_tip(in->d.ntaxa, -1), of_slot(nslots, -1);
+        auto input = [&](plk_end e) {
+            int& m = e.is_tip ? of_tip[e.id] : of_slot[e.id];
+            if (m < 0 && (e.is_tip || writer[e.id] < 0)) {
+                m = (int)inputs.size();
+                inputs.push_back(e);
+            }
+        };
+        for (const plk_op& op : sorted)
+            for (int k = 0; k < 2; k++) input(plk_end{op.child[k].is_tip, op.child[k].id});
+        input(node);
+        input(dad);
+        nends = (int)inputs.size();
+        for (const plk_op& op : sorted) of_slot[op.dad_slot] = nends++;
+        auto idx = [&](int is_tip, int id) { return is_tip ? of_tip[id] : of_slot[id]; };
+        lops.resize(nops);
+        for (int o = 0; o < nops; o++)
+            lops[o] = make_int4(idx(sorted[o].child[0].is_tip, sorted[o].child[0].id),
+                                idx(sorted[o].child[1].is_tip, sorted[o].child[1].id), of_slot[sorted[o].dad_slot],
+                                sorted[o].dad_slot);
+        bnode = idx(node.is_tip, node.id);
+        bdad = idx(dad.is_tip, dad.id);
+        for (int W : {8, 4, 2, 1})
+            if (pars_lds_bytes(n, W, nends) <= 60 * 1024 && nw % W == 0) {
+                pars_w = W;
+                break;
+            }
+    }
+    // one upload: sorted ops | level starts | LDS ops | LDS inputs
+    const size_t off_lvl = round_up(sizeof(plk_op) * nops, 16);
+    const size_t off_lops = off_lvl + round_up(sizeof(int) * (nlevels + 1), 16);
+    const size_t off_in = off_lops + round_up(sizeof(int4) * lops.size(), 16);
+    const size_t need = off_in + sizeof(plk_end) * inputs.size();
     if ((size_t)in->pars_ops_cap < need) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         dfree(ctx, in, in->pars_ops, in->pars_ops_cap);
         in->pars_ops = nullptr;
         in->pars_ops_cap = 0;
-        const size_t cap = std::max(need, round_up(sizeof(plk_op) * nslots, 16) + sizeof(int) * (nslots + 1));
+        const size_t cap = std::max(need, (size_t)64 * (in->d.ntaxa + nslots + 16));
         if ((st = dmalloc(ctx, in, &in->pars_ops, cap)) != PLK_OK) return st;
         in->pars_ops_cap = (int)cap;
     }
-    int* d_level_start = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(in->pars_ops) + ops_bytes);
-    if (nops) HIP_TRY(hipMemcpyAsync(in->pars_ops, sorted.data(), sizeof(plk_op) * nops, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d_level_start, level_start.data(), sizeof(int) * (nlevels + 1), hipMemcpyHostToDevice,
-                           ctx->stream));
+    std::vector<unsigned char> up(need);
+    if (nops) std::memcpy(up.data(), sorted.data(), sizeof(plk_op) * nops);
+    std::memcpy(up.data() + off_lvl, level_start.data(), sizeof(int) * (nlevels + 1));
+    if (!lops.empty()) std::memcpy(up.data() + off_lops, lops.data(), sizeof(int4) * lops.size());
+    if (!inputs.empty()) std::memcpy(up.data() + off_in, inputs.data(), sizeof(plk_end) * inputs.size());
+    HIP_TRY(hipMemcpyAsync(in->pars_ops, up.data(), need, hipMemcpyHostToDevice, ctx->stream));
+    unsigned char* dbase = reinterpret_cast<unsigned char*>(in->pars_ops);
+    const int* d_level_start = reinterpret_cast<const int*>(dbase + off_lvl);
+    const int4* d_lops = reinterpret_cast<const int4*>(dbase + off_lops);
+    const plk_end* d_inputs = reinterpret_cast<const plk_end*>(dbase + off_in);
     ParsDev D;
     D.P = in->d.npatterns;
     D.Ppad = (int)Ppad;
@@ -2101,9 +2147,28 @@ plk_status plk_parsimony_branch(plk_context* ctx, int id, const plk_op* ops, int
         HIP_TRY(hipGetLastError());
         in->pars_tips_ready = true;
     }
-    const int grid = planes ? (int)((nw + kParsWords - 1) / kParsWords) : (int)((Ppad + kThreads - 1) / kThreads);
+    const int grid = pars_w ? (int)(nw / pars_w)
+                     : planes ? (int)((nw + kParsWords - 1) / kParsWords) : (int)((Ppad + kThreads - 1) / kThreads);
     const int pe = prof_begin(ctx);
-    if (n == 4)
+    if (pars_w) {
+        const size_t smem = pars_lds_bytes(n, pars_w, nends);
+        const int nin = (int)inputs.size();
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), smem, ctx->stream, D, d_lops, d_level_start,
+                               nlevels, d_inputs, nin, nends, bnode, bdad, in->block_sums);
+        };
+        if (n == 4) {
+            if (pars_w == 8) go(k_pars_lds<4, 8>);
+            else if (pars_w == 4) go(k_pars_lds<4, 4>);
+            else if (pars_w == 2) go(k_pars_lds<4, 2>);
+            else go(k_pars_lds<4, 1>);
+        } else {
+            if (pars_w == 8) go(k_pars_lds<20, 8>);
+            else if (pars_w == 4) go(k_pars_lds<20, 4>);
+            else if (pars_w == 2) go(k_pars_lds<20, 2>);
+            else go(k_pars_lds<20, 1>);
+        }
+    } else if (n == 4)
         hipLaunchKernelGGL(k_pars_levels<4>, dim3(grid), dim3(kThreads), 0, ctx->stream, D, in->pars_ops,
                            d_level_start, nlevels, node, dad, in->block_sums);
     else if (n == 20)

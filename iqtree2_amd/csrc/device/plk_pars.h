@@ -172,6 +172,89 @@ k_pars_levels(ParsDev D, const plk_op* __restrict__ ops, const int* __restrict__
     }
 }
 
+// The same ops with the workgroup's working set in LDS. The host numbers
+// every branch end the call touches: inputs first (tips, and internal slots
+// computed by an earlier call), loaded once, then the slots the ops write.
+// LDS holds their planes ([end][W][N]), counts ([end][W]) and the W words'
+// pattern frequencies; a level reads and writes LDS only (results are also
+// written back to HBM), so a level costs an LDS round trip and a barrier
+// instead of an L2 round trip. lops[o] = (LDS end of child 0, of child 1,
+// LDS end of the dad, the dad's slot).
+template <int N, int W>
+__global__ void __launch_bounds__(kThreads)
+k_pars_lds(ParsDev D, const int4* __restrict__ lops, const int* __restrict__ level_start, int nlevels,
+           const plk_end* __restrict__ inputs, int nin, int nends, int bnode, int bdad,
+           double* __restrict__ block_sums) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[2 * kThreads / 64];
+    uint32_t* pl = reinterpret_cast<uint32_t*>(smem);                      // [nends][W][N]
+    double* cnt = reinterpret_cast<double*>(smem + (size_t)nends * W * N * 4);   // [nends][W]
+    double* fq = cnt + (size_t)nends * W;                                  // [W * 32]
+    const size_t nw = D.nwords;
+    const int w0 = blockIdx.x * W;
+    for (int t = threadIdx.x; t < nin * W * N; t += kThreads) {
+        const int e = t / (W * N), r = t - e * (W * N), word = r / N, st = r - word * N;
+        const plk_end in = inputs[e];
+        const uint32_t* src = in.is_tip ? D.tip_planes : D.planes;
+        pl[t] = src[((size_t)in.id * nw + w0 + word) * N + st];
+    }
+    for (int t = threadIdx.x; t < nin * W; t += kThreads) {
+        const plk_end in = inputs[t / W];
+        cnt[t] = in.is_tip ? 0.0 : D.counts[(size_t)in.id * nw + w0 + t % W];
+    }
+    for (int t = threadIdx.x; t < W * 32; t += kThreads) fq[t] = D.freq[32 * w0 + t];
+    __syncthreads();
+    constexpr int kLanes = kThreads / W;
+    const int word = (int)threadIdx.x % W, lane = (int)threadIdx.x / W;
+    auto planes_of = [&](int e) { return load_planes<N>(pl + ((size_t)e * W + word) * N); };
+    auto weigh_lds = [&](uint32_t e) {
+        double s = 0.0;
+        while (e) {
+            s += fq[32 * word + __ffs(e) - 1];
+            e &= e - 1;
+        }
+        return s;
+    };
+    for (int l = 0; l < nlevels; l++) {
+        for (int o = level_start[l] + lane; o < level_start[l + 1]; o += kLanes) {
+            const int4 op = lops[o];
+            const Planes<N> x = planes_of(op.x), y = planes_of(op.y);
+            const uint32_t e = disjoint<N>(x, y);
+            Planes<N> z;
+#pragma unroll
+            for (int k = 0; k < N / 4; k++) {
+                const uint4 a = x.v[k], b = y.v[k];
+                z.v[k].x = (a.x & b.x) | (e & (a.x | b.x));
+                z.v[k].y = (a.y & b.y) | (e & (a.y | b.y));
+                z.v[k].z = (a.z & b.z) | (e & (a.z | b.z));
+                z.v[k].w = (a.w & b.w) | (e & (a.w | b.w));
+            }
+            const double c = cnt[op.x * W + word] + cnt[op.y * W + word] + weigh_lds(e);
+            store_planes<N>(pl + ((size_t)op.z * W + word) * N, z);
+            cnt[op.z * W + word] = c;
+            store_planes<N>(D.planes + ((size_t)op.w * nw + w0 + word) * N, z);
+            D.counts[(size_t)op.w * nw + w0 + word] = c;
+        }
+        __syncthreads();   // this level's sets visible to the next
+    }
+    double score = 0.0, bsub = 0.0;
+    if (lane == 0) {
+        bsub = weigh_lds(disjoint<N>(planes_of(bnode), planes_of(bdad)));
+        score = cnt[bnode * W + word] + cnt[bdad * W + word] + bsub;
+    }
+    double v[2] = {score, bsub};
+    block_sum<2, false>(v, red);
+    if (threadIdx.x == 0) {
+        block_sums[2 * blockIdx.x] = v[0];
+        block_sums[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// LDS bytes of k_pars_lds
+__host__ __device__ inline size_t pars_lds_bytes(int n, int W, int nends) {
+    return (size_t)nends * W * n * 4 + (size_t)nends * W * 8 + (size_t)W * 32 * 8;
+}
+
 // Codons: one thread per pattern, one 64-bit set and the number of changes
 // in the subtree per pattern, weighted by the frequency at the branch.
 __device__ inline void pars_end64(const ParsDev& D, plk_end e, int p, uint64_t& set, int& c) {

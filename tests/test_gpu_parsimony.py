@@ -35,6 +35,27 @@ def test_parsimony_vs_oracle(ctx, seq_type, ntaxa, nsite):
         planner.close()
 
 
+def test_parsimony_hbm_form(ctx):
+    """The forms that keep the sets in HBM: a protein tree whose working set
+    does not fit the LDS form (380 taxa), and an op list that writes a slot
+    twice (run in its own order)."""
+    w = ia.Workload(seq_type=ia.PROTEIN, ntaxa=380, nsite=300, ncat=4, mean_len=0.1, alpha=0.6,
+                    seed=661)
+    print("parsimony protein 380 taxa", flush=True)
+    inst = ia.Instance.from_workload(ctx, w)
+    orc = OracleTree.from_workload(w)
+    planner = ia.PhyloTree(None, w)
+    try:
+        for a, b in planner.branches()[::40]:
+            ops, node, dad = planner.plan_full(a, b)
+            assert inst.parsimony_branch(ops, node, dad) == orc.parsimony_branch(ops, node, dad)
+            twice = np.concatenate([ops, ops[-1:]])
+            assert inst.parsimony_branch(twice, node, dad) == orc.parsimony_branch(twice, node, dad)
+    finally:
+        inst.close()
+        planner.close()
+
+
 @pytest.mark.parametrize("seq_type", [ia.DNA, ia.PROTEIN, ia.CODON])
 def test_parsimony_driver_incremental(ctx, seq_type):
     """Every branch in turn through the tree driver, which recomputes only

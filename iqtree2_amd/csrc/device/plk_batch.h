@@ -368,10 +368,12 @@ struct XcdSplit {
 // is drained; every workgroup ends after trying each segment's counter. MASK
 // names the instance classes the launch may meet; a tile's code path is
 // uniform across its workgroup.
-// The protein FP64 class is held to 3 waves per SIMD (<= 168 registers,
-// which it fits without spilling); the compiler's own choice left it at 2.
+// The protein classes are held to 3 (FP64, <= 168 registers) and 4 (FP32,
+// <= 128) waves per SIMD, which they fit without spilling; the compiler's
+// own choice left each one wave lower.
 template <int MASK>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MASK == BC_PROT64 ? 3 : 1)))
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(MASK == BC_PROT64 ? 3 : (MASK == BC_PROT32 ? 4 : 1))))
 k_batch(const BatchItem* __restrict__ items, const int* __restrict__ tile_item, XcdSplit xs,
         unsigned* __restrict__ counters, double* __restrict__ tile_sums) {
     extern __shared__ __align__(16) unsigned char smem[];

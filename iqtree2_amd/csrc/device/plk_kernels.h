@@ -33,6 +33,7 @@ template <> struct Prec<double> {
     using scale_t = uint16_t;
     // one rescale by 2^256 per underflowing node (tree/phylokernelnew.h:1993-2030)
     __device__ static bool underflow(double m) { return m < kScaleThreshold64; }
+    static constexpr double threshold = kScaleThreshold64;   // a block max at or above it never rescales
     __device__ static int exponent(double) { return 1; }
     __device__ static double apply(double v, int) { return ldexp(v, kScaleExp64); }
     // safe mode: a category scaled once more than the least-scaled one is
@@ -50,6 +51,7 @@ template <> struct Prec<float> {
     using scale_t = int32_t;
     // exact power-of-two rescale that brings the block max into [0.5, 1)
     __device__ static bool underflow(float m) { return m < kScaleThreshold32 && m > 0.0f; }
+    static constexpr float threshold = kScaleThreshold32;
     __device__ static int exponent(float m) { return -(ilogbf(m) + 1); }
     __device__ static float apply(float v, int k) { return ldexpf(v, k); }
     // counters are exponents, so categories differ by an exact power of two

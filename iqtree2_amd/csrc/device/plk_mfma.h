@@ -194,7 +194,12 @@ __device__ inline void prot_ops_tile(const InstDev& I, const double* __restrict_
 #pragma unroll
                         for (int ib = 0; ib < 2; ib++) d[ib] = M::mma(prod[xb][r], ub_sh[((ib * 2 + xb) * 4 + r) * 64 + lane], d[ib]);
                     }
-                // unscaled store and the per-(pattern, category) maxima
+                // unscaled store and the per-(pattern, category) maxima. Only
+                // whether a maximum is below the rescale threshold matters
+                // unless it is (and then its value, for the FP32 exponent):
+                // when every row of the register holds a value at or above
+                // the threshold (one ballot), each row records 1, which gives
+                // the same decisions; otherwise the rows are reduced exactly.
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     T* row = dst + (size_t)(p0 + M::row(g, r)) * block + c * N;
@@ -204,7 +209,10 @@ __device__ inline void prot_ops_tile(const InstDev& I, const double* __restrict_
                         row[16 + col] = d[1][r];
                         m = fmax(m, fabs(d[1][r]));
                     }
-                    m = row16_max(m);
+                    const unsigned long long big = __builtin_amdgcn_ballot_w64(m >= Prec<T>::threshold);
+                    const bool rows_big = (big & 0xFFFFull) && (big & 0xFFFF0000ull) &&
+                                          (big & 0xFFFF00000000ull) && (big >> 48);
+                    m = rows_big ? (T)1 : row16_max(m);
                     if (col == 0) mx[(s * C + c) * 16 + M::row(g, r)] = m;
                 }
             }

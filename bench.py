@@ -422,6 +422,69 @@ def cpu_parsimony(ia, w, budget_s, threads):
 
 
 # ---------------------------------------------------------------------------
+# ModelFinder's Gamma-shape loop (RateGamma::optimizeParameters): Brent over
+# the shape, one full traversal per evaluation
+def leg_modelfinder(ia, ctx, w, parts, nparts=100):
+    """The shape of the DNA 200 x 100k tree optimised on the device; then the
+    shapes of the first nparts partitions of the 500-partition workload, one
+    after another and all together (one batched traversal launch per round)."""
+    tree = ia.PhyloTree(ctx, w)
+    try:
+        tree.compute_likelihood()
+        ctx.synchronize()
+        t = time.perf_counter()
+        a, lnl, n = tree.optimize_gamma_shape()
+        dt = time.perf_counter() - t
+    finally:
+        tree.close()
+    sub = parts[:nparts]
+    alone = [ia.PhyloTree(ctx, p) for p in sub]
+    batch = [ia.PhyloTree(ctx, p) for p in sub]
+    try:
+        for x in alone + batch:
+            x.compute_likelihood()
+        ctx.synchronize()
+        t = time.perf_counter()
+        ref = [x.optimize_gamma_shape() for x in alone]
+        dts = time.perf_counter() - t
+        t = time.perf_counter()
+        ab, lb, nb, st = ia.models_optimize_gamma_shape(batch)
+        dtb = time.perf_counter() - t
+        same = all((ab[k], lb[k], nb[k]) == ref[k] for k in range(len(sub))) and not any(st)
+    finally:
+        for x in alone + batch:
+            x.close()
+    return {"dna_200x100k": {"shape": a, "lnl": lnl, "evaluations": n, "ms": 1e3 * dt,
+                             "ms_per_evaluation": 1e3 * dt / n},
+            "partitions": {"count": len(sub), "evaluations": int(sum(r[2] for r in ref)),
+                           "sequential_ms": 1e3 * dts, "batched_ms": 1e3 * dtb,
+                           "speedup": dts / dtb, "batched_identical_to_alone": bool(same)}}
+
+
+def cpu_modelfinder(ia, w, threads):
+    """The same Brent loop (oracle/brent.py) over the oracle's likelihood."""
+    from oracle import brent
+    from oracle.binding import OracleTree
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w, nthreads=threads)
+    try:
+        ops, node, dad, length = planner.plan(*planner.branches()[0])
+        m = w.model_arrays()
+
+        def neg_lnl(alpha):
+            rates, props = ia.gamma_rates(w.ncat, alpha, 0.2)
+            orc.set_model(m["eval"], m["evec"], m["inv_evec"], m["tip"], rates, props)
+            return -orc.likelihood_branch(ops, node, dad, length)
+        t = time.perf_counter()
+        a, lnl, n = brent.optimize_gamma_shape(neg_lnl, 0.5)
+        dt = time.perf_counter() - t
+    finally:
+        planner.close()
+    return {"shape": a, "lnl": lnl, "evaluations": n, "ms": 1e3 * dt, "cores": threads,
+            "cpu_model": cpu_model(), "kind": "port", "sample": f"one optimisation, {n} evaluations"}
+
+
+# ---------------------------------------------------------------------------
 # matrix-core traversals: protein 100 x 20,000 and codon 64 x 5,000
 def leg_mfma(ia, ctx, reps=3):
     from iqtree2_amd import workloads as wl
@@ -588,7 +651,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy", "nni",
-                                       "parsimony"),
+                                       "parsimony", "modelfinder"),
                     help="run one leg alone (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -621,6 +684,12 @@ def main():
                 if not args.no_cpu:
                     out["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
                 w.close()
+            elif args.only == "modelfinder":
+                w = wl.dna_200x100k()
+                out = leg_modelfinder(ia, ctx, w, wl.partitions_500(args.partitions))
+                if not args.no_cpu:
+                    out["cpu_baseline"] = cpu_modelfinder(ia, w, host_threads())
+                w.close()
             else:
                 out = leg_serving(ia, ctx, args.requests)
             print(json.dumps({args.only: out}), flush=True)
@@ -648,6 +717,9 @@ def main():
             dna["parsimony"] = leg_parsimony(ia, ctx, w)
             if not args.no_cpu:
                 dna["parsimony"]["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
+            dna["modelfinder_gamma"] = leg_modelfinder(ia, ctx, w, parts)
+            if not args.no_cpu:
+                dna["modelfinder_gamma"]["cpu_baseline"] = cpu_modelfinder(ia, w, host_threads())
             w.close()
             log(f"dna: {json.dumps(dna)}")
         if not args.no_mfma:

@@ -34,6 +34,11 @@ int plkh_decompose(int n, const double* Q, const double* pi, double* eval, doubl
 /* discrete-Gamma mean rates and weights with +I rescaling
  * (model/rategamma.cpp:98-171, model/rategammainvar.cpp:25-42) */
 int plkh_gamma_rates(int ncat, double alpha, double p_inv, double* rates, double* props);
+/* Optimization::minimizeOneDimen (utils/optimization.cpp:322-373, with
+ * brent_opt 185-320): the minimum of f(x, arg) on [xmin, xmax] from the
+ * guess; *x the abscissa, *fx f there, *nevals the evaluations of f */
+int plkh_minimize_one_dimen(double (*f)(double, void*), void* arg, double xmin, double xguess, double xmax,
+                            double tol, double* x, double* fx, int* nevals);
 /* eigen-basis tip vectors [SU+1][n] (tree/phylotreesse.cpp:455-466) */
 int plkh_tip_vectors(int seq_type, const double* inv_evec, double* tip);
 /* GY94 codon exchangeabilities (model/modelcodon.cpp:845-870) */
@@ -127,6 +132,23 @@ int plkh_tree_plan_full(plkh_tree* t, int a, int b, plk_op* ops, int cap, int* n
  * needed direction are recomputed (the reference's partial_lh_computed & 2);
  * an NNI swap drops them all. */
 int plkh_tree_parsimony_branch(plkh_tree* t, int a, int b, double* score, double* branch_subst);
+/* RateGamma::setGammaShape + computeRates (model/rategamma.cpp:98-171): the
+ * category rates of shape alpha (p_inv kept) into the tree's instance; every
+ * partial becomes stale */
+int plkh_tree_set_gamma_shape(plkh_tree* t, double alpha);
+/* RateGamma::optimizeParameters (model/rategamma.cpp:232-246): Brent over the
+ * shape in [0.02, 1000] from the current one, tolerance max(eps, 0.001), each
+ * evaluation a full traversal on the device; the tree keeps the optimum.
+ * *alpha the shape, *lnl the lnL there, *nevals the likelihood evaluations */
+int plkh_tree_optimize_gamma_shape(plkh_tree* t, double eps, double* alpha, double* lnl, int* nevals);
+/* optimize_gamma_shape of every tree (ModelFinder's candidate models of one
+ * alignment, ModelFactory::optimizeParameters, model/modelfactory.cpp:
+ * 1053-1135, or partitions; all on one context) advanced together: one
+ * batched traversal launch per round for the trees that need a likelihood.
+ * Each tree gets the shape, lnL and evaluation count it gets alone;
+ * status[i] per tree, the return value the first error. */
+int plkh_models_optimize_gamma_shape(plkh_tree* const* trees, int ntrees, double eps, double* alpha,
+                                     double* lnl, int32_t* nevals, int32_t* status);
 /* PhyloSuperTree::optimizeAllBranches (tree/phylosupertree.cpp:773-792):
  * optimizeAllBranches(iterations, tolerance / min(ntrees, 10), max_nr) on every
  * tree (all on one context), with the host Newton loop of each partition

@@ -7,11 +7,13 @@ from . import _capi
 from ._capi import PREC_F32, PREC_F64, PlkError, lib
 from .engine import (BatchPlan, Context, Instance, Patterns, PhyloTree, Workload, end,
                      batch_evaluate, likelihood_branch_batch, likelihood_derv_batch,
-                     ops_array, supertree_optimize_all_branches)
+                     ops_array, supertree_optimize_all_branches, minimize_one_dimen, gamma_rates,
+                     models_optimize_gamma_shape)
 
 DNA, PROTEIN, CODON = _capi.DNA, _capi.PROTEIN, _capi.CODON
 
 __all__ = ["BatchPlan", "Context", "Instance", "Patterns", "PhyloTree", "Workload", "end", "ops_array",
            "likelihood_branch_batch", "likelihood_derv_batch", "batch_evaluate",
-           "supertree_optimize_all_branches",
+           "supertree_optimize_all_branches", "minimize_one_dimen", "gamma_rates",
+           "models_optimize_gamma_shape",
            "PREC_F32", "PREC_F64", "PlkError", "lib", "DNA", "PROTEIN", "CODON"]

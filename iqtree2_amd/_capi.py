@@ -101,6 +101,7 @@ _DP = P(c_double)
 _U8 = P(ctypes.c_uint8)
 _I32 = P(ctypes.c_int32)
 _F32 = P(ctypes.c_float)
+BRENT_FN = ctypes.CFUNCTYPE(c_double, c_double, c_void_p)   # plkh_minimize_one_dimen's f(x, arg)
 
 # name -> (restype, argtypes)
 _PROTOS = {
@@ -153,6 +154,11 @@ _PROTOS = {
     "plkh_rate_matrix": (c_int, [c_int, _DP, _DP, _DP]),
     "plkh_decompose": (c_int, [c_int, _DP, _DP, _DP, _DP, _DP]),
     "plkh_gamma_rates": (c_int, [c_int, c_double, c_double, _DP, _DP]),
+    "plkh_minimize_one_dimen": (c_int, [BRENT_FN, c_void_p, c_double, c_double, c_double, c_double, _DP, _DP,
+                                        ctypes.POINTER(c_int)]),
+    "plkh_tree_set_gamma_shape": (c_int, [c_void_p, c_double]),
+    "plkh_tree_optimize_gamma_shape": (c_int, [c_void_p, c_double, _DP, _DP, ctypes.POINTER(c_int)]),
+    "plkh_models_optimize_gamma_shape": (c_int, [P(c_void_p), c_int, c_double, _DP, _DP, _I32, _I32]),
     "plkh_tip_vectors": (c_int, [c_int, _DP, _DP]),
     "plkh_codon_exchangeabilities": (c_int, [c_double, c_double, _DP]),
     "plkh_patterns_build": (c_void_p, [c_int, c_int, c_int, c_char_p, P(c_int)]),

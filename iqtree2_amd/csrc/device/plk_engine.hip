@@ -120,6 +120,8 @@ struct Instance {
     double* sw_res = nullptr;         // per-branch lengths, then requested lnL values
     size_t sw_res_cap = 0;
     long long* sw_timing = nullptr;   // phase ticks of the last sweeps (PLK_SWEEP_TIMING)
+    unsigned char* sw_tabs = nullptr; // codon sweep: two steps' child tables
+    size_t sw_tabs_cap = 0;
     void* tables = nullptr;
     bool patterns_set = false, model_set = false, theta_ready = false, ptn_lh_ready = false;
     double theta_length = 0.0;
@@ -340,7 +342,7 @@ plk_status destroy_instance(plk_context* ctx, int id) {
     void* bufs[] = {in->states, in->freq, in->invar, in->model, in->partial, in->scale,
                     in->theta, in->buffer_scale, in->ptn_lh, in->block_sums, in->red_out,
                     in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
-                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res,
+                    in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res, in->sw_tabs,
                     in->pars_tip_planes, in->pars_planes, in->pars_counts, in->pars_sets,
                     in->pars_changes, in->pars_ops};
     for (void* b : bufs) dfree(ctx, nullptr, b, 0);
@@ -1258,7 +1260,9 @@ plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need)
 
 template <typename T, int N, int C>
 plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& grid_out) {
-    constexpr int kLdsMax = 64 * 1024;
+    // codon: the codon tile's E and U^-1 need more than 64 KB in FP64, and
+    // one workgroup per CU is enough for its 64-pattern tiles
+    const int kLdsMax = N == kCodonN && ctx->lds_per_cu > 64 * 1024 ? ctx->lds_per_cu : 64 * 1024;
     const SweepSmem L0(N, C, in->d.state_unknown, in->sweep_table_size, a.nedges, (int)sizeof(T), 0);
     if (L0.total > kLdsMax) return fail(PLK_ERR_UNSUPPORTED, "sweep LDS need %d bytes", L0.total);
     int nb = 0;
@@ -1317,9 +1321,12 @@ plk_status launch_sweep_n(plk_context* ctx, Instance* in, SweepArgs& a, int& gri
     case 1: return launch_sweep_t<T, N, 1>(ctx, in, a, grid);
     case 2: return launch_sweep_t<T, N, 2>(ctx, in, a, grid);
     case 4: return launch_sweep_t<T, N, 4>(ctx, in, a, grid);
-    case 8: return launch_sweep_t<T, N, 8>(ctx, in, a, grid);
+    case 8:
+        // codon tiles are 64 patterns, the sweep's tiles 256/C: C <= 4
+        if constexpr (N != kCodonN) return launch_sweep_t<T, N, 8>(ctx, in, a, grid);
+        break;
     }
-    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+    return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built for %d states in the sweep", in->d.ncat, N);
 }
 
 plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid) {
@@ -1327,6 +1334,8 @@ plk_status launch_sweep(plk_context* ctx, Instance* in, SweepArgs& a, int& grid)
     switch (in->d.nstates) {
     case 4: return f32 ? launch_sweep_n<float, 4>(ctx, in, a, grid) : launch_sweep_n<double, 4>(ctx, in, a, grid);
     case 20: return f32 ? launch_sweep_n<float, 20>(ctx, in, a, grid) : launch_sweep_n<double, 20>(ctx, in, a, grid);
+    case kCodonN:
+        return f32 ? launch_sweep_n<float, kCodonN>(ctx, in, a, grid) : launch_sweep_n<double, kCodonN>(ctx, in, a, grid);
     }
     return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
 }
@@ -1930,6 +1939,14 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     a.out = in->sw_out;
     a.timing = nullptr;
     a.lds_tiles = 0;
+    a.tabs = nullptr;
+    if (in->d.nstates == kCodonN) {
+        // codon: a step's tables live in HBM, two steps' worth
+        const size_t elem = in->d.precision == PLK_PREC_F32 ? 4 : 8;
+        if ((st = grow(ctx, in, &in->sw_tabs, in->sw_tabs_cap, 4 * (size_t)in->sweep_table_size * elem)) != PLK_OK)
+            return st;
+        a.tabs = in->sw_tabs;
+    }
     static const bool want_timing = std::getenv("PLK_SWEEP_TIMING") != nullptr;
     if (want_timing) {
         if (!in->sw_timing) {

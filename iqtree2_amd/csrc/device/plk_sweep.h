@@ -27,8 +27,15 @@
 // Every spin is bounded: a workgroup that waits too long raises a shared
 // abort flag, every spinning workgroup checks it, and all of them leave the
 // kernel, which then reports an error.
+//
+// 61-state (codon) models: a step's transition tables (C x 61 x 61 per
+// child) do not fit LDS, so the workgroups build them together into a
+// double-buffered HBM area, meet at one grid-wide exchange, and run the
+// step on their tiles with the codon matrix-core tile (plk_codon.h, one pass
+// per rate category); theta stays in HBM, and each thread streams its
+// (pattern, category)'s 61 states.
 #pragma once
-#include "plk_kernels.h"
+#include "plk_codon.h"
 
 namespace plkdev {
 
@@ -65,6 +72,7 @@ struct SweepArgs {
     long long* timing;    // nullable: wall-clock ticks of workgroup 0 per phase
     int lds_tiles;        // > 0: theta, buffer_scale, freq and invar of the workgroup's
                           // tiles (at most lds_tiles) live in LDS for the whole launch
+    void* tabs;           // codon: [2][2][table_size] tables of a step, by step parity
 };
 
 // phases accumulated in SweepArgs::timing (workgroup 0, thread 0)
@@ -189,6 +197,19 @@ struct SweepSmem {
                                   int lds_tiles) {
         const int pats = lds_tiles * (kThreads / C);   // patterns resident per workgroup
         int off = 0;
+        if (N == kCodonN) {
+            // codon: the codon tile's E and U^-1 (row stride kCodonStride),
+            // no tip or table copies (the tables are in HBM) and no resident theta
+            tip = ex = bs = fq = iv = th = 0;
+            val = off;  off += 3 * C * N * 8;
+            red = off;  off += 2 * kThreads * 8;
+            lens = off; off += nedges * 8;
+            ctl = off;  off += 8 * 8;
+            inv = off;  off += (N * kCodonStride * tsize + 15) / 16 * 16;
+            tab = off;  off += (N * kCodonStride * tsize + 15) / 16 * 16;
+            total = (off + 15) / 16 * 16;
+            return;
+        }
         tip = off;  off += (SU + 1) * N * 8;
         ex = off;   off += 2 * C * N * 8;
         val = off;  off += 3 * C * N * 8;
@@ -551,6 +572,221 @@ __device__ inline bool theta_first_eval(const SweepArgs& a, const SweepRes<T>& R
     return ok;
 }
 
+// ---- 61 states ----
+
+// 61 states and an even block (C even): the reference's derivative sums count
+// the last (category, state) term twice (buffer_block); so does the sweep's
+// derivative, whose lh then only serves the derivative ratios
+template <int C>
+constexpr bool kCodonTwiceLast = ((kCodonN * C) & 1) == 0;
+
+// Both children's tables of sweep op `op` into tb (k_prep_ops' arithmetic),
+// the element range split over the whole grid.
+template <typename T, int C>
+__device__ inline void codon_sweep_tables(const SweepArgs& a, const plk_sweep_op& op, const double* lens,
+                                          T* tb) {
+    constexpr int N = kCodonN;
+    const int SU = a.I.SU, ts = a.table_size;
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < 2 * (size_t)ts; t += stride) {
+        const int k = (int)(t / ts), r = (int)(t % ts);
+        const double len = lens[op.child[k].edge];
+        T* out = tb + (size_t)k * ts;
+        if (!op.child[k].is_tip) {
+            if (r >= C * N * N) continue;
+            const int i = r % N, x = (r / N) % N, c = r / (N * N);
+            out[r] = (T)(a.m.evec[x * N + i] * exp(a.m.eval[i] * (a.m.rates[c] * len)));
+        } else {
+            if (r >= (SU + 1) * C * N) continue;
+            const int x = r % N, c = (r / N) % C, s = r / (N * C);
+            double v;
+            if (s == SU) {
+                v = 1.0;
+            } else {
+                const double* ev = a.m.evec + x * N;
+                const double* tp = a.m.tip + s * N;
+                v = ev[0] * exp(a.m.eval[0] * (a.m.rates[c] * len)) * tp[0];
+                for (int i = 1; i < N; i++) v += ev[i] * exp(a.m.eval[i] * (a.m.rates[c] * len)) * tp[i];
+            }
+            out[r] = (T)v;
+        }
+    }
+}
+
+// theta of branch (node, dad) into HBM and the first (df, ddf) at x from the
+// stored values: one thread per (pattern, category), the 61 states streamed
+template <typename T, int C>
+__device__ inline bool codon_theta_first_eval(const SweepArgs& a, plk_end node, plk_end dad, double x,
+                                              unsigned& epoch, double* val_sh, double* red_sh, double* res) {
+    constexpr int N = kCodonN, PB = kThreads / C, block = N * C;
+    const InstDev& I = a.I;
+    for (int t = threadIdx.x; t < N * C; t += kThreads) {
+        const int cc = t / N, i = t % N;
+        const double cof = a.m.eval[i] * a.m.rates[cc];
+        const double v0 = exp(cof * x) * a.m.props[cc];
+        const double v1 = cof * v0;
+        val_sh[t] = v0;
+        val_sh[N * C + t] = v1;
+        val_sh[2 * N * C + t] = cof * v1;
+    }
+    __syncthreads();
+    using ST = typename Prec<T>::scale_t;
+    const T* partial = reinterpret_cast<const T*>(I.partial);
+    const ST* scale = reinterpret_cast<const ST*>(I.scale);
+    const int c = threadIdx.x % C;
+    const int ntiles = I.Ppad / PB;
+    double acc[2] = {0.0, 0.0};
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int p = tile * PB + threadIdx.x / C;
+        const size_t sidx = I.safe ? (size_t)p * C + c : (size_t)p;
+        const T* nv = partial + (size_t)node.id * I.slot_stride + (size_t)p * block + c * N;
+        int sc = scale[(size_t)node.id * I.scale_stride + sidx];
+        const T* dv = nullptr;
+        const double* tp = nullptr;
+        if (dad.is_tip) {
+            tp = a.m.tip + (size_t)I.states[(size_t)dad.id * I.Ppad + p] * N;
+        } else {
+            dv = partial + (size_t)dad.id * I.slot_stride + (size_t)p * block + c * N;
+            sc += scale[(size_t)dad.id * I.scale_stride + sidx];
+        }
+        int minsc = sc;
+        double adjust = 1.0;
+        if (I.safe) {
+            minsc = group_min_int<C>(sc);
+            adjust = Prec<T>::safe_adjust(sc - minsc);
+        }
+        T* th = reinterpret_cast<T*>(I.theta) + (size_t)p * block + c * N;
+        const double* v0 = val_sh + c * N;
+        const double* v1 = val_sh + N * C + c * N;
+        const double* v2 = val_sh + 2 * N * C + c * N;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        T last = (T)0;
+        for (int i = 0; i < N; i++) {
+            const double d = dad.is_tip ? tp[i] : (double)dv[i];
+            const T tv = (T)((double)nv[i] * d * adjust);
+            th[i] = tv;
+            last = tv;
+            const double t = (double)tv;
+            s0 = fma(v0[i], t, s0);
+            s1 = fma(v1[i], t, s1);
+            s2 = fma(v2[i], t, s2);
+        }
+        // the reference's derivative counts the last (category, state) term
+        // twice when the block is even (dotProductTriple; see buffer_block)
+        if (kCodonTwiceLast<C> && c == C - 1) {
+            const double t = (double)last;
+            s0 = fma(v0[N - 1], t, s0);
+            s1 = fma(v1[N - 1], t, s1);
+            s2 = fma(v2[N - 1], t, s2);
+        }
+        if (c == 0) I.buffer_scale[p] = (double)minsc * Prec<T>::log_unit;
+        s0 = group_sum_ordered<C>(s0);
+        s1 = group_sum_ordered<C>(s1);
+        s2 = group_sum_ordered<C>(s2);
+        if (c == 0 && p < I.P) {
+            const double f = I.freq[p];
+            const double lh = fabs(s0) + I.invar[p];
+            const double inv = 1.0 / lh;
+            const double df_frac = s1 * inv, ddf_frac = s2 * inv;
+            const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
+            acc[0] += tmp1;
+            acc[1] += tmp2 - tmp1 * df_frac;
+        }
+    }
+    block_sum<2, false>(acc, red_sh);
+    epoch++;
+    return grid_sum2(a, acc[0], acc[1], epoch, red_sh, res);
+}
+
+// sweep_eval for 61 states: theta read from HBM, the states streamed
+template <typename T, int C>
+__device__ inline bool codon_sweep_eval(const SweepArgs& a, int mode, double x, double x2, bool store_ptn,
+                                        unsigned& epoch, double* val_sh, double* red_sh, double* res) {
+    constexpr int N = kCodonN, PB = kThreads / C, block = N * C;
+    const InstDev& I = a.I;
+    for (int t = threadIdx.x; t < N * C; t += kThreads) {
+        const int cc = t / N, i = t % N;
+        const double cof = a.m.eval[i] * a.m.rates[cc];
+        if (mode == 0) {
+            const double v0 = exp(cof * x) * a.m.props[cc];
+            const double v1 = cof * v0;
+            val_sh[t] = v0;
+            val_sh[N * C + t] = v1;
+            val_sh[2 * N * C + t] = cof * v1;
+        } else {
+            val_sh[t] = exp(cof * x) * a.m.props[cc];
+            val_sh[N * C + t] = x2 >= 0.0 ? exp(cof * x2) * a.m.props[cc] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int c = threadIdx.x % C;
+    const int ntiles = I.Ppad / PB;
+    double acc[2] = {0.0, 0.0};
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int p = tile * PB + threadIdx.x / C;
+        const T* th = reinterpret_cast<const T*>(I.theta) + (size_t)p * block + c * N;
+        const double* w0 = val_sh + c * N;
+        const double* w1 = val_sh + N * C + c * N;
+        const double* w2 = val_sh + 2 * N * C + c * N;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < N; i++) {
+            const double t = (double)th[i];
+            s0 = fma(w0[i], t, s0);
+            s1 = fma(w1[i], t, s1);
+            if (mode == 0) s2 = fma(w2[i], t, s2);
+        }
+        if (mode == 0 && kCodonTwiceLast<C> && c == C - 1) {   // as in codon_theta_first_eval
+            const double t = (double)th[N - 1];
+            s0 = fma(w0[N - 1], t, s0);
+            s1 = fma(w1[N - 1], t, s1);
+            s2 = fma(w2[N - 1], t, s2);
+        }
+        s0 = group_sum_ordered<C>(s0);
+        s1 = group_sum_ordered<C>(s1);
+        if (mode == 0) s2 = group_sum_ordered<C>(s2);
+        if (c == 0 && p < I.P) {
+            const double f = I.freq[p], pinv = I.invar[p];
+            if (mode == 0) {
+                const double lh = fabs(s0) + pinv;
+                const double inv = 1.0 / lh;
+                const double df_frac = s1 * inv, ddf_frac = s2 * inv;
+                const double tmp1 = df_frac * f, tmp2 = ddf_frac * f;
+                acc[0] += tmp1;
+                acc[1] += tmp2 - tmp1 * df_frac;
+            } else {
+                const double bsc = I.buffer_scale[p];
+                const double lnp = log(fabs(s0) + pinv) + bsc;
+                if (store_ptn) I.ptn_lh[p] = lnp;
+                acc[0] += lnp * f;
+                if (x2 >= 0.0) acc[1] += (log(fabs(s1) + pinv) + bsc) * f;
+            }
+        }
+    }
+    block_sum<2, false>(acc, red_sh);
+    epoch++;
+    return grid_sum2(a, acc[0], acc[1], epoch, red_sh, res);
+}
+
+// the state count's theta / evaluation forms
+template <typename T, int N, int C>
+__device__ inline bool any_first_eval(const SweepArgs& a, const SweepRes<T>& R, bool write_global, plk_end node,
+                                      plk_end dad, double x, unsigned& epoch, const double* tip_sh, double* val_sh,
+                                      double* red_sh, double* res) {
+    if constexpr (N == kCodonN)
+        return codon_theta_first_eval<T, C>(a, node, dad, x, epoch, val_sh, red_sh, res);
+    else
+        return theta_first_eval<T, N, C>(a, R, write_global, node, dad, x, epoch, tip_sh, val_sh, red_sh, res);
+}
+
+template <typename T, int N, int C>
+__device__ inline bool any_eval(const SweepArgs& a, const SweepRes<T>& R, int mode, double x, double x2,
+                                bool store_ptn, unsigned& epoch, double* val_sh, double* red_sh, double* res) {
+    if constexpr (N == kCodonN)
+        return codon_sweep_eval<T, C>(a, mode, x, x2, store_ptn, epoch, val_sh, red_sh, res);
+    else
+        return sweep_eval<T, N, C>(a, R, mode, x, x2, store_ptn, epoch, val_sh, red_sh, res);
+}
+
 template <typename T, int N, int C>
 __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -568,8 +804,14 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     const int c = threadIdx.x % C;
     const int ntiles = I.Ppad / PB;
 
-    for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)a.m.inv_evec[t];
-    for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) tip_sh[t] = a.m.tip[t];
+    if constexpr (N == kCodonN) {
+        // the codon tile's U^-1 layout (row i, column x, stride kCodonStride)
+        for (int t = threadIdx.x; t < N * N; t += kThreads)
+            inv_sh[(t / N) * kCodonStride + t % N] = (T)a.m.inv_evec[t];
+    } else {
+        for (int t = threadIdx.x; t < N * N; t += kThreads) inv_sh[t] = (T)a.m.inv_evec[t];
+        for (int t = threadIdx.x; t < (I.SU + 1) * N; t += kThreads) tip_sh[t] = a.m.tip[t];
+    }
     for (int t = threadIdx.x; t < a.nedges; t += kThreads) lens[t] = a.len[t];
     SweepRes<T> R{a.lds_tiles > 0, (Lds<T>*)(smem + L.th), (Lds<double>*)(smem + L.bs),
                   (Lds<double>*)(smem + L.fq), (Lds<double>*)(smem + L.iv), a.lds_tiles * kThreads};
@@ -592,19 +834,57 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         const plk_sweep_branch B = a.br[j];
         long long tp = wall_clock64();
         // stale partials of this branch
-        for (int o = B.op_begin; o < B.op_begin + B.op_count; o++) {
-            const plk_sweep_op op = a.ops[o];
-            int left = 0, right = 1;
-            if (!op.child[0].is_tip && op.child[1].is_tip) { left = 1; right = 0; }
-            const int tips[2] = {op.child[left].is_tip, op.child[right].is_tip};
-            const double ln[2] = {lens[op.child[left].edge], lens[op.child[right].edge]};
-            build_tables<T, N, C>(a.m, I.SU, a.table_size, tips, ln, tab_sh, ex_sh);
-            for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-                const int p = tile * PB + threadIdx.x / C;
-                op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + a.table_size, tips[0],
-                                  op.child[left].id, tips[1], op.child[right].id, op.dad_slot, p, c);
+        if constexpr (N == kCodonN) {
+            // the step's tables built by the whole grid into the HBM buffer of
+            // the step's parity (the other buffer may still be read by a
+            // workgroup finishing the previous step), published with a
+            // device-scope fence before the exchange that every workgroup
+            // passes, and acquired after it; then the codon tile on the
+            // workgroup's own patterns, 64 at a time, one pass per category
+            __shared__ plk_op op_sh;
+            for (int o = B.op_begin; o < B.op_begin + B.op_count && status == 0; o++) {
+                const plk_sweep_op op = a.ops[o];
+                T* tb = reinterpret_cast<T*>(a.tabs) + (size_t)(o & 1) * 2 * a.table_size;
+                codon_sweep_tables<T, C>(a, op, lens, tb);
+                if (threadIdx.x == 0) {
+                    op_sh.dad_slot = op.dad_slot;
+                    op_sh.nchildren = 2;
+                    for (int k = 0; k < 2; k++) {
+                        op_sh.child[k].is_tip = op.child[k].is_tip;
+                        op_sh.child[k].id = op.child[k].id;
+                        op_sh.child[k].length = lens[op.child[k].edge];
+                    }
+                }
+                __threadfence();
+                __syncthreads();
+                epoch++;
+                if (!grid_sum2(a, 0.0, 0.0, epoch, red_sh, ctl)) {
+                    status = PLK_ERR_DEVICE;
+                    break;
+                }
+                __threadfence();
+                for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+                    for (int q = 0; q < PB / kCodonTile; q++)
+                        for (int cc = 0; cc < C; cc++)
+                            codon_ops_tile<T>(I, &op_sh, 1, tb, a.table_size, cc, tile * PB + q * kCodonTile,
+                                              tab_sh, inv_sh);
             }
-            __syncthreads();
+            if (status != 0) break;
+        } else {
+            for (int o = B.op_begin; o < B.op_begin + B.op_count; o++) {
+                const plk_sweep_op op = a.ops[o];
+                int left = 0, right = 1;
+                if (!op.child[0].is_tip && op.child[1].is_tip) { left = 1; right = 0; }
+                const int tips[2] = {op.child[left].is_tip, op.child[right].is_tip};
+                const double ln[2] = {lens[op.child[left].edge], lens[op.child[right].edge]};
+                build_tables<T, N, C>(a.m, I.SU, a.table_size, tips, ln, tab_sh, ex_sh);
+                for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+                    const int p = tile * PB + threadIdx.x / C;
+                    op_apply<T, N, C>(I, inv_sh, tab_sh, tab_sh + a.table_size, tips[0],
+                                      op.child[left].id, tips[1], op.child[right].id, op.dad_slot, p, c);
+                }
+                __syncthreads();
+            }
         }
         if (timed) a.timing[PH_OPS] += wall_clock64() - tp;
         // minimizeNewton(min_len, len, max_len, xacc = min_len, max_nr) on
@@ -620,7 +900,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         // the instance keeps the last branch's theta (plk_likelihood_derv
         // after the sweep); earlier ones live in LDS only
         const bool last_branch = j == a.nbranches - 1;
-        if (!theta_first_eval<T, N, C>(a, R, last_branch, B.node, B.dad, rts, epoch, tip_sh, val_sh,
+        if (!any_first_eval<T, N, C>(a, R, last_branch, B.node, B.dad, rts, epoch, tip_sh, val_sh,
                                        red_sh, ctl)) {
             status = PLK_ERR_DEVICE;
             break;
@@ -658,7 +938,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
                     if (temp == rts) { result = rts; done = true; break; }
                 }
                 if (fabs(dx) < xacc || it == a.max_nr) { result = rts_old; done = true; break; }
-                if (!sweep_eval<T, N, C>(a, R, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+                if (!any_eval<T, N, C>(a, R, 0, rts, -1.0, false, epoch, val_sh, red_sh, ctl)) {
                     status = PLK_ERR_DEVICE;
                     break;
                 }
@@ -680,7 +960,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         // divergence guard of optimizeOneBranch (tree/phylotree.cpp:2616-2625):
         // lnL at the last evaluated length against lnL at the old one
         if (optx > x2 * 0.95) {
-            if (!sweep_eval<T, N, C>(a, R, 1, xlast, len0, false, epoch, val_sh, red_sh, ctl)) {
+            if (!any_eval<T, N, C>(a, R, 1, xlast, len0, false, epoch, val_sh, red_sh, ctl)) {
                 status = PLK_ERR_DEVICE;
                 break;
             }
@@ -696,7 +976,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
         __syncthreads();
         // computeLikelihoodFromBuffer at the new length (current_it->length)
         if (B.lnl_index >= 0 && B.lnl_index < a.nlnl) {
-            if (!sweep_eval<T, N, C>(a, R, 1, optx, -1.0, false, epoch, val_sh, red_sh, ctl)) {
+            if (!any_eval<T, N, C>(a, R, 1, optx, -1.0, false, epoch, val_sh, red_sh, ctl)) {
                 status = PLK_ERR_DEVICE;
                 break;
             }
@@ -710,7 +990,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(SweepArgs a) {
     // computeLikelihoodFromBuffer at the last branch
     if (status == 0 && a.nbranches > 0) {
         const double last = lens[a.br[a.nbranches - 1].edge];
-        if (!sweep_eval<T, N, C>(a, R, 1, last, -1.0, true, epoch, val_sh, red_sh, ctl))
+        if (!any_eval<T, N, C>(a, R, 1, last, -1.0, true, epoch, val_sh, red_sh, ctl))
             status = PLK_ERR_DEVICE;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {

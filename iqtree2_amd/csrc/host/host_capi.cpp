@@ -3,6 +3,7 @@
 #include <string>
 
 #include "alignment.h"
+#include "brent.h"
 #include "model.h"
 #include "phylotree_gpu.h"
 #include "plk_host.h"
@@ -50,6 +51,18 @@ int plkh_gamma_rates(int ncat, double alpha, double p_inv, double* rates, double
     if (ncat < 1 || !(alpha > 0.0) || !(p_inv >= 0.0 && p_inv < 1.0) || !rates || !props)
         return PLK_ERR_ARG;
     gamma_mean_rates(ncat, alpha, p_inv, rates, props);
+    return PLK_OK;
+}
+
+int plkh_minimize_one_dimen(double (*f)(double, void*), void* arg, double xmin, double xguess, double xmax,
+                            double tol, double* x, double* fx, int* nevals) {
+    if (!f || !x || !(xmin <= xmax) || !(tol > 0.0)) return PLK_ERR_ARG;
+    BrentRun br;
+    br.start(xmin, xguess, xmax, tol);
+    while (!br.done()) br.feed(f(br.x(), arg));
+    *x = br.result();
+    if (fx) *fx = br.fx();
+    if (nevals) *nevals = br.evals();
     return PLK_OK;
 }
 
@@ -345,6 +358,33 @@ int plkh_tree_plan_full(plkh_tree* t, int a, int b, plk_op* ops, int cap, int* n
 int plkh_tree_parsimony_branch(plkh_tree* t, int a, int b, double* score, double* branch_subst) {
     if (!t || !score || !is_branch(t->t, a, b)) return PLK_ERR_ARG;
     return t->t->parsimony_branch(a, b, score, branch_subst);
+}
+
+int plkh_tree_set_gamma_shape(plkh_tree* t, double alpha) {
+    if (!t) return PLK_ERR_ARG;
+    return t->t->set_gamma_shape(alpha);
+}
+
+int plkh_tree_optimize_gamma_shape(plkh_tree* t, double eps, double* alpha, double* lnl, int* nevals) {
+    if (!t || !(eps >= 0.0)) return PLK_ERR_ARG;
+    return t->t->optimize_gamma_shape(eps, alpha, lnl, nevals);
+}
+
+int plkh_models_optimize_gamma_shape(plkh_tree* const* trees, int ntrees, double eps, double* alpha,
+                                     double* lnl, int32_t* nevals, int32_t* status) {
+    if (ntrees < 0 || (ntrees > 0 && !trees) || !(eps >= 0.0)) return PLK_ERR_ARG;
+    std::vector<PhyloTreeGPU*> ts(ntrees);
+    for (int i = 0; i < ntrees; i++) {
+        if (!trees[i]) return PLK_ERR_ARG;
+        ts[i] = trees[i]->t;
+    }
+    std::vector<int> ne(ntrees), st(ntrees);
+    const int r = PhyloTreeGPU::optimize_gamma_shape_batch(ts, eps, alpha, lnl, ne.data(), st.data());
+    for (int i = 0; i < ntrees; i++) {
+        if (nevals) nevals[i] = ne[i];
+        if (status) status[i] = st[i];
+    }
+    return r;
 }
 
 int plkh_supertree_optimize_all_branches(plkh_tree* const* trees, int ntrees, int iterations,

@@ -1,12 +1,14 @@
 #include "phylotree_gpu.h"
 
+#include "model.h"
+
 #include <algorithm>
 #include <cmath>
 
 namespace plk {
 
 PhyloTreeGPU::PhyloTreeGPU(plk_context* ctx, const Workload& w, int precision, int safe_numeric)
-    : tree_(w.tree), initial_len_(w.tree.len), ctx_(ctx) {
+    : tree_(w.tree), initial_len_(w.tree.len), ctx_(ctx), model_(w.model) {
     computed_.assign((size_t)tree_.nnodes * 3, 0);
     owner_.assign(tree_.nnodes, -1);
     pars_dir_.assign(tree_.nnodes, -1);
@@ -332,8 +334,10 @@ int PhyloTreeGPU::optimize_branch(int n1, int n2, int max_nr, double* new_len, i
 int PhyloTreeGPU::optimize_all_branches(int iterations, double tolerance, int max_nr, double* lnl,
                                         int* n_derv) {
     if (!ctx_) return PLK_ERR_STATE;
-    // the device sweep is built for 4 and 20 states; codon models take the host loop
-    if (!device_newton || (nstates_ != 4 && nstates_ != 20))
+    // the device sweep is built for 4, 20 and 61 states (61 with at most 4
+    // rate categories; other shapes report PLK_ERR_UNSUPPORTED and take the
+    // host loop below)
+    if (!device_newton || (nstates_ != 4 && nstates_ != 20 && nstates_ != 61))
         return optimize_all_branches_host(iterations, tolerance, max_nr, lnl, n_derv);
     // a sweep holds a pass of at most PLK_SWEEP_MAX_BRANCHES branches
     if (tree_.nbranch() > PLK_SWEEP_MAX_BRANCHES)
@@ -699,6 +703,27 @@ int PhyloTreeGPU::nni_evaluate_all(int max_nr, std::vector<double>& lnl, std::ve
         }
     theta_a_ = theta_b_ = -1;
     return PLK_OK;
+}
+
+
+int PhyloTreeGPU::set_gamma_shape(double alpha) {
+    if (!ctx_) return PLK_ERR_STATE;
+    if (!(alpha > 0.0) || !std::isfinite(alpha)) return PLK_ERR_ARG;
+    model_.alpha = alpha;
+    gamma_mean_rates(model_.ncat, alpha, model_.p_inv, model_.rates.data(), model_.props.data());
+    const int st = plk_set_model(ctx_, inst_, model_.eval.data(), model_.evec.data(), model_.inv_evec.data(),
+                                 model_.tip.data(), model_.rates.data(), model_.props.data());
+    clear_all_partial();
+    return st;
+}
+
+int PhyloTreeGPU::optimize_gamma_shape(double eps, double* alpha, double* lnl, int* nevals) {
+    // the batched driver with one tree: each evaluation is the same
+    // plk_batch_evaluate call as in a batch of candidates, so a tree ends
+    // where it ends in any batch, bit for bit
+    int st = PLK_OK;
+    const int r = optimize_gamma_shape_batch({this}, eps, alpha, lnl, nevals, &st);
+    return r != PLK_OK ? r : st;
 }
 
 }  // namespace plk

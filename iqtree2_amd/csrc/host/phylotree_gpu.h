@@ -56,6 +56,23 @@ public:
     void clear_reverse(int node, int dad);
 
     int compute_likelihood(double* lnl, double* pattern_lh);
+    // ---- rate heterogeneity (model/rategamma.cpp) ----
+    // RateGamma::setGammaShape + computeRates: the category rates of the new
+    // shape (p_inv unchanged) into the instance's model, every partial stale
+    int set_gamma_shape(double alpha);
+    double gamma_shape() const { return model_.alpha; }
+    // RateGamma::optimizeParameters (model/rategamma.cpp:232-246): Brent's
+    // minimizeOneDimen over [0.02, 1000] from the current shape, tolerance
+    // max(eps, 0.001), one full traversal per evaluation (a batch of one in
+    // optimize_gamma_shape_batch); the shape is left at the optimum, *lnl is
+    // the lnL there, *nevals the evaluations
+    int optimize_gamma_shape(double eps, double* alpha, double* lnl, int* nevals);
+    // optimize_gamma_shape of many trees on one context (ModelFinder's
+    // candidate models of one alignment, or partitions) advanced together:
+    // one batched traversal launch per round for every tree that needs a
+    // likelihood, so each tree takes the same steps as when run alone
+    static int optimize_gamma_shape_batch(const std::vector<PhyloTreeGPU*>& trees, double eps, double* alpha,
+                                          double* lnl, int* nevals, int* status);
     int likelihood_branch(int a, int b, double* lnl, double* pattern_lh);
     int likelihood_derv(int a, int b, double* df, double* ddf);
     // optimizeOneBranch(a, b, clearLH, max_nr)
@@ -119,6 +136,7 @@ private:
     std::vector<char> computed_;   // [nnodes*3]: partial of v seen from adj[v][k]
     std::vector<int> owner_;       // [nnodes]: orientation holding v's buffer
     std::vector<int> pars_dir_;    // [nnodes]: neighbour v's parsimony sets point away from; -1 none
+    ModelParams model_;            // the instance's model (rates follow set_gamma_shape)
     std::vector<int> size_;        // [nnodes*3] leaves below v away from adj[v][k]; 0 = unknown
     int theta_a_ = -1, theta_b_ = -1;   // branch whose theta buffer is current
     int cur_a_ = -1, cur_b_ = -1;       // current_it of PhyloTree

@@ -391,6 +391,35 @@ def likelihood_derv_batch(ctx, pairs):
     return df, ddf, lnl
 
 
+def minimize_one_dimen(f, xmin, xguess, xmax, tol):
+    """Optimization::minimizeOneDimen in the host library: (x, f(x), evaluations)."""
+    cb = _capi.BRENT_FN(lambda x, _arg: float(f(x)))
+    x, fx, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    check(lib().plkh_minimize_one_dimen(cb, None, xmin, xguess, xmax, tol, ctypes.byref(x), ctypes.byref(fx),
+                                        ctypes.byref(n)),
+          "plkh_minimize_one_dimen")
+    return x.value, fx.value, n.value
+
+
+def models_optimize_gamma_shape(trees, eps=0.0):
+    """optimize_gamma_shape of every tree advanced together (one batched
+    traversal per round): (shapes, lnL values, evaluations, statuses)."""
+    n = len(trees)
+    hs = (ctypes.c_void_p * max(n, 1))(*[t._h for t in trees])
+    a, lnl = np.zeros(n), np.zeros(n)
+    ne, st = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    check(lib().plkh_models_optimize_gamma_shape(hs, n, eps, dptr(a), dptr(lnl), i32ptr(ne), i32ptr(st)),
+          "plkh_models_optimize_gamma_shape")
+    return a, lnl, ne, st
+
+
+def gamma_rates(ncat, alpha, p_inv=0.0):
+    """Discrete-Gamma category rates and weights (host library)."""
+    rates, props = np.zeros(ncat), np.zeros(ncat)
+    check(lib().plkh_gamma_rates(ncat, alpha, p_inv, dptr(rates), dptr(props)), "plkh_gamma_rates")
+    return rates, props
+
+
 def supertree_optimize_all_branches(trees, iterations=1, tolerance=0.001, max_nr=100):
     """PhyloSuperTree::optimizeAllBranches (tree/phylosupertree.cpp:773-792):
     every tree's optimizeAllBranches(iterations, tolerance / min(n, 10),
@@ -513,6 +542,18 @@ class PhyloTree:
         check(lib().plkh_tree_optimize_branch(self._h, a, b, max_nr, ctypes.byref(x),
                                               ctypes.byref(nd)), "plkh_tree_optimize_branch")
         return x.value, nd.value
+
+    def set_gamma_shape(self, alpha):
+        """RateGamma::setGammaShape: new category rates, every partial stale."""
+        check(lib().plkh_tree_set_gamma_shape(self._h, alpha), "plkh_tree_set_gamma_shape")
+
+    def optimize_gamma_shape(self, eps=0.0):
+        """RateGamma::optimizeParameters on the device: (shape, lnL, evaluations)."""
+        a, lnl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(lib().plkh_tree_optimize_gamma_shape(self._h, eps, ctypes.byref(a), ctypes.byref(lnl),
+                                                   ctypes.byref(n)),
+              "plkh_tree_optimize_gamma_shape")
+        return a.value, lnl.value, n.value
 
     def optimize_all_branches(self, iterations=1, tolerance=0.001, max_nr=100):
         lnl, nd = ctypes.c_double(), ctypes.c_int()

@@ -167,7 +167,7 @@ def test_codon_fp32_storage(ctx):
 
 @pytest.mark.parametrize("ncat", [1, 4])
 def test_codon_optimize_all_branches(ctx, ncat):
-    """The driver takes the host Newton loop for 61 states; it matches the
+    """The driver runs the device sweep for 61 states; it matches the
     oracle replaying optimizeAllBranches (tree/phylotree.cpp:2698-2773) with
     the same plans, including the revert of every length when the pass
     lowered the lnL. With ncat = 4 the reference's derivative double-counts
@@ -355,6 +355,32 @@ def test_device_newton_sweep_matches_host_newton(ctx, seq_type, precision, safe,
     # the tree lnL at the new lengths, recomputed from scratch
     dev.clear_all_partial()
     assert abs(dev.compute_likelihood() - ld) <= 1e-8 * abs(ld)
+
+
+@pytest.mark.parametrize("precision,ncat", [(ia.PREC_F64, 4), (ia.PREC_F64, 2), (ia.PREC_F32, 1)])
+def test_codon_device_sweep_matches_host_newton(ctx, precision, ncat):
+    """61 states in the device sweep (tables built by the grid into HBM,
+    the codon matrix-core tile, streamed theta) against minimizeNewton on the
+    host: the same lengths, evaluation counts and lnL over two passes."""
+    w = ia.Workload(seq_type=ia.CODON, ntaxa=16, nsite=600, ncat=ncat, mean_len=0.1, alpha=0.5,
+                    seed=4343 + ncat)
+    print(f"codon sweep precision={precision} ncat={ncat}", flush=True)
+    dev = ia.PhyloTree(ctx, w, precision=precision)
+    host = ia.PhyloTree(ctx, w, precision=precision)
+    host.set_device_newton(False)
+    try:
+        ld, nd = dev.optimize_all_branches(iterations=2, tolerance=1e-3, max_nr=100)
+        lh, nh = host.optimize_all_branches(iterations=2, tolerance=1e-3, max_nr=100)
+        assert nd == nh
+        assert abs(ld - lh) <= 1e-9 * abs(lh), (ld, lh)
+        for a, b in dev.branches():
+            xd, xh = dev.get_length(a, b), host.get_length(a, b)
+            assert abs(xd - xh) <= 1e-9 * max(xh, 1e-6), (a, b, xd, xh)
+        dev.clear_all_partial()
+        assert abs(dev.compute_likelihood() - ld) <= 1e-8 * abs(ld)
+    finally:
+        dev.close()
+        host.close()
 
 
 def context_with(env):

@@ -599,7 +599,7 @@ int PhyloTreeGPU::nni_evaluate(int node1, int node2, int max_nr, double lnl[2], 
 int PhyloTreeGPU::nni_evaluate_all(int max_nr, std::vector<double>& lnl, std::vector<double>& lens,
                                    std::vector<int>& nd) {
     if (!ctx_) return PLK_ERR_STATE;
-    if (nstates_ != 4 && nstates_ != 20) return PLK_ERR_UNSUPPORTED;
+    if (nstates_ != 4 && nstates_ != 20 && nstates_ != 61) return PLK_ERR_UNSUPPORTED;
     std::vector<int> I1, I2;
     inner_branches(I1, I2);
     const int m = (int)I1.size(), ne = tree_.nbranch();

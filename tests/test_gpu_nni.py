@@ -125,10 +125,12 @@ def test_nni_replay_engine_and_oracle(ctx, seq_type):
         inst.close()
 
 
-@pytest.mark.parametrize("seq_type", [ia.DNA, ia.PROTEIN])
+@pytest.mark.parametrize("seq_type", [ia.DNA, ia.PROTEIN, ia.CODON])
 def test_nni_driver_host_and_device_vs_oracle(ctx, seq_type):
-    w = ia.Workload(seq_type=seq_type, ntaxa=40, nsite=2500 if seq_type == ia.DNA else 800,
-                    ncat=4, mean_len=0.12, alpha=0.6, seed=71 + seq_type)
+    ntaxa, nsite = {ia.DNA: (40, 2500), ia.PROTEIN: (40, 800), ia.CODON: (14, 300)}[seq_type]
+    w = ia.Workload(seq_type=seq_type, ntaxa=ntaxa, nsite=nsite, ncat=4, mean_len=0.12, alpha=0.6,
+                    seed=71 + seq_type)
+    print(f"nni driver seq_type={seq_type}", flush=True)
     tree = ia.PhyloTree(ctx, w)
     try:
         inner = tree.inner_branches()
@@ -146,7 +148,7 @@ def test_nni_driver_host_and_device_vs_oracle(ctx, seq_type):
         assert abs(lnl_fresh - lnl_after) <= 1e-12 * abs(lnl_fresh)
     finally:
         tree.close()
-    orc = OracleTree.from_workload(w)
+    orc = OracleTree.from_workload(w, safe_numeric=seq_type == ia.CODON)
     ref = oracle_replay_all(w, orc, inner)
     for j, ((lh, Lh, nh), r) in enumerate(zip(host, ref)):
         for cnt in range(2):

@@ -181,121 +181,135 @@ void step(plks_server* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stats.traverse_s += s->now() - tb;
     }
-    // one derivative evaluation for every optimisation in progress, and the
-    // final lnL evaluations of those whose Newton iteration has ended
-    std::vector<Req*> dv;
-    // which: 0 Newton step, 1 lnL at optx, 2 lnL at the old length, 3 lnL at
-    // the last evaluated length
-    std::vector<int> ids, which;
-    std::vector<double> xs;
-    for (Req* q : s->inflight) {
-        if (q->phase == Req::NEWTON) {
-            dv.push_back(q);
-            ids.push_back(q->inst);
-            xs.push_back(q->nr.x());
-            which.push_back(0);
-        } else if (q->phase == Req::FINAL) {
-            dv.push_back(q);
-            ids.push_back(q->inst);
-            xs.push_back(q->optx);
-            which.push_back(1);
-            if (q->need_orig) {
-                dv.push_back(q);
-                ids.push_back(q->inst);
-                xs.push_back(q->r.length);
-                which.push_back(2);
-            }
-            if (q->need_last) {
-                dv.push_back(q);
-                ids.push_back(q->inst);
-                xs.push_back(q->nr.last_x());
-                which.push_back(3);
-            }
-        }
-    }
-    if (!dv.empty()) {
-        did = true;
-        const size_t n = dv.size();
-        const double tb = s->now();
-        std::vector<double> df(n), ddf(n), lnl(n);
-        const int st = plk_likelihood_derv_batch(ctx, ids.data(), xs.data(), (int)n, df.data(), ddf.data(),
-                                                 lnl.data());
-        {
+    // derivative rounds: one derivative evaluation for every optimisation in
+    // progress per round, and the final lnL evaluations of those whose Newton
+    // iteration has ended. Several rounds run back to back (a round is one
+    // small batched launch, far cheaper than an iteration with admissions and
+    // a traversal batch), so an optimisation takes few scheduler iterations;
+    // the rounds stop when none is left, after kDervRounds, or after two when
+    // new requests wait to be admitted. Every request's Newton steps are the
+    // same whatever the rounds.
+    constexpr int kDervRounds = 16;
+    for (int round = 0; round < kDervRounds; round++) {
+        if (round >= 2) {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->stats.derv_batches++;
+            if (!s->queue.empty()) break;
         }
-        if (st != PLK_OK && st != PLK_ERR_NONFINITE) {
-            for (Req* q : dv) {
-                q->res.status = st;
-                q->phase = Req::DONE;
-            }
-        } else {
-            std::vector<Req*> finals;
-            for (size_t k = 0; k < n; k++) {
-                Req* q = dv[k];
-                if (q->phase == Req::DONE) continue;
-                if (which[k] == 0) {
-                    // a non-finite derivative (marked by lnl = NaN, df = ddf = 0):
-                    // an error of the normal kernel, df = ddf = 0 for the safe
-                    // one (tree/phylokernelnew.h:2577-2579, 2629-2632)
-                    if (std::isnan(lnl[k]) && !q->r.desc.safe_numeric) {
-                        q->res.status = PLK_ERR_NONFINITE;
-                        q->phase = Req::DONE;
-                        continue;
-                    }
-                    q->nr.feed(-df[k], -ddf[k]);
-                    if (!q->nr.done()) continue;
-                    if (q->nr.failed()) {
-                        q->res.status = PLK_ERR_NEWTON;
-                        q->phase = Req::DONE;
-                        continue;
-                    }
-                    // optimizeOneBranch's divergence guard needs lnL at optx and
-                    // at the old length (tree/phylotree.cpp:2616-2625)
-                    q->optx = q->nr.result();
-                    q->need_orig = q->optx > q->r.max_len * 0.95;
-                    q->need_last = q->need_orig && q->nr.last_x() != q->optx;
-                    q->phase = Req::FINAL;
-                } else if (which[k] == 1) {
-                    q->lnl_opt = lnl[k];
-                    finals.push_back(q);
-                } else if (which[k] == 2) {
-                    q->lnl_orig = lnl[k];
-                } else {
-                    q->lnl_last = lnl[k];
+        std::vector<Req*> dv;
+        // which: 0 Newton step, 1 lnL at optx, 2 lnL at the old length, 3 lnL at
+        // the last evaluated length
+        std::vector<int> ids, which;
+        std::vector<double> xs;
+        for (Req* q : s->inflight) {
+            if (q->phase == Req::NEWTON) {
+                dv.push_back(q);
+                ids.push_back(q->inst);
+                xs.push_back(q->nr.x());
+                which.push_back(0);
+            } else if (q->phase == Req::FINAL) {
+                dv.push_back(q);
+                ids.push_back(q->inst);
+                xs.push_back(q->optx);
+                which.push_back(1);
+                if (q->need_orig) {
+                    dv.push_back(q);
+                    ids.push_back(q->inst);
+                    xs.push_back(q->r.length);
+                    which.push_back(2);
+                }
+                if (q->need_last) {
+                    dv.push_back(q);
+                    ids.push_back(q->inst);
+                    xs.push_back(q->nr.last_x());
+                    which.push_back(3);
                 }
             }
-            for (Req* q : finals) {
-                // lnL values the batch could not form: computeLikelihoodFromBuffer
-                // with its per-pattern fix-up, one request at a time
-                int fst = PLK_OK;
-                auto fixed = [&](double x, double& v) {
-                    if (fst == PLK_OK && !std::isfinite(v))
-                        fst = plk_likelihood_from_buffer(ctx, q->inst, x, &v, nullptr);
-                };
-                fixed(q->optx, q->lnl_opt);
-                if (q->need_orig) fixed(q->r.length, q->lnl_orig);
-                if (q->need_last) fixed(q->nr.last_x(), q->lnl_last);
-                if (fst != PLK_OK) {
-                    q->res.status = fst;
+        }
+        if (!dv.empty()) {
+            did = true;
+            const size_t n = dv.size();
+            const double tb = s->now();
+            std::vector<double> df(n), ddf(n), lnl(n);
+            const int st = plk_likelihood_derv_batch(ctx, ids.data(), xs.data(), (int)n, df.data(), ddf.data(),
+                                                     lnl.data());
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->stats.derv_batches++;
+            }
+            if (st != PLK_OK && st != PLK_ERR_NONFINITE) {
+                for (Req* q : dv) {
+                    q->res.status = st;
                     q->phase = Req::DONE;
-                    continue;
                 }
-                const double guard_lh = q->need_last ? q->lnl_last : q->lnl_opt;
-                if (q->need_orig && q->lnl_orig > guard_lh) {
-                    q->res.length = q->r.length;
-                    q->res.lnl = q->lnl_orig;
-                } else {
-                    q->res.length = q->optx;
-                    q->res.lnl = q->lnl_opt;
+            } else {
+                std::vector<Req*> finals;
+                for (size_t k = 0; k < n; k++) {
+                    Req* q = dv[k];
+                    if (q->phase == Req::DONE) continue;
+                    if (which[k] == 0) {
+                        // a non-finite derivative (marked by lnl = NaN, df = ddf = 0):
+                        // an error of the normal kernel, df = ddf = 0 for the safe
+                        // one (tree/phylokernelnew.h:2577-2579, 2629-2632)
+                        if (std::isnan(lnl[k]) && !q->r.desc.safe_numeric) {
+                            q->res.status = PLK_ERR_NONFINITE;
+                            q->phase = Req::DONE;
+                            continue;
+                        }
+                        q->nr.feed(-df[k], -ddf[k]);
+                        if (!q->nr.done()) continue;
+                        if (q->nr.failed()) {
+                            q->res.status = PLK_ERR_NEWTON;
+                            q->phase = Req::DONE;
+                            continue;
+                        }
+                        // optimizeOneBranch's divergence guard needs lnL at optx and
+                        // at the old length (tree/phylotree.cpp:2616-2625)
+                        q->optx = q->nr.result();
+                        q->need_orig = q->optx > q->r.max_len * 0.95;
+                        q->need_last = q->need_orig && q->nr.last_x() != q->optx;
+                        q->phase = Req::FINAL;
+                    } else if (which[k] == 1) {
+                        q->lnl_opt = lnl[k];
+                        finals.push_back(q);
+                    } else if (which[k] == 2) {
+                        q->lnl_orig = lnl[k];
+                    } else {
+                        q->lnl_last = lnl[k];
+                    }
                 }
-                q->res.n_derv = q->nr.evals();
-                q->res.status = PLK_OK;
-                q->phase = Req::DONE;
+                for (Req* q : finals) {
+                    // lnL values the batch could not form: computeLikelihoodFromBuffer
+                    // with its per-pattern fix-up, one request at a time
+                    int fst = PLK_OK;
+                    auto fixed = [&](double x, double& v) {
+                        if (fst == PLK_OK && !std::isfinite(v))
+                            fst = plk_likelihood_from_buffer(ctx, q->inst, x, &v, nullptr);
+                    };
+                    fixed(q->optx, q->lnl_opt);
+                    if (q->need_orig) fixed(q->r.length, q->lnl_orig);
+                    if (q->need_last) fixed(q->nr.last_x(), q->lnl_last);
+                    if (fst != PLK_OK) {
+                        q->res.status = fst;
+                        q->phase = Req::DONE;
+                        continue;
+                    }
+                    const double guard_lh = q->need_last ? q->lnl_last : q->lnl_opt;
+                    if (q->need_orig && q->lnl_orig > guard_lh) {
+                        q->res.length = q->r.length;
+                        q->res.lnl = q->lnl_orig;
+                    } else {
+                        q->res.length = q->optx;
+                        q->res.lnl = q->lnl_opt;
+                    }
+                    q->res.n_derv = q->nr.evals();
+                    q->res.status = PLK_OK;
+                    q->phase = Req::DONE;
+                }
             }
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stats.derv_s += s->now() - tb;
         }
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stats.derv_s += s->now() - tb;
+        if (dv.empty()) break;
     }
     if (did) {
         std::lock_guard<std::mutex> lk(s->mu);

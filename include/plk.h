@@ -73,7 +73,11 @@ typedef struct {
 #define PLK_MAX_CHILDREN 2
 
 /* TraversalInfo (tree/phylotree.h:320-331): compute slot dad_slot from its
- * children. Steps are executed in list order (post-order). */
+ * children. Steps are executed in list order (post-order). A node of degree
+ * k > 3 (the reference's multifurcating branch, tree/phylokernelnew.h:
+ * 1405-1631: the product over all children) is planned as k - 2 steps, each
+ * intermediate slot the child of the next step through a zero-length branch
+ * (transition matrix = the identity), which gives the same product. */
 typedef struct {
     int dad_slot;
     int nchildren;        /* 2 (binary nodes) */

@@ -117,13 +117,15 @@ plk_status plk_set_model(plk_context* ctx, int instance, const double* eval,
                          const double* evec, const double* inv_evec, const double* tip_eigen,
                          const double* cat_rates, const double* cat_props);
 
-/* Lewis's ascertainment bias correction, ASC_VARIANT (+ASC; model/modelfactory.cpp
- * getASC, tree/phylokernelnew.h:2580-2628, 3175-3221, 3385-3425): the last nasc of
- * the instance's npatterns patterns are the unobserved constant patterns
- * (Alignment::getUnobservedConstPatterns, alignment/alignment.cpp:4140-4157),
- * passed with weight 0 and +I mass 0. Every lnL, pattern lnL and derivative is
- * then conditioned on variable sites: lnL - nsite * log(1 - P(const)), with
- * nsite the weight of the observed patterns. nasc = 0 turns it off. The batched
+/* Lewis's ascertainment bias correction, ASC_VARIANT and ASC_INFORMATIVE
+ * (+ASC, +ASC_INF; model/modelfactory.cpp getASC, tree/phylokernelnew.h:
+ * 2272-2273, 2580-2628, 3175-3221, 3385-3425): the last nasc of the instance's
+ * npatterns patterns are the unobserved patterns
+ * (Alignment::getUnobservedConstPatterns, alignment/alignment.cpp:4140-4196:
+ * the constant patterns, or every parsimony-uninformative pattern), passed
+ * with weight 0 and +I mass 0. Every lnL, pattern lnL and derivative is then
+ * conditioned on the observable sites: lnL - nsite * log(1 - P(unobserved)),
+ * with nsite the weight of the observed patterns. nasc = 0 turns it off. The batched
  * derivative call and the device sweep do not apply it (PLK_ERR_UNSUPPORTED);
  * batched traversals evaluate such items alone. pattern_lh outputs hold the
  * conditioned values of the observed patterns, then nasc zeros. */

@@ -56,6 +56,15 @@ int plkh_patterns_get(const plkh_patterns* p, uint8_t* states, int32_t* freq, in
                       int32_t* site_pattern);
 /* ptn_invar (tree/phylotreesse.cpp:549-617) for the first npattern entries */
 int plkh_patterns_invar(const plkh_patterns* p, const double* pi, double p_inv, double* invar);
+/* The unobserved patterns of Lewis's correction for parsimony-informative
+ * sites (ASC_INFORMATIVE, Alignment::getUnobservedConstPatterns,
+ * alignment/alignment.cpp:4181-4196): every parsimony-uninformative pattern
+ * of nseq sequences over the nstates states, in the reference's order, as
+ * rows of nseq states. *count gets the number; out (nullable) receives
+ * min(count, cap) rows. PLK_ERR_UNSUPPORTED when there would be more than
+ * max_patterns, or more than 31 states. Append them to the observed patterns
+ * (weight 0, +I mass 0) and call plk_set_asc with their count. */
+int plkh_uninformative_patterns(int nstates, int nseq, long max_patterns, uint8_t* out, long cap, long* count);
 
 /* ---- synthetic workloads ---------------------------------------------- */
 typedef struct {

@@ -154,6 +154,8 @@ _PROTOS = {
     "plkh_rate_matrix": (c_int, [c_int, _DP, _DP, _DP]),
     "plkh_decompose": (c_int, [c_int, _DP, _DP, _DP, _DP, _DP]),
     "plkh_gamma_rates": (c_int, [c_int, c_double, c_double, _DP, _DP]),
+    "plkh_uninformative_patterns": (c_int, [c_int, c_int, ctypes.c_long, _U8, ctypes.c_long,
+                                            ctypes.POINTER(ctypes.c_long)]),
     "plkh_minimize_one_dimen": (c_int, [BRENT_FN, c_void_p, c_double, c_double, c_double, c_double, _DP, _DP,
                                         ctypes.POINTER(c_int)]),
     "plkh_tree_set_gamma_shape": (c_int, [c_void_p, c_double]),

@@ -134,8 +134,10 @@ struct Instance {
     bool pars_tips_ready = false;          // tip planes of the current patterns
     plk_op* pars_ops = nullptr;            // ops sorted by level, then level starts (int)
     int pars_ops_cap = 0;
-    // +ASC (Lewis): the last nasc patterns are the unobserved constant ones
+    // +ASC (Lewis): the last nasc patterns are the unobserved ones
     int nasc = 0;
+    double* asc_vals = nullptr;  // more than kAscMax of them: [nasc][4] values, then the 4 sums
+    size_t asc_cap = 0;
     std::vector<double> hfreq;   // pattern weights (host copy)
     double nsite = 0.0;          // sites of the observed patterns
     double staged_bytes = 0.0, staged_flops = 0.0;   // per-pattern cost of the staged ops
@@ -143,8 +145,10 @@ struct Instance {
 
 size_t alloc_class(size_t bytes);
 
-// red_out: 8 reduced scalars, then four values per unobserved constant
-// pattern of +ASC (at most one per state, 61 for codons)
+// red_out: 8 reduced scalars, then four values per unobserved pattern of
+// +ASC when there are at most kAscMax of them (the constant patterns of
+// ASC_VARIANT, one per state); the larger sets of ASC_INFORMATIVE live in
+// the instance's asc_vals
 constexpr int kAscMax = 64;
 constexpr int kRedOutDoubles = 8 + 4 * kAscMax;
 
@@ -344,7 +348,7 @@ plk_status destroy_instance(plk_context* ctx, int id) {
                     in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
                     in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res, in->sw_tabs,
                     in->pars_tip_planes, in->pars_planes, in->pars_counts, in->pars_sets,
-                    in->pars_changes, in->pars_ops};
+                    in->pars_changes, in->pars_ops, in->asc_vals};
     for (void* b : bufs) dfree(ctx, nullptr, b, 0);
     delete in;
     ctx->inst[id] = nullptr;
@@ -408,7 +412,7 @@ InstDev inst_dev(const Instance* in) {
     I.slot_stride = (size_t)in->Ppad * in->block;
     I.scale_stride = (size_t)in->Ppad * (in->d.safe_numeric ? in->d.ncat : 1);
     I.nasc = in->nasc;
-    I.asc = in->red_out + 8;
+    I.asc = in->nasc > kAscMax ? in->asc_vals : in->red_out + 8;
     return I;
 }
 
@@ -738,9 +742,18 @@ plk_status fixup_nonfinite(plk_context* ctx, Instance* in, double* lnl, double* 
     return PLK_OK;
 }
 
-// +ASC: the four per-pattern values of the unobserved constant patterns,
-// each summed in pattern order
+// +ASC: the four per-pattern values of the unobserved patterns, each summed
+// in pattern order on the host, or by the fixed-order device reduction for
+// sets larger than kAscMax
 plk_status asc_sums(plk_context* ctx, Instance* in, double sums[4]) {
+    if (in->nasc > kAscMax) {
+        double* out = in->asc_vals + 4 * (size_t)in->nasc;
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->asc_vals, in->nasc, 4, out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(sums, out, sizeof(double) * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return PLK_OK;
+    }
     double h[4 * kAscMax];
     HIP_TRY(hipMemcpyAsync(h, in->red_out + 8, sizeof(double) * 4 * in->nasc, hipMemcpyDeviceToHost,
                            ctx->stream));
@@ -770,6 +783,22 @@ plk_status asc_correct_lnl(plk_context* ctx, Instance* in, double prob_const, do
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PLK_OK;
+}
+
+// Grows a device buffer to hold at least `need` elements (capacities are
+// element counts; byte sizes never pass through int).
+template <typename T>
+plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need) {
+    if (need <= cap && *p) return PLK_OK;
+    if (need > (std::numeric_limits<size_t>::max() / 2) / sizeof(T))
+        return fail(PLK_ERR_NOMEM, "buffer of %zu elements exceeds the address space", need);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dfree(ctx, in, *p, sizeof(T) * cap);
+    *p = nullptr;
+    cap = std::max(need, 2 * cap);
+    plk_status st = dmalloc(ctx, in, p, sizeof(T) * cap);
+    if (st != PLK_OK) cap = 0;
+    return st;
 }
 
 }  // namespace
@@ -1016,9 +1045,13 @@ plk_status plk_set_patterns(plk_context* ctx, int id, const uint8_t* states, con
 plk_status plk_set_asc(plk_context* ctx, int id, int nasc) {
     Instance* in = get_instance(ctx, id);
     if (!in) return fail(PLK_ERR_ARG, "invalid instance");
-    if (nasc < 0 || nasc > std::min(kAscMax, in->d.nstates) || nasc >= in->d.npatterns)
-        return fail(PLK_ERR_ARG, "%d unobserved constant patterns for %d states", nasc, in->d.nstates);
+    if (nasc < 0 || nasc >= in->d.npatterns)
+        return fail(PLK_ERR_ARG, "%d unobserved patterns of %d", nasc, in->d.npatterns);
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (nasc > kAscMax) {
+        plk_status st = grow(ctx, in, &in->asc_vals, in->asc_cap, 4 * ((size_t)nasc + 1));
+        if (st != PLK_OK) return st;
+    }
     in->nasc = nasc;
     in->nsite = 0.0;
     for (int p = 0; p < (int)in->hfreq.size() - nasc; p++) in->nsite += in->hfreq[p];
@@ -1241,22 +1274,6 @@ plk_status plk_optimize_branch(plk_context* ctx, int id, const plk_op* ops, int 
 }  // extern "C"
 
 namespace {
-
-// Grows a device buffer to hold at least `need` elements (capacities are
-// element counts; byte sizes never pass through int).
-template <typename T>
-plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need) {
-    if (need <= cap && *p) return PLK_OK;
-    if (need > (std::numeric_limits<size_t>::max() / 2) / sizeof(T))
-        return fail(PLK_ERR_NOMEM, "buffer of %zu elements exceeds the address space", need);
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    dfree(ctx, in, *p, sizeof(T) * cap);
-    *p = nullptr;
-    cap = std::max(need, 2 * cap);
-    plk_status st = dmalloc(ctx, in, p, sizeof(T) * cap);
-    if (st != PLK_OK) cap = 0;
-    return st;
-}
 
 template <typename T, int N, int C>
 plk_status launch_sweep_t(plk_context* ctx, Instance* in, SweepArgs& a, int& grid_out) {

@@ -208,4 +208,52 @@ void compute_ptn_invar(SeqType t, const PatternSet& ps, const double* pi, double
     for (int p = ps.npattern; p < padded_len; p++) invar[p] = p_inv;
 }
 
+namespace {
+
+// generateUninfPatterns: the states of S placed, in order, into distinct
+// sequences chosen in increasing index at every depth
+bool place(int nseq, uint8_t repeat, const std::vector<uint8_t>& sub, std::vector<int>& pos, long max_patterns,
+           long& count, std::vector<uint8_t>& out) {
+    if (pos.size() == sub.size()) {
+        if (++count > max_patterns) return false;
+        const size_t base = out.size();
+        out.resize(base + nseq, repeat);
+        for (size_t i = 0; i < pos.size(); i++) out[base + pos[i]] = sub[i];
+        return true;
+    }
+    for (int seq = 0; seq < nseq; seq++) {
+        bool used = false;
+        for (int q : pos) used = used || q == seq;
+        if (used) continue;
+        pos.push_back(seq);
+        const bool ok = place(nseq, repeat, sub, pos, max_patterns, count, out);
+        pos.pop_back();
+        if (!ok) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+long uninformative_patterns(int nstates, int nseq, long max_patterns, std::vector<uint8_t>& out) {
+    if (nstates < 2 || nstates > 31 || nseq < 2) return -1;
+    long count = 0;
+    std::vector<int> pos;
+    for (int r = 0; r < nstates; r++) {
+        std::vector<uint8_t> rest;
+        for (int s = 0; s < nstates; s++)
+            if (s != r) rest.push_back((uint8_t)s);
+        const uint64_t total = (uint64_t)1 << rest.size();
+        for (uint64_t bin = 0; bin < total; bin++) {
+            std::vector<uint8_t> sub;
+            for (size_t i = 0; i < rest.size(); i++)
+                if (bin & ((uint64_t)1 << i)) sub.push_back(rest[i]);
+            const int k = (int)sub.size();
+            if (!(k < nseq - 1 || (k == nseq - 1 && r == 0))) continue;
+            if (!place(nseq, (uint8_t)r, sub, pos, max_patterns, count, out)) return -1;
+        }
+    }
+    return count;
+}
+
 }  // namespace plk

@@ -38,6 +38,17 @@ struct PatternSet {
 // Returns 0, or a negative error (invalid character / stop codon).
 int build_patterns(SeqType t, int nseq, int nchar, const char* chars, PatternSet& out);
 
+// The unobserved patterns of Lewis's correction for parsimony-informative
+// sites (ASC_INFORMATIVE; Alignment::getUnobservedConstPatterns,
+// alignment/alignment.cpp:4181-4196, with generateSubsets and
+// generateUninfPatterns 4106-4138): for every majority state r, every subset
+// S of the other states in generateSubsets' order, and every placement of S's
+// states into |S| distinct sequences in generateUninfPatterns' order, the
+// pattern holding S's states there and r elsewhere (|S| <= nseq - 2, or
+// nseq - 1 with r = 0). Rows of nseq states are appended to out; returns the
+// count, or -1 when it would exceed max_patterns.
+long uninformative_patterns(int nstates, int nseq, long max_patterns, std::vector<uint8_t>& out);
+
 // const_char of one column of states (alignment/alignment.cpp:925-1022).
 int compute_const_char(SeqType t, int nseq, const uint8_t* column);
 

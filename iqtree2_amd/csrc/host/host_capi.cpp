@@ -54,6 +54,16 @@ int plkh_gamma_rates(int ncat, double alpha, double p_inv, double* rates, double
     return PLK_OK;
 }
 
+int plkh_uninformative_patterns(int nstates, int nseq, long max_patterns, uint8_t* out, long cap, long* count) {
+    if (!count || nstates < 2 || nseq < 2 || max_patterns < 0 || cap < 0) return PLK_ERR_ARG;
+    std::vector<uint8_t> rows;
+    const long n = uninformative_patterns(nstates, nseq, max_patterns, rows);
+    if (n < 0) return PLK_ERR_UNSUPPORTED;
+    *count = n;
+    if (out) std::memcpy(out, rows.data(), (size_t)std::min(n, cap) * nseq);
+    return PLK_OK;
+}
+
 int plkh_minimize_one_dimen(double (*f)(double, void*), void* arg, double xmin, double xguess, double xmax,
                             double tol, double* x, double* fx, int* nevals) {
     if (!f || !x || !(xmin <= xmax) || !(tol > 0.0)) return PLK_ERR_ARG;

@@ -263,7 +263,8 @@ class Instance:
 
     def set_asc(self, nasc):
         """Lewis's ascertainment bias correction: the last nasc patterns are the
-        unobserved constant patterns (weight 0, +I mass 0)."""
+        unobserved ones (weight 0, +I mass 0): the constant patterns for +ASC,
+        uninformative_patterns() for +ASC_INF."""
         check(lib().plk_set_asc(self.ctx.handle, self.id, int(nasc)), "plk_set_asc")
 
     def set_patterns(self, states, freq, invar):
@@ -411,6 +412,17 @@ def models_optimize_gamma_shape(trees, eps=0.0):
     check(lib().plkh_models_optimize_gamma_shape(hs, n, eps, dptr(a), dptr(lnl), i32ptr(ne), i32ptr(st)),
           "plkh_models_optimize_gamma_shape")
     return a, lnl, ne, st
+
+
+def uninformative_patterns(nstates, nseq, max_patterns=1 << 24):
+    """ASC_INFORMATIVE's unobserved patterns: array [count][nseq] of states."""
+    n = ctypes.c_long()
+    check(lib().plkh_uninformative_patterns(nstates, nseq, max_patterns, None, 0, ctypes.byref(n)),
+          "plkh_uninformative_patterns")
+    out = np.zeros((n.value, nseq), np.uint8)
+    check(lib().plkh_uninformative_patterns(nstates, nseq, max_patterns, u8ptr(out), n.value, ctypes.byref(n)),
+          "plkh_uninformative_patterns")
+    return out
 
 
 def gamma_rates(ncat, alpha, p_inv=0.0):

@@ -111,3 +111,45 @@ def test_asc_rell_fp32_and_batches(ctx):
         for x in (inst, inst32, plain):
             x.close()
         planner.close()
+
+
+@pytest.mark.parametrize("seq_type,ntaxa,nsite,safe", [(ia.DNA, 12, 3000, False),
+                                                       (ia.DNA, 9, 2000, True),
+                                                       (ia.PROTEIN, 4, 4000, False)])
+def test_asc_informative_vs_oracle(ctx, seq_type, ntaxa, nsite, safe):
+    """+ASC_INF: the informative columns, then every uninformative pattern
+    as the unobserved set (the oracle's form is pinned by brute force in
+    tests/test_oracle_asc_informative.py)."""
+    from test_oracle_asc_informative import informative_set
+    w = ia.Workload(seq_type=seq_type, ntaxa=ntaxa, nsite=nsite, ncat=4, mean_len=0.3, alpha=0.6,
+                    seed=913 + ntaxa)
+    st, fq, inv, nasc = informative_set(w)
+    pobs = st.shape[1] - nasc
+    assert pobs > 10
+    inst = asc_instance(ctx, w, st, fq, inv, nasc, safe=safe)
+    orc = asc_oracle(w, st, fq, inv, nasc, safe)
+    planner = ia.PhyloTree(None, w)
+    try:
+        for a, b in planner.branches()[::3]:
+            ops, node, dad, length = planner.plan(a, b)
+            lg, pg = inst.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+            lo, po = orc.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+            assert abs(lg - lo) <= 1e-9 * abs(lo), (a, b, lg, lo)
+            np.testing.assert_allclose(pg[:pobs], po, rtol=0, atol=1e-9)
+            inst.likelihood_buffer(ops, node, dad)
+            orc.likelihood_buffer(ops, node, dad)
+            for t in (length, 0.5 * length + 1e-4):
+                dfg, ddfg, _ = inst.likelihood_derv(t)
+                dfo, ddfo = orc.likelihood_derv(t)
+                assert abs(dfg - dfo) <= 1e-7 * max(1.0, abs(dfo))
+                assert abs(ddfg - ddfo) <= 1e-7 * max(1.0, abs(ddfo))
+        a, b = planner.branches()[1]
+        ops, node, dad, length = planner.plan(b, a)
+        xg, lg, ng = inst.optimize_branch(ops, node, dad, length)
+        xo, lo, no, so = orc.optimize_branch(ops, node, dad, length)
+        assert so == 0 and ng == no
+        assert abs(xg - xo) <= 1e-7 * max(1.0, xo)
+        assert abs(lg - lo) <= 1e-9 * abs(lo)
+    finally:
+        inst.close()
+        planner.close()

@@ -37,6 +37,7 @@ _PROTOS = {
                                        ctypes.POINTER(c_int)]),
     "orc_read_partial": (None, [c_void_p, c_int, _DP, ctypes.POINTER(ctypes.c_int32)]),
     "orc_set_asc": (None, [c_void_p, c_int]),
+    "orc_set_model_nonrev": (c_int, [c_void_p, c_int, c_int] + [_DP] * 8 + [c_int, _DP, c_int]),
     "orc_parsimony_branch": (c_double, [c_void_p, c_void_p, c_int, OrcEnd, OrcEnd,
                                         ctypes.POINTER(ctypes.c_int32), _DP]),
     "orc_parsimony_fast": (c_double, [c_void_p, c_void_p, c_int, OrcEnd, OrcEnd, _DP]),
@@ -114,6 +115,24 @@ class OracleTree:
         pattern lnL outputs then cover the observed patterns only."""
         lib().orc_set_asc(self._h, int(nasc))
         self.nasc = int(nasc)
+
+    def set_model_nonrev(self, nm, tip, rooted=False):
+        """State-basis (non-reversible / mixture) model: nm is a dict with
+        ceval [nmix][n] complex, cevec and cinv_evec [nmix][n][n] complex, Q
+        [nmix][n][n], pi [nmix][n], mixw [nmix], rates and props [nrates] and
+        fused; tip [SU+1][n] the plain tip vectors."""
+        def cplx(a):
+            a = np.asarray(a, np.complex128)
+            return np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1), np.float64)
+        arrs = [cplx(nm["ceval"]), cplx(nm["cevec"]), cplx(nm["cinv_evec"])]
+        arrs += [np.ascontiguousarray(nm[k], np.float64) for k in ("Q", "pi", "mixw", "rates", "props")]
+        tip = np.ascontiguousarray(tip, np.float64)
+        nmix = len(nm["mixw"])
+        st = lib().orc_set_model_nonrev(self._h, nmix, int(bool(nm["fused"])),
+                                        *[a.ctypes.data_as(_DP) for a in arrs], len(nm["rates"]),
+                                        tip.ctypes.data_as(_DP), int(bool(rooted)))
+        if st != 0:
+            raise ValueError("state-basis model does not match the instance's categories")
 
     def set_threads(self, n):
         lib().orc_set_threads(self._h, n)

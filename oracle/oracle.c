@@ -50,6 +50,15 @@ struct orc_tree {
     uint32_t* pf_tip;      /* [ntaxa][nblocks][n][8] */
     uint32_t* pf_node;     /* [nslots][nblocks][n][8] */
     int64_t* pf_score;     /* [nslots] subtree score */
+    /* non-reversible and mixture models in the state basis
+     * (tree/phylokernelnonrev.h): per mixture component the complex
+     * eigensystem (re, im pairs), the rate matrix and the frequencies */
+    int nonrev, rooted, nmix;
+    double *nr_ceval, *nr_cevec, *nr_cinv, *nr_Q, *nr_pi;
+    int* nr_mix;           /* [C] mixture component of each category */
+    uint8_t* nr_flags;     /* [2*nops] child flags of the staged ops */
+    int nr_flags_cap;
+    orc_end nr_node, nr_dad;
 };
 
 static void* amalloc(size_t bytes) {
@@ -339,6 +348,8 @@ void orc_destroy(orc_tree* t) {
     free(t->pf_tip);
     free(t->pf_node);
     free(t->pf_score);
+    free(t->nr_ceval); free(t->nr_cevec); free(t->nr_cinv); free(t->nr_Q); free(t->nr_pi);
+    free(t->nr_mix); free(t->nr_flags);
     free(t);
 }
 
@@ -407,6 +418,9 @@ typedef struct {
     const double* echild[2];
     const double* leaf[2];
 } op_info;
+
+static void nr_partial_info(orc_tree* t, const orc_op* op, const uint8_t* flags, double* ech,
+                            double* leaf);
 
 static void partial_info(orc_tree* t, const orc_op* op, double* ech, double* leaf) {
     const int n = t->n, C = t->C;
@@ -581,7 +595,8 @@ static void prepare_ops(orc_tree* t, const orc_op* ops, int nops, op_info* infos
     for (int i = 0; i < nops; i++) {
         double* ech = op_echild(t, i);
         double* leaf = op_leaf(t, i);
-        partial_info(t, &ops[i], ech, leaf);
+        if (t->nonrev) nr_partial_info(t, &ops[i], t->nr_flags + 2 * i, ech, leaf);
+        else partial_info(t, &ops[i], ech, leaf);
         for (int k = 0; k < 2; k++) {
             infos[i].echild[k] = ech + (size_t)k * t->block * t->n;
             infos[i].leaf[k] = leaf + (size_t)k * (t->SU + 1) * t->block;
@@ -595,7 +610,347 @@ static void run_ops_packet(orc_tree* t, const orc_op* ops, int nops, const op_in
     for (int i = 0; i < nops; i++) partial_dispatch(t, &ops[i], &infos[i], lo, hi, tipbuf);
 }
 
+
+/* ---- non-reversible and mixture models in the state basis ---------------
+ * The kernel set of tree/phylokernelnonrev.h: partials hold state-basis
+ * likelihoods, each child's table is its transition matrix P(r_c t) of the
+ * category's mixture component (computePartialInfo's !useRevKernel branch,
+ * tree/phylokernelnew.h:934-986: transposed for a child whose direction is
+ * toward the root, the state frequencies for the root leaf), and the branch
+ * likelihood and its derivatives apply P, QP and QQP between the two ends
+ * (tree/phylokernelnonrev.h:586-1056, 1062-1528). The partial update is the
+ * binary-node body above with the identity as the output transform: the
+ * products (E_l l) * (E_r r) of the reference's nonrev partial kernel
+ * (tree/phylokernelnonrev.h:36-580). Child and end flags ride in is_tip:
+ * bit 0 a tip, bit 1 toward the root, bit 2 the root leaf. */
+
+/* ModelMarkov::computeTransMatrixNonrev, eigen route (model/modelmarkov.cpp:
+ * 480-492): P(t) = Re(V diag(exp(lambda t)) V^-1) */
+static void nr_trans(const orc_tree* t, int m, double len, double* P) {
+    const int n = t->n;
+    const double* ev = t->nr_ceval + (size_t)m * 2 * n;
+    const double* V = t->nr_cevec + (size_t)m * 2 * n * n;
+    const double* W = t->nr_cinv + (size_t)m * 2 * n * n;
+    double er[64], ei[64];
+    for (int i = 0; i < n; i++) {
+        const double a = exp(ev[2 * i] * len), b = ev[2 * i + 1] * len;
+        er[i] = a * cos(b);
+        ei[i] = a * sin(b);
+    }
+    for (int x = 0; x < n; x++)
+        for (int y = 0; y < n; y++) {
+            double s = 0.0;
+            for (int i = 0; i < n; i++) {
+                const double vr = V[2 * (x * n + i)], vi = V[2 * (x * n + i) + 1];
+                const double dr = vr * er[i] - vi * ei[i], di = vr * ei[i] + vi * er[i];
+                s += dr * W[2 * (i * n + y)] - di * W[2 * (i * n + y) + 1];
+            }
+            P[x * n + y] = s;
+        }
+}
+
+/* computeTransDerv, non-reversible (model/modelmarkov.cpp:735-766):
+ * P, Q P, Q Q P */
+static void nr_trans_derv(const orc_tree* t, int m, double len, double* P, double* D1, double* D2) {
+    const int n = t->n;
+    const double* Q = t->nr_Q + (size_t)m * n * n;
+    nr_trans(t, m, len, P);
+    for (int x = 0; x < n; x++)
+        for (int y = 0; y < n; y++) {
+            double s = 0.0;
+            for (int k = 0; k < n; k++) s += Q[x * n + k] * P[k * n + y];
+            D1[x * n + y] = s;
+        }
+    for (int x = 0; x < n; x++)
+        for (int y = 0; y < n; y++) {
+            double s = 0.0;
+            for (int k = 0; k < n; k++) s += Q[x * n + k] * D1[k * n + y];
+            D2[x * n + y] = s;
+        }
+}
+
+int orc_set_model_nonrev(orc_tree* t, int nmix, int fused, const double* ceval, const double* cevec,
+                         const double* cinv_evec, const double* rate_matrix, const double* state_freq,
+                         const double* mix_weights, const double* site_rates, const double* site_props,
+                         int nrates, const double* tip, int rooted) {
+    const int n = t->n, C = t->C;
+    if (nmix < 1 || nrates < 1 || n > 64 || C != (fused ? nrates : nrates * nmix)) return -1;
+    if (fused && nmix != nrates) return -1;
+    const size_t nn = (size_t)n * n;
+    free(t->nr_ceval); free(t->nr_cevec); free(t->nr_cinv); free(t->nr_Q); free(t->nr_pi);
+    free(t->nr_mix);
+    t->nr_ceval = (double*)amalloc((size_t)nmix * 2 * n * 8);
+    t->nr_cevec = (double*)amalloc((size_t)nmix * 2 * nn * 8);
+    t->nr_cinv = (double*)amalloc((size_t)nmix * 2 * nn * 8);
+    t->nr_Q = (double*)amalloc((size_t)nmix * nn * 8);
+    t->nr_pi = (double*)amalloc((size_t)nmix * n * 8);
+    t->nr_mix = (int*)amalloc((size_t)C * sizeof(int));
+    memcpy(t->nr_ceval, ceval, (size_t)nmix * 2 * n * 8);
+    memcpy(t->nr_cevec, cevec, (size_t)nmix * 2 * nn * 8);
+    memcpy(t->nr_cinv, cinv_evec, (size_t)nmix * 2 * nn * 8);
+    memcpy(t->nr_Q, rate_matrix, (size_t)nmix * nn * 8);
+    memcpy(t->nr_pi, state_freq, (size_t)nmix * n * 8);
+    /* ncat_mix categories: rate category c % ncat of mixture component
+     * c / denom, weight getProp * getMixtureWeight (tree/phylokernelnonrev.h:
+     * 1107-1114) */
+    const int denom = fused ? 1 : nrates;
+    for (int c = 0; c < C; c++) {
+        const int m = c / denom;
+        t->nr_mix[c] = m;
+        t->rates[c] = site_rates[c % nrates];
+        t->props[c] = site_props[c % nrates] * mix_weights[m];
+    }
+    memset(t->inv_evec, 0, nn * 8);
+    for (int i = 0; i < n; i++) t->inv_evec[i * n + i] = 1.0;
+    memcpy(t->tip, tip, (size_t)(t->SU + 1) * n * 8);
+    t->nmix = nmix;
+    t->rooted = rooted;
+    t->nonrev = 1;
+    t->theta_ready = 0;
+    return 0;
+}
+
+static void nr_partial_info(orc_tree* t, const orc_op* op, const uint8_t* flags, double* ech,
+                            double* leaf) {
+    const int n = t->n, C = t->C;
+    const size_t block = t->block;
+    double* P = (double*)malloc((size_t)n * n * 8);
+    for (int k = 0; k < 2; k++) {
+        double* E = ech + (size_t)k * block * n;
+        for (int c = 0; c < C; c++) {
+            nr_trans(t, t->nr_mix[c], t->rates[c] * op->child[k].length, P);
+            double* e = E + (size_t)c * n * n;
+            for (int x = 0; x < n; x++)
+                for (int i = 0; i < n; i++) e[x * n + i] = (flags[k] & 2) ? P[i * n + x] : P[x * n + i];
+        }
+        if (!op->child[k].is_tip) continue;
+        double* L = leaf + (size_t)k * (t->SU + 1) * block;
+        if (flags[k] & 4) {
+            /* the root leaf: the state frequencies (tree/phylokernelnew.h:961-966) */
+            for (int s = 0; s <= t->SU; s++)
+                for (int c = 0; c < C; c++)
+                    memcpy(L + (size_t)s * block + c * n, t->nr_pi + (size_t)t->nr_mix[c] * n, n * 8);
+            continue;
+        }
+        for (int s = 0; s < t->SU; s++)
+            for (int c = 0; c < C; c++)
+                for (int x = 0; x < n; x++) {
+                    const double* e = E + ((size_t)c * n + x) * n;
+                    double v = 0.0;
+                    for (int i = 0; i < n; i++) v += e[i] * t->tip[s * n + i];
+                    L[(size_t)s * block + c * n + x] = v;
+                }
+        for (size_t x = 0; x < block; x++) L[(size_t)t->SU * block + x] = 1.0;
+    }
+    free(P);
+}
+
+/* the ops with their flags moved to nr_flags (a copy, freed by the caller) */
+static orc_op* nr_strip(orc_tree* t, const orc_op* ops, int nops) {
+    if (nops <= 0) return NULL;
+    if (2 * nops > t->nr_flags_cap) {
+        free(t->nr_flags);
+        t->nr_flags = (uint8_t*)amalloc((size_t)2 * nops);
+        t->nr_flags_cap = 2 * nops;
+    }
+    orc_op* c = (orc_op*)malloc(sizeof(orc_op) * nops);
+    memcpy(c, ops, sizeof(orc_op) * nops);
+    for (int i = 0; i < nops; i++)
+        for (int k = 0; k < 2; k++) {
+            t->nr_flags[2 * i + k] = (uint8_t)(c[i].child[k].is_tip & 6);
+            c[i].child[k].is_tip &= 1;
+        }
+    return c;
+}
+
+static void nr_run_ops(orc_tree* t, const orc_op* ops, int nops) {
+    if (nops <= 0) return;
+    orc_op* own = nr_strip(t, ops, nops);
+    op_info* infos = (op_info*)malloc(sizeof(op_info) * nops);
+    prepare_ops(t, own, nops, infos);
+    size_t limits[1024];
+    compute_bounds(t->nthreads, t->npackets, t->Ppad, limits);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(t->nthreads)
+    for (int pk = 0; pk < t->npackets; pk++) run_ops_packet(t, own, nops, infos, limits[pk], limits[pk + 1], pk);
+    free(infos);
+    free(own);
+}
+
+INLINE double nr_elem(const double* slot, size_t block, size_t ptn, size_t i) {
+    return slot[(ptn / VS) * block * VS + i * VS + ptn % VS];
+}
+
+/* the safe kernel's per-category factor for a scale count above the minimum */
+INLINE double nr_adj(int d) { return d == 0 ? 1.0 : (d == 1 ? SCALING_THRESHOLD : 0.0); }
+
+/* branch lnL (derv = 0; pattern lnL kept) or df, ddf (derv = 1) between the
+ * buffered ends at `length` */
+static double nr_eval(orc_tree* t, double length, int derv, double* df_out, double* ddf_out, int* status) {
+    const int n = t->n, C = t->C, safe = t->safe;
+    const size_t block = t->block, nn = (size_t)n * n;
+    const orc_end node = t->nr_node, dad = t->nr_dad;
+    const int dad_tip = dad.is_tip & 1, dad_root = (dad.is_tip & 4) != 0;
+    /* trans_mat (and derivatives) per category: * prop, * rate, rows * pi
+     * when unrooted (tree/phylokernelnonrev.h:630-666, 1107-1131) */
+    double* T = (double*)amalloc(3 * (size_t)C * nn * 8);
+    double* D1 = (double*)malloc(nn * 8);
+    double* D2 = (double*)malloc(nn * 8);
+    for (int c = 0; c < C; c++) {
+        const int m = t->nr_mix[c];
+        const double r = t->rates[c], w = t->props[c];
+        double* T0 = T + (size_t)c * nn;
+        double* T1 = T + ((size_t)C + c) * nn;
+        double* T2 = T + ((size_t)2 * C + c) * nn;
+        nr_trans_derv(t, m, r * length, T0, D1, D2);
+        for (size_t i = 0; i < nn; i++) {
+            T0[i] *= w;
+            T1[i] = D1[i] * (w * r);
+            T2[i] = D2[i] * (w * r * r);
+        }
+        if (!t->rooted)
+            for (int x = 0; x < n; x++)
+                for (int y = 0; y < n; y++) {
+                    const double f = t->nr_pi[(size_t)m * n + x];
+                    T0[x * n + y] *= f;
+                    T1[x * n + y] *= f;
+                    T2[x * n + y] *= f;
+                }
+    }
+    free(D1);
+    free(D2);
+    /* a tip dad: lut[k][s][c][x] = sum_y Tk[c][x][y] tip[s][y]; the root leaf:
+     * the frequencies times prop, no derivative */
+    double* lut = NULL;
+    if (dad_tip) {
+        const size_t ls = (size_t)(t->SU + 1) * block;
+        lut = (double*)amalloc(3 * ls * 8);
+        for (int s = 0; s <= t->SU; s++)
+            for (int c = 0; c < C; c++)
+                for (int x = 0; x < n; x++) {
+                    if (dad_root) {
+                        lut[(size_t)s * block + c * n + x] = t->nr_pi[(size_t)t->nr_mix[c] * n + x] * t->props[c];
+                        continue;
+                    }
+                    for (int k = 0; k < 3; k++) {
+                        const double* Tk = T + ((size_t)k * C + c) * nn + (size_t)x * n;
+                        double v = 0.0;
+                        for (int y = 0; y < n; y++) v += Tk[y] * t->tip[s * n + y];
+                        lut[k * ls + (size_t)s * block + c * n + x] = v;
+                    }
+                }
+    }
+    const size_t ls = (size_t)(t->SU + 1) * block;
+    const double* pn = slot_ptr(t, node.id);
+    const uint16_t* sn = scale_ptr(t, node.id);
+    const double* pd = dad_tip ? NULL : slot_ptr(t, dad.id);
+    const uint16_t* sd = dad_tip ? NULL : scale_ptr(t, dad.id);
+    const uint8_t* drow = dad_tip ? t->states + (size_t)dad.id * t->Ppad : NULL;
+    const size_t nobs = (size_t)(t->P - t->nasc);
+    double all = 0.0, all_df = 0.0, all_ddf = 0.0;
+#pragma omp parallel for num_threads(t->nthreads) reduction(+ : all, all_df, all_ddf)
+    for (size_t p = 0; p < nobs; p++) {
+        double adjn[16], adjd[16], lh_cat[16];
+        int minsc = 0;
+        if (safe) {
+            if (!derv) {
+                /* the branch lnL: per-category sums against their minimum (1422-1442) */
+                int sum[16];
+                for (int c = 0; c < C; c++) {
+                    sum[c] = sn[p * C + c] + (sd ? sd[p * C + c] : 0);
+                    if (c == 0 || sum[c] < minsc) minsc = sum[c];
+                }
+                for (int c = 0; c < C; c++) { adjn[c] = nr_adj(sum[c] - minsc); adjd[c] = 1.0; }
+            } else {
+                /* derivatives: each end against its own minimum (893-947) */
+                int mn = 0, md = 0;
+                for (int c = 0; c < C; c++) if (c == 0 || sn[p * C + c] < mn) mn = sn[p * C + c];
+                if (sd) for (int c = 0; c < C; c++) if (c == 0 || sd[p * C + c] < md) md = sd[p * C + c];
+                for (int c = 0; c < C; c++) {
+                    adjn[c] = nr_adj(sn[p * C + c] - mn);
+                    adjd[c] = sd ? nr_adj(sd[p * C + c] - md) : 1.0;
+                }
+                minsc = mn + md;
+            }
+        } else {
+            minsc = sn[p] + (sd ? sd[p] : 0);
+            for (int c = 0; c < C; c++) adjn[c] = adjd[c] = 1.0;
+        }
+        double lh = 0.0, d1 = 0.0, d2 = 0.0;
+        for (int c = 0; c < C; c++) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int x = 0; x < n; x++) {
+                const double nv = nr_elem(pn, block, p, (size_t)c * n + x) * (derv ? adjn[c] : 1.0);
+                double v0, v1 = 0.0, v2 = 0.0;
+                if (dad_tip) {
+                    const size_t o = (size_t)drow[p] * block + c * n + x;
+                    v0 = lut[o];
+                    if (derv && !dad_root) { v1 = lut[ls + o]; v2 = lut[2 * ls + o]; }
+                } else {
+                    const double* T0 = T + (size_t)c * nn + (size_t)x * n;
+                    const double* T1 = T0 + (size_t)C * nn;
+                    const double* T2 = T1 + (size_t)C * nn;
+                    v0 = 0.0;
+                    for (int y = 0; y < n; y++) {
+                        const double dv = nr_elem(pd, block, p, (size_t)c * n + y) * (derv ? adjd[c] : 1.0);
+                        v0 += T0[y] * dv;
+                        if (derv) { v1 += T1[y] * dv; v2 += T2[y] * dv; }
+                    }
+                }
+                a0 += nv * v0;
+                a1 += nv * v1;
+                a2 += nv * v2;
+            }
+            lh_cat[c] = a0;
+            d1 += a1;
+            d2 += a2;
+        }
+        for (int c = 0; c < C; c++) lh += lh_cat[c] * (derv ? 1.0 : adjn[c]);
+        lh += t->invar[p];
+        if (!derv) {
+            const double lp = log(lh) + minsc * LOG_SCALING_THRESHOLD;
+            t->pattern_lh[p] = lp;
+            all += lp * t->freq[p];
+        } else {
+            const double inv = 1.0 / lh, df_frac = d1 * inv, ddf_frac = d2 * inv;
+            const double tmp1 = df_frac * t->freq[p], tmp2 = ddf_frac * t->freq[p];
+            all_df += tmp1;
+            all_ddf += tmp2 - tmp1 * df_frac;
+        }
+    }
+    free(T);
+    free(lut);
+    int st = 0;
+    if (derv) {
+        *df_out = all_df;
+        *ddf_out = all_ddf;
+        if (!isfinite(all_df)) {
+            *df_out = *ddf_out = 0.0;
+            st = 4;
+        }
+    } else if (!isfinite(all)) {
+        /* per-pattern fix-up (tree/phylokernelnonrev.h:1486-1497) */
+        st = 4;
+        all = 0.0;
+        for (size_t p = 0; p < nobs; p++) {
+            if (!isfinite(t->pattern_lh[p])) t->pattern_lh[p] = LOG_SCALING_THRESHOLD * 4;
+            all += t->pattern_lh[p] * t->freq[p];
+        }
+    }
+    if (status) *status = st;
+    return all;
+}
+
+static orc_end nr_end(orc_end e) {
+    orc_end r = e;
+    r.is_tip = e.is_tip & 1 ? (e.is_tip & 4) | 1 : 0;
+    return r;
+}
+
 void orc_update_partials(orc_tree* t, const orc_op* ops, int nops) {
+    if (t->nonrev) {
+        nr_run_ops(t, ops, nops);
+        return;
+    }
     if (nops <= 0) return;
     op_info* infos = (op_info*)malloc(sizeof(op_info) * nops);
     prepare_ops(t, ops, nops, infos);
@@ -693,6 +1048,14 @@ INLINE double branch_body(orc_tree* t, const orc_op* ops, int nops, const op_inf
 
 double orc_likelihood_branch(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
                              double length, double* pattern_lh, int* status) {
+    if (t->nonrev) {
+        nr_run_ops(t, ops, nops);
+        t->nr_node = nr_end(node);
+        t->nr_dad = nr_end(dad);
+        const double lnl = nr_eval(t, length, 0, NULL, NULL, status);
+        if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)(t->P - t->nasc) * 8);
+        return lnl;
+    }
     const int n = t->n, C = t->C;
     const size_t block = t->block;
     op_info* infos = (op_info*)malloc(sizeof(op_info) * (nops > 0 ? nops : 1));
@@ -741,6 +1104,14 @@ double orc_likelihood_branch(orc_tree* t, const orc_op* ops, int nops, orc_end n
 
 /* computeLikelihoodBufferSIMD (tree/phylokernelnew.h:2048-2231) */
 void orc_likelihood_buffer(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad) {
+    if (t->nonrev) {
+        /* no theta buffer in the nonrev kernel set: the ends, evaluated on demand */
+        nr_run_ops(t, ops, nops);
+        t->nr_node = nr_end(node);
+        t->nr_dad = nr_end(dad);
+        t->theta_ready = 1;
+        return;
+    }
     const int n = t->n, C = t->C, safe = t->safe;
     const size_t block = t->block;
     op_info* infos = (op_info*)malloc(sizeof(op_info) * (nops > 0 ? nops : 1));
@@ -804,6 +1175,12 @@ void orc_likelihood_buffer(orc_tree* t, const orc_op* ops, int nops, orc_end nod
 
 /* computeLikelihoodDervSIMD, normal branch-length model (tree/phylokernelnew.h:2233-2633) */
 int orc_likelihood_derv(orc_tree* t, double length, double* df, double* ddf) {
+    if (t->nonrev) {
+        int st = 0;
+        t->last_x = length;
+        nr_eval(t, length, 1, df, ddf, &st);
+        return st;
+    }
     const int n = t->n, C = t->C;
     const size_t block = t->block;
     double* val0 = (double*)amalloc(block * 8);
@@ -875,6 +1252,11 @@ int orc_likelihood_derv(orc_tree* t, double length, double* df, double* ddf) {
 
 /* computeLikelihoodFromBufferSIMD (tree/phylokernelnew.h:3232-3428) */
 double orc_likelihood_from_buffer(orc_tree* t, double length, double* pattern_lh) {
+    if (t->nonrev) {
+        const double lnl = nr_eval(t, length, 0, NULL, NULL, NULL);
+        if (pattern_lh) memcpy(pattern_lh, t->pattern_lh, (size_t)(t->P - t->nasc) * 8);
+        return lnl;
+    }
     const int n = t->n, C = t->C;
     const size_t block = t->block;
     double* val0 = (double*)amalloc(block * 8);

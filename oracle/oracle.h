@@ -53,6 +53,20 @@ void orc_set_model(orc_tree* t, const double* eval, const double* evec, const do
  * lnL, pattern lnL and derivatives are conditioned on variable sites.
  * pattern_lh outputs then hold the npatterns - nasc observed patterns. */
 void orc_set_asc(orc_tree* t, int nasc);
+/* non-reversible and mixture models in the state basis (tree/phylokernelnonrev.h):
+ * per mixture component m < nmix the complex eigensystem (re, im pairs: ceval
+ * [nmix][n], cevec and cinv_evec [nmix][n][n]), the rate matrix [nmix][n][n]
+ * and state_freq [nmix][n]; mix_weights [nmix]; the nrates site-rate
+ * categories; tip [SU+1][n] the plain tip vectors (tip_partial_lh). The
+ * instance's ncat must be nrates (fused_mix_rate, nmix == nrates) or
+ * nrates * nmix. Unrooted trees put the frequencies at the node end of the
+ * evaluated branch; rooted ones take them from the root leaf. Op children
+ * and branch ends carry flags in is_tip: 1 tip, 2 toward the root (the
+ * transposed matrix), 4 the root leaf. Returns 0, or -1 on a shape mismatch. */
+int orc_set_model_nonrev(orc_tree* t, int nmix, int fused, const double* ceval, const double* cevec,
+                         const double* cinv_evec, const double* rate_matrix, const double* state_freq,
+                         const double* mix_weights, const double* site_rates, const double* site_props,
+                         int nrates, const double* tip, int rooted);
 
 void orc_update_partials(orc_tree* t, const orc_op* ops, int nops);
 /* returns tree lnL; status (nullable) 0 ok, 4 non-finite fixed up */

@@ -117,6 +117,31 @@ plk_status plk_set_model(plk_context* ctx, int instance, const double* eval,
                          const double* evec, const double* inv_evec, const double* tip_eigen,
                          const double* cat_rates, const double* cat_props);
 
+/* Non-reversible and mixture models: the state-basis kernel set of
+ * tree/phylokernelnonrev.h (non-reversible models, and any model under
+ * kernel_nonrev), selected by PhyloTree::setLikelihoodKernel when
+ * !model->useRevKernel() (tree/phylotreeavx.cpp:105-125). Per mixture
+ * component m < nmix: ModelMarkov's complex eigensystem as (re, im) pairs
+ * (ceval [nmix][n], cevec and cinv_evec [nmix][n][n], model/modelmarkov.cpp:
+ * 1300-1370), rate_matrix [nmix][n][n] and state_freq [nmix][n];
+ * mix_weights [nmix] (getMixtureWeight); the nrates site-rate categories
+ * (getRate, getProp); tip [SU+1][n] the plain tip vectors (tip_partial_lh,
+ * tree/phylotreesse.cpp:387-420). The instance's ncat is ncat_mix: nrates
+ * with fused_mix_rate (nmix == nrates), else nrates * nmix. rooted = the
+ * tree's `rooted`: 0 puts the frequencies at the node end of the evaluated
+ * branch, 1 takes them from the root leaf. Op children and branch ends then
+ * carry flags in is_tip: 1 a tip, 2 (children) direction TOWARD_ROOT (the
+ * transposed transition matrix), 4 the root leaf (with 1). 4 or 20 states;
+ * plk_set_model switches back to the reversible kernels. Branch calls,
+ * theta/derivative/from-buffer calls and plk_optimize_branch apply it;
+ * batched traversals evaluate such items alone; the batched derivative
+ * call, the device sweep and +ASC refuse it (PLK_ERR_UNSUPPORTED). */
+plk_status plk_set_model_nonrev(plk_context* ctx, int instance, int nmix, int fused,
+                                const double* ceval, const double* cevec, const double* cinv_evec,
+                                const double* rate_matrix, const double* state_freq,
+                                const double* mix_weights, const double* site_rates,
+                                const double* site_props, int nrates, const double* tip, int rooted);
+
 /* Lewis's ascertainment bias correction, ASC_VARIANT and ASC_INFORMATIVE
  * (+ASC, +ASC_INF; model/modelfactory.cpp getASC, tree/phylokernelnew.h:
  * 2272-2273, 2580-2628, 3175-3221, 3385-3425): the last nasc of the instance's

@@ -22,6 +22,7 @@
 #include "plk_codon.h"
 #include "plk_mfma.h"
 #include "plk_pars.h"
+#include "plk_nonrev.h"
 #include "newton.h"
 
 using namespace plkdev;
@@ -138,6 +139,16 @@ struct Instance {
     int nasc = 0;
     double* asc_vals = nullptr;  // more than kAscMax of them: [nasc][4] values, then the 4 sums
     size_t asc_cap = 0;
+    // state-basis models (non-reversible, mixtures): plk_set_model_nonrev
+    bool nonrev = false;
+    int nr_nmix = 0, nr_rooted = 0;
+    double* nr_model = nullptr;   // ceval | cevec | cinv | Q | pi | rate | wt, then int mix[C]
+    size_t nr_model_cap = 0;
+    double* nr_root = nullptr;    // root tables T[3][C][n][n], lookups [3][SU+1][C][n]
+    size_t nr_root_cap = 0;
+    uint8_t* nr_flags = nullptr;  // child flags of the staged ops [2*nops]
+    size_t nr_flags_cap = 0;
+    plk_end nr_node{0, 0}, nr_dad{0, 0};   // ends of the last buffer call
     std::vector<double> hfreq;   // pattern weights (host copy)
     double nsite = 0.0;          // sites of the observed patterns
     double staged_bytes = 0.0, staged_flops = 0.0;   // per-pattern cost of the staged ops
@@ -185,12 +196,12 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
                 K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH,
                 K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_BATCH_CODON, K_PARSIMONY,
-                K_NKIND };
+                K_NR_PREP, K_NR_ROOT, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
                                           "reduce", "rell", "newton_sweep", "batch", "prep_batch",
                                           "traverse_mfma", "root", "batch_dna", "batch_protein",
-                                          "batch_codon", "parsimony"};
+                                          "batch_codon", "parsimony", "nonrev_prep", "nonrev_root"};
 
 struct ProfRec {
     int kind;
@@ -348,7 +359,8 @@ plk_status destroy_instance(plk_context* ctx, int id) {
                     in->root_tab, in->ops, in->tables, in->sw_ops, in->sw_br, in->sw_len,
                     in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res, in->sw_tabs,
                     in->pars_tip_planes, in->pars_planes, in->pars_counts, in->pars_sets,
-                    in->pars_changes, in->pars_ops, in->asc_vals};
+                    in->pars_changes, in->pars_ops, in->asc_vals, in->nr_model, in->nr_root,
+                    in->nr_flags};
     for (void* b : bufs) dfree(ctx, nullptr, b, 0);
     delete in;
     ctx->inst[id] = nullptr;
@@ -428,6 +440,23 @@ ModelDev model_dev(const Instance* in) {
     return m;
 }
 
+NrDev nr_dev(const Instance* in) {
+    const int n = in->d.nstates, C = in->d.ncat, nm = in->nr_nmix;
+    NrDev D;
+    D.ceval = in->nr_model;
+    D.cevec = D.ceval + (size_t)nm * 2 * n;
+    D.cinv = D.cevec + (size_t)nm * 2 * n * n;
+    D.Q = D.cinv + (size_t)nm * 2 * n * n;
+    D.pi = D.Q + (size_t)nm * n * n;
+    D.rate = D.pi + (size_t)nm * n;
+    D.wt = D.rate + C;
+    D.mix = reinterpret_cast<const int*>(D.wt + C);
+    D.n = n;
+    D.C = C;
+    D.rooted = in->nr_rooted;
+    return D;
+}
+
 plk_status check_ops(const Instance* in, const plk_op* ops, int nops) {
     if (nops < 0 || (nops > 0 && !ops)) return fail(PLK_ERR_ARG, "invalid op list");
     for (int o = 0; o < nops; o++) {
@@ -438,10 +467,14 @@ plk_status check_ops(const Instance* in, const plk_op* ops, int nops) {
             return fail(PLK_ERR_UNSUPPORTED, "op %d: %d children (binary nodes only)", o, op.nchildren);
         for (int k = 0; k < 2; k++) {
             const plk_child& ch = op.child[k];
-            const int lim = ch.is_tip ? in->d.ntaxa : in->d.nslots;
+            // state-basis instances: bit 1 toward the root, bit 2 the root leaf
+            const int allowed = in->nonrev ? (kNrTip | kNrTowardRoot | kNrRootLeaf) : 1;
+            if ((ch.is_tip & ~allowed) || ((ch.is_tip & kNrRootLeaf) && !(ch.is_tip & kNrTip)))
+                return fail(PLK_ERR_ARG, "op %d child %d: invalid flags %d", o, k, ch.is_tip);
+            const int lim = (ch.is_tip & kNrTip) ? in->d.ntaxa : in->d.nslots;
             if (ch.id < 0 || ch.id >= lim)
                 return fail(PLK_ERR_ARG, "op %d child %d: id %d out of range", o, k, ch.id);
-            if (!ch.is_tip && ch.id == op.dad_slot)
+            if (!(ch.is_tip & kNrTip) && ch.id == op.dad_slot)
                 return fail(PLK_ERR_ARG, "op %d: child slot equals dad slot", o);
             if (!(ch.length >= 0.0) || !std::isfinite(ch.length))
                 return fail(PLK_ERR_ARG, "op %d child %d: invalid length", o, k);
@@ -451,6 +484,9 @@ plk_status check_ops(const Instance* in, const plk_op* ops, int nops) {
 }
 
 plk_status check_end(const Instance* in, plk_end e, bool allow_tip) {
+    const int allowed = in->nonrev ? (kNrTip | kNrRootLeaf) : 1;
+    if ((e.is_tip & ~allowed) || ((e.is_tip & kNrRootLeaf) && !(e.is_tip & kNrTip)))
+        return fail(PLK_ERR_ARG, "invalid end flags %d", e.is_tip);
     if (e.is_tip) {
         if (!allow_tip) return fail(PLK_ERR_ARG, "branch end must be a partial slot");
         if (e.id < 0 || e.id >= in->d.ntaxa) return fail(PLK_ERR_ARG, "tip id %d out of range", e.id);
@@ -663,6 +699,9 @@ plk_status launch_buffer(plk_context* ctx, Instance* in, double length, int mode
     return fail(PLK_ERR_UNSUPPORTED, "nstates %d not built", in->d.nstates);
 }
 
+template <typename T>
+plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need);
+
 // upload ops and build their transition tables
 plk_status stage_ops(plk_context* ctx, Instance* in, const plk_op* ops, int nops) {
     plk_status st = check_ops(in, ops, nops);
@@ -683,7 +722,24 @@ plk_status stage_ops(plk_context* ctx, Instance* in, const plk_op* ops, int nops
         if (st != PLK_OK) return st;
         in->ops_cap = cap;
     }
+    std::vector<plk_op> stripped;
+    if (in->nonrev) {
+        // the traversal kernels see plain tip bits; the flags go beside them
+        stripped.assign(ops, ops + nops);
+        std::vector<uint8_t> fl(2 * (size_t)nops);
+        for (int o = 0; o < nops; o++)
+            for (int k = 0; k < 2; k++) {
+                fl[2 * o + k] = (uint8_t)(stripped[o].child[k].is_tip & (kNrTowardRoot | kNrRootLeaf));
+                stripped[o].child[k].is_tip &= kNrTip;
+            }
+        st = grow(ctx, in, &in->nr_flags, in->nr_flags_cap, fl.size());
+        if (st != PLK_OK) return st;
+        HIP_TRY(hipMemcpyAsync(in->nr_flags, fl.data(), fl.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));   // fl leaves scope
+        ops = stripped.data();
+    }
     HIP_TRY(hipMemcpyAsync(in->ops, ops, sizeof(plk_op) * nops, hipMemcpyHostToDevice, ctx->stream));
+    if (in->nonrev) HIP_TRY(hipStreamSynchronize(ctx->stream));   // stripped leaves scope
     in->staged_bytes = in->staged_flops = 0.0;
     for (int o = 0; o < nops; o++) {
         double b, f;
@@ -695,7 +751,19 @@ plk_status stage_ops(plk_context* ctx, Instance* in, const plk_op* ops, int nops
     const size_t smem = (n + n * n + SU * n) * sizeof(double);
     const dim3 grid(2 * nops, in->d.ncat);
     const int pe = prof_begin(ctx);
-    if (in->d.precision == PLK_PREC_F32) {
+    if (in->nonrev) {
+        const size_t nsmem = (2 * n + n * n) * sizeof(double);
+        const double* tip = model_dev(in).tip;
+        if (in->d.precision == PLK_PREC_F32) {
+            hipLaunchKernelGGL(k_nr_prep_ops<float>, grid, dim3(kThreads), nsmem, ctx->stream, in->ops,
+                               in->nr_flags, nops, nr_dev(in), tip, in->d.state_unknown, in->table_size,
+                               reinterpret_cast<float*>(in->tables));
+        } else {
+            hipLaunchKernelGGL(k_nr_prep_ops<double>, grid, dim3(kThreads), nsmem, ctx->stream, in->ops,
+                               in->nr_flags, nops, nr_dev(in), tip, in->d.state_unknown, in->table_size,
+                               reinterpret_cast<double*>(in->tables));
+        }
+    } else if (in->d.precision == PLK_PREC_F32) {
         hipLaunchKernelGGL(k_prep_ops<float>, grid, dim3(kThreads), smem, ctx->stream,
                            in->ops, nops, model_dev(in), in->d.nstates, in->d.ncat,
                            in->d.state_unknown, in->table_size, reinterpret_cast<float*>(in->tables));
@@ -799,6 +867,72 @@ plk_status grow(plk_context* ctx, Instance* in, T** p, size_t& cap, size_t need)
     plk_status st = dmalloc(ctx, in, p, sizeof(T) * cap);
     if (st != PLK_OK) cap = 0;
     return st;
+}
+
+template <typename T, int N>
+plk_status launch_nr_root_n(plk_context* ctx, Instance* in, const NrRoot& R, int grid) {
+    switch (in->d.ncat) {
+    case 1: hipLaunchKernelGGL((k_nr_root<T, N, 1>), dim3(grid), dim3(kThreads), 0, ctx->stream, inst_dev(in), R, in->block_sums); break;
+    case 2: hipLaunchKernelGGL((k_nr_root<T, N, 2>), dim3(grid), dim3(kThreads), 0, ctx->stream, inst_dev(in), R, in->block_sums); break;
+    case 4: hipLaunchKernelGGL((k_nr_root<T, N, 4>), dim3(grid), dim3(kThreads), 0, ctx->stream, inst_dev(in), R, in->block_sums); break;
+    case 8: hipLaunchKernelGGL((k_nr_root<T, N, 8>), dim3(grid), dim3(kThreads), 0, ctx->stream, inst_dev(in), R, in->block_sums); break;
+    default: return fail(PLK_ERR_UNSUPPORTED, "ncat %d not built", in->d.ncat);
+    }
+    HIP_TRY(hipGetLastError());
+    return PLK_OK;
+}
+
+// State-basis branch evaluation between the buffered ends at `length`:
+// h_out[0] = lnL (derv = 0); h_out[0..2] = df, ddf, lnL (derv = 1)
+plk_status nr_eval(plk_context* ctx, Instance* in, double length, int derv) {
+    const int n = in->d.nstates, C = in->d.ncat, SU = in->d.state_unknown;
+    const plk_end node = in->nr_node, dad = in->nr_dad;
+    const int dad_tip = dad.is_tip & kNrTip, dad_root = (dad.is_tip & kNrRootLeaf) ? 1 : 0;
+    double* T = in->nr_root;
+    double* lut = T + 3 * (size_t)C * n * n;
+    const size_t smem = (2 * (size_t)n + 3 * (size_t)n * n) * sizeof(double);
+    int pe = prof_begin(ctx);
+    hipLaunchKernelGGL(k_nr_prep_root, dim3(C), dim3(kThreads), smem, ctx->stream, nr_dev(in),
+                       model_dev(in).tip, SU, length, derv, dad_tip, dad_root, T, lut);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_NR_PREP, 3.0 * C * n * (n + SU + 1.0) * 8, 0.0);
+    NrRoot R{T, lut, node.id, dad_tip ? 0 : dad.id, dad_tip, dad_tip ? dad.id : 0, derv};
+    const int grid = (in->Ppad + kThreads - 1) / kThreads;
+    pe = prof_begin(ctx);
+    const bool f32 = in->d.precision == PLK_PREC_F32;
+    plk_status st;
+    if (n == 4) st = f32 ? launch_nr_root_n<float, 4>(ctx, in, R, grid) : launch_nr_root_n<double, 4>(ctx, in, R, grid);
+    else st = f32 ? launch_nr_root_n<float, kProtN>(ctx, in, R, grid) : launch_nr_root_n<double, kProtN>(ctx, in, R, grid);
+    if (st != PLK_OK) return st;
+    {
+        const double nint = dad_tip ? 1.0 : 2.0, ncnt = in->d.safe_numeric ? C : 1;
+        prof_end(ctx, pe, K_NR_ROOT,
+                 (double)in->d.npatterns * (in->block * in->elem * nint + in->scale_elem * ncnt * nint + 24.0),
+                 (double)in->d.npatterns * C * (dad_tip ? 2.0 * n : 2.0 * n * (n + 1)) * (derv ? 3 : 1));
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->block_sums, grid, 3,
+                       in->red_out);
+    HIP_TRY(hipGetLastError());
+    return fetch_scalars(ctx, in, 3);
+}
+
+// plk_likelihood_branch / plk_likelihood_from_buffer of a state-basis
+// instance once its ends are set: lnL, the per-pattern fix-up
+// (tree/phylokernelnonrev.h:1486-1497), pattern lnL
+plk_status nr_lnl(plk_context* ctx, Instance* in, double length, double* lnl, double* pattern_lh) {
+    plk_status st = nr_eval(ctx, in, length, 0);
+    if (st != PLK_OK) return st;
+    *lnl = ctx->h_out[0];
+    in->ptn_lh_ready = true;
+    const bool nonfinite = !std::isfinite(*lnl);
+    if (nonfinite && (st = fixup_nonfinite(ctx, in, lnl, pattern_lh)) != PLK_OK) return st;
+    if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch; per-pattern fix-up applied");
+    if (pattern_lh) {
+        HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * in->d.npatterns,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return PLK_OK;
 }
 
 }  // namespace
@@ -1047,6 +1181,7 @@ plk_status plk_set_asc(plk_context* ctx, int id, int nasc) {
     if (!in) return fail(PLK_ERR_ARG, "invalid instance");
     if (nasc < 0 || nasc >= in->d.npatterns)
         return fail(PLK_ERR_ARG, "%d unobserved patterns of %d", nasc, in->d.npatterns);
+    if (nasc && in->nonrev) return fail(PLK_ERR_UNSUPPORTED, "+ASC with a state-basis model");
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (nasc > kAscMax) {
         plk_status st = grow(ctx, in, &in->asc_vals, in->asc_cap, 4 * ((size_t)nasc + 1));
@@ -1080,6 +1215,73 @@ plk_status plk_set_model(plk_context* ctx, int id, const double* eval, const dou
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     in->model_set = true;
+    in->nonrev = false;
+    in->theta_ready = false;
+    return PLK_OK;
+}
+
+plk_status plk_set_model_nonrev(plk_context* ctx, int id, int nmix, int fused, const double* ceval,
+                                const double* cevec, const double* cinv_evec, const double* rate_matrix,
+                                const double* state_freq, const double* mix_weights, const double* site_rates,
+                                const double* site_props, int nrates, const double* tip, int rooted) {
+    Instance* in = get_instance(ctx, id);
+    if (!in || !ceval || !cevec || !cinv_evec || !rate_matrix || !state_freq || !mix_weights || !site_rates ||
+        !site_props || !tip)
+        return fail(PLK_ERR_ARG, "null argument");
+    const int n = in->d.nstates, C = in->d.ncat, SU = in->d.state_unknown;
+    if (n != 4 && n != kProtN) return fail(PLK_ERR_UNSUPPORTED, "state-basis models for %d states", n);
+    if (nmix < 1 || nrates < 1 || C != (fused ? nrates : nrates * nmix) || (fused && nmix != nrates))
+        return fail(PLK_ERR_ARG, "%d categories for %d mixture components and %d rates%s", C, nmix, nrates,
+                    fused ? " (fused)" : "");
+    if (in->nasc) return fail(PLK_ERR_UNSUPPORTED, "+ASC with a state-basis model");
+    const size_t nn = (size_t)n * n;
+    std::vector<double> h;
+    h.insert(h.end(), ceval, ceval + (size_t)nmix * 2 * n);
+    h.insert(h.end(), cevec, cevec + (size_t)nmix * 2 * nn);
+    h.insert(h.end(), cinv_evec, cinv_evec + (size_t)nmix * 2 * nn);
+    h.insert(h.end(), rate_matrix, rate_matrix + (size_t)nmix * nn);
+    h.insert(h.end(), state_freq, state_freq + (size_t)nmix * n);
+    // ncat_mix categories: rate category c % ncat of component c / denom
+    // (tree/phylokernelnonrev.h:1107-1114)
+    const int denom = fused ? 1 : nrates;
+    std::vector<double> rate(C), wt(C);
+    std::vector<int> mix(C);
+    for (int c = 0; c < C; c++) {
+        mix[c] = c / denom;
+        rate[c] = site_rates[c % nrates];
+        wt[c] = site_props[c % nrates] * mix_weights[mix[c]];
+    }
+    h.insert(h.end(), rate.begin(), rate.end());
+    h.insert(h.end(), wt.begin(), wt.end());
+    for (double v : h)
+        if (!std::isfinite(v)) return fail(PLK_ERR_ARG, "non-finite model parameter");
+    for (int i = 0; i < (SU + 1) * n; i++)
+        if (!std::isfinite(tip[i])) return fail(PLK_ERR_ARG, "non-finite tip vector");
+    const size_t ints = (C * sizeof(int) + 7) / 8;
+    const size_t words = h.size() + ints;
+    h.resize(words, 0.0);
+    std::memcpy(h.data() + words - ints, mix.data(), C * sizeof(int));
+    // the traversal kernels' model: identity output transform, plain tip vectors
+    std::vector<double> m;
+    m.resize(n, 0.0);
+    for (int pass = 0; pass < 2; pass++)
+        for (int x = 0; x < n; x++)
+            for (int i = 0; i < n; i++) m.push_back(x == i ? 1.0 : 0.0);
+    m.insert(m.end(), tip, tip + (size_t)(SU + 1) * n);
+    m.insert(m.end(), rate.begin(), rate.end());
+    m.insert(m.end(), wt.begin(), wt.end());
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    plk_status st = grow(ctx, in, &in->nr_model, in->nr_model_cap, words);
+    if (st != PLK_OK) return st;
+    st = grow(ctx, in, &in->nr_root, in->nr_root_cap, 3 * (size_t)C * nn + 3 * (size_t)(SU + 1) * C * n);
+    if (st != PLK_OK) return st;
+    HIP_TRY(hipMemcpyAsync(in->nr_model, h.data(), sizeof(double) * words, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(in->model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    in->nr_nmix = nmix;
+    in->nr_rooted = rooted ? 1 : 0;
+    in->nonrev = true;
+    in->model_set = true;
     in->theta_ready = false;
     return PLK_OK;
 }
@@ -1111,6 +1313,14 @@ plk_status plk_likelihood_branch(plk_context* ctx, int id, const plk_op* ops, in
     std::lock_guard<std::mutex> lock(ctx->mu);
     st = stage_ops(ctx, in, ops, nops);
     if (st != PLK_OK) return st;
+    if (in->nonrev) {
+        const plk_end none{0, 0};
+        if (nops && (st = launch_traverse(ctx, in, nops, MODE_PARTIALS, none, none)) != PLK_OK) return st;
+        in->nr_node = node;
+        in->nr_dad = dad;
+        in->theta_ready = false;
+        return nr_lnl(ctx, in, length, lnl, pattern_lh);
+    }
     int pe = prof_begin(ctx);
     hipLaunchKernelGGL(k_prep_root, dim3(1), dim3(kThreads), 0, ctx->stream, model_dev(in),
                        in->d.nstates, in->d.ncat, in->d.state_unknown, length, dad.is_tip,
@@ -1154,6 +1364,16 @@ plk_status plk_likelihood_buffer(plk_context* ctx, int id, const plk_op* ops, in
     std::lock_guard<std::mutex> lock(ctx->mu);
     st = stage_ops(ctx, in, ops, nops);
     if (st != PLK_OK) return st;
+    if (in->nonrev) {
+        // no theta buffer in the nonrev kernel set: the ends, evaluated on demand
+        const plk_end none{0, 0};
+        if (nops && (st = launch_traverse(ctx, in, nops, MODE_PARTIALS, none, none)) != PLK_OK) return st;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        in->nr_node = node;
+        in->nr_dad = dad;
+        in->theta_ready = true;
+        return PLK_OK;
+    }
     st = launch_traverse(ctx, in, nops, MODE_THETA, node, dad);
     if (st != PLK_OK) return st;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1170,6 +1390,17 @@ plk_status plk_likelihood_derv(plk_context* ctx, int id, double length, double* 
     if (!in->theta_ready) return fail(PLK_ERR_STATE, "theta buffer not computed");
     if (!std::isfinite(length)) return fail(PLK_ERR_ARG, "invalid length");
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (in->nonrev) {
+        if ((st = nr_eval(ctx, in, length, 1)) != PLK_OK) return st;
+        *df = ctx->h_out[0];
+        *ddf = ctx->h_out[1];
+        if (lnl) *lnl = ctx->h_out[2];
+        if (!std::isfinite(*df)) {
+            *df = *ddf = 0.0;
+            return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-derivative");
+        }
+        return PLK_OK;
+    }
     st = launch_buffer(ctx, in, length, BUF_DERV);
     if (st != PLK_OK) return st;
     if ((st = fetch_scalars(ctx, in, 3)) != PLK_OK) return st;
@@ -1203,6 +1434,7 @@ plk_status plk_likelihood_from_buffer(plk_context* ctx, int id, double length, d
     if (!lnl) return fail(PLK_ERR_ARG, "null output");
     if (!in->theta_ready) return fail(PLK_ERR_STATE, "theta buffer not computed");
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (in->nonrev) return nr_lnl(ctx, in, length, lnl, pattern_lh);
     st = launch_buffer(ctx, in, length, BUF_LNL);
     if (st != PLK_OK) return st;
     if ((st = fetch_scalars(ctx, in, 3)) != PLK_OK) return st;
@@ -1719,7 +1951,7 @@ plk_status plk_batch_evaluate(plk_context* ctx, const plk_batch_item* items, con
     bool nonfinite = false;
     for (int i = 0; i < nitems; i++) {
         const Instance* in = ins[i];
-        if (in->d.ncat == 4 && in->nasc == 0 &&
+        if (in->d.ncat == 4 && in->nasc == 0 && !in->nonrev &&
             (in->d.nstates == 4 || in->d.nstates == 20 || in->d.nstates == kCodonN)) {
             fused.push_back(items[i]);
             fused_ins.push_back(ins[i]);
@@ -1769,6 +2001,8 @@ plk_status derv_batch_run(plk_context* ctx, const int* instances, const double* 
         if ((st = ready(ins[i])) != PLK_OK) return st;
         if (!ins[i]->theta_ready) return fail(PLK_ERR_STATE, "item %d: theta buffer not computed", i);
         if (ins[i]->nasc) return fail(PLK_ERR_UNSUPPORTED, "item %d: +ASC instances take plk_likelihood_derv", i);
+        if (ins[i]->nonrev)
+            return fail(PLK_ERR_UNSUPPORTED, "item %d: state-basis instances take plk_likelihood_derv", i);
         if (!std::isfinite(lengths[i])) return fail(PLK_ERR_ARG, "item %d: invalid length", i);
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1890,6 +2124,7 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     plk_status st = ready(in);
     if (st != PLK_OK) return st;
     if (in->nasc) return fail(PLK_ERR_UNSUPPORTED, "the device sweep does not apply +ASC");
+    if (in->nonrev) return fail(PLK_ERR_UNSUPPORTED, "the device sweep runs reversible models");
     if (nops < 0 || (nops > 0 && !ops) || nbranches < 1 || !branches || !lengths || nedges < 1 ||
         max_nr < 1 || !(min_len > 0.0) || !(max_len > min_len))
         return fail(PLK_ERR_ARG, "invalid sweep arguments");
@@ -2057,6 +2292,16 @@ plk_status plk_parsimony_branch(plk_context* ctx, int id, const plk_op* ops, int
     if (st != PLK_OK) return st;
     if ((st = check_end(in, node, true)) != PLK_OK) return st;
     if ((st = check_end(in, dad, true)) != PLK_OK) return st;
+    // parsimony has no direction: a state-basis instance's flags are dropped
+    std::vector<plk_op> plain;
+    if (in->nonrev && nops > 0) {
+        plain.assign(ops, ops + nops);
+        for (plk_op& o : plain)
+            for (int k = 0; k < 2; k++) o.child[k].is_tip &= kNrTip;
+        ops = plain.data();
+        node.is_tip &= kNrTip;
+        dad.is_tip &= kNrTip;
+    }
     if (node.is_tip && dad.is_tip && in->d.ntaxa > 2) return fail(PLK_ERR_ARG, "branch between two tips");
     if (in->d.nstates == kCodonN && in->d.ntaxa > 65535)
         return fail(PLK_ERR_UNSUPPORTED, "%d taxa: codon change counts are 16-bit", in->d.ntaxa);

@@ -261,6 +261,22 @@ class Instance:
               "plk_parsimony_branch")
         return score.value, sub.value
 
+    def set_model_nonrev(self, nm, tip, rooted=False):
+        """State-basis (non-reversible / mixture) model: nm holds ceval
+        [nmix][n], cevec and cinv_evec [nmix][n][n] (complex), Q [nmix][n][n],
+        pi [nmix][n], mixw [nmix], rates and props [nrates] and fused; tip
+        [SU+1][n] the plain tip vectors. Op children and ends then carry the
+        flags 2 (toward the root) and 4 (the root leaf) in is_tip."""
+        def cplx(a):
+            a = np.asarray(a, np.complex128)
+            return np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1), np.float64)
+        arrs = [cplx(nm["ceval"]), cplx(nm["cevec"]), cplx(nm["cinv_evec"])]
+        arrs += [np.ascontiguousarray(nm[k], np.float64) for k in ("Q", "pi", "mixw", "rates", "props")]
+        t = np.ascontiguousarray(tip, np.float64)
+        check(lib().plk_set_model_nonrev(self.ctx.handle, self.id, len(nm["mixw"]), int(bool(nm["fused"])),
+                                         *[dptr(a) for a in arrs], len(nm["rates"]), dptr(t),
+                                         int(bool(rooted))), "plk_set_model_nonrev")
+
     def set_asc(self, nasc):
         """Lewis's ascertainment bias correction: the last nasc patterns are the
         unobserved ones (weight 0, +I mass 0): the constant patterns for +ASC,

@@ -8,12 +8,7 @@ import itertools
 import numpy as np
 from scipy.linalg import expm
 
-from iqtree2_amd.engine import OP_DTYPE
-
-
-class End:
-    def __init__(self, is_tip, id_):
-        self.is_tip, self.id = int(is_tip), int(id_)
+from iqtree2_amd.engine import OP_DTYPE, end as End
 
 
 def stationary(Q):

@@ -485,6 +485,113 @@ def cpu_modelfinder(ia, w, threads):
 
 
 # ---------------------------------------------------------------------------
+# non-reversible model through the state-basis kernels, DNA 200 x 100,000
+def nonrev_inputs(w, seed=7):
+    """A random non-reversible DNA rate matrix (every off-diagonal rate free,
+    normalised to one substitution per unit time at its stationary
+    distribution) with the workload's rate categories, and the plain DNA tip
+    vectors (ambiguity code = bitmask + 3, STATE_UNKNOWN = every state)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    Q = rng.uniform(0.05, 2.0, (4, 4))
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(axis=1))
+    ev, vl = np.linalg.eig(Q.T)
+    pi = np.real(vl[:, np.argmin(np.abs(ev))])
+    pi /= pi.sum()
+    Q /= -(pi * np.diag(Q)).sum()
+    lam, V = np.linalg.eig(Q)
+    m = w.model_arrays()
+    nm = dict(Q=Q[None], pi=pi[None], ceval=lam[None], cevec=V[None], cinv_evec=np.linalg.inv(V)[None],
+              mixw=np.ones(1), rates=m["rates"], props=m["props"], fused=False)
+    SU = w.state_unknown
+    tip = np.zeros((SU + 1, 4))
+    for s in range(SU + 1):
+        if s < 4:
+            tip[s, s] = 1.0
+        elif s == SU:
+            tip[s] = 1.0
+        else:
+            tip[s] = [(s - 3) >> x & 1 for x in range(4)]
+    return nm, tip
+
+
+def leg_nonrev(ia, ctx, w, reps=10):
+    """Full traversal + branch lnL, and derivative calls, of a non-reversible
+    model (the state-basis kernel set) against the reversible kernels on the
+    same tree and patterns."""
+    import numpy as np
+    nm, tip = nonrev_inputs(w)
+    inst = ia.Instance(ctx, w.ntaxa, 4, w.ncat, w.npatterns, w.ntaxa, w.state_unknown)
+    inst.set_patterns(w.get("states"), w.get("freq"), np.zeros(w.npatterns))
+    inst.set_model_nonrev(nm, tip)
+    rev = ia.Instance.from_workload(ctx, w)
+    tree = ia.PhyloTree(None, w)
+    try:
+        a, b = tree.branches()[0]
+        ops, node, dad = tree.plan_full(a, b)
+        length = tree.get_length(a, b)
+        inst.likelihood_branch(ops, node, dad, length)
+        rev.likelihood_branch(ops, node, dad, length)
+        ctx.synchronize()
+        ctx.profile_start()
+        t = time.perf_counter()
+        for _ in range(reps):
+            lnl = inst.likelihood_branch(ops, node, dad, length)
+        dt = (time.perf_counter() - t) / reps
+        prof = ctx.profile_stop()
+        t = time.perf_counter()
+        for _ in range(reps):
+            rev.likelihood_branch(ops, node, dad, length)
+        dt_rev = (time.perf_counter() - t) / reps
+        inst.likelihood_buffer(ops, node, dad)
+        ctx.profile_start()
+        t = time.perf_counter()
+        for k in range(5 * reps):
+            inst.likelihood_derv(length * (1.0 + 0.01 * k))
+        dtd = (time.perf_counter() - t) / (5 * reps)
+        profd = ctx.profile_stop()
+        P = w.npatterns
+        return {"lnl": lnl, "ops": len(ops), "patterns": P, "traversal_ms": 1e3 * dt,
+                "traversal_ple_per_s": P / dt, "reversible_traversal_ms": 1e3 * dt_rev,
+                "derivative_call_us": 1e6 * dtd,
+                "kernels_us": {k: 1e3 * v["total_ms"] / v["launches"] for k, v in prof.items()
+                               if v["launches"]},
+                "roofline": roofline_of(prof, ("traverse", "nonrev_root"), "{kernel}_nonrev_dna_200x100k"),
+                "roofline_derivative": roofline_of(profd, ("nonrev_root",), "{kernel}_nonrev_derv_dna_200x100k")}
+    finally:
+        inst.close()
+        rev.close()
+        tree.close()
+
+
+def cpu_nonrev(ia, w, budget_s, threads):
+    """The oracle's state-basis kernels on the same full traversal and branch."""
+    import numpy as np
+    from oracle.binding import OracleTree
+    nm, tip = nonrev_inputs(w)
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree(w.ntaxa, 4, w.ncat, w.npatterns, w.ntaxa, w.state_unknown, False, threads)
+    orc.set_patterns(w.get("states"), w.get("freq"), np.zeros(w.npatterns))
+    orc.set_model_nonrev(nm, tip)
+    try:
+        a, b = planner.branches()[0]
+        ops, node, dad = planner.plan_full(a, b)
+        length = planner.get_length(a, b)
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s or n == 0:
+            lnl = orc.likelihood_branch(ops, node, dad, length)
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+    finally:
+        planner.close()
+    return {"traversal_ple_per_s": w.npatterns / dt, "traversal_ms": 1e3 * dt, "lnl": lnl,
+            "cores": threads, "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"{n} full traversals + branch lnL of the 200 x 100k workload"}
+
+
+# ---------------------------------------------------------------------------
 # matrix-core traversals: protein 100 x 20,000 and codon 64 x 5,000
 def leg_mfma(ia, ctx, reps=3):
     from iqtree2_amd import workloads as wl
@@ -651,7 +758,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--only", choices=("partitions", "dna", "mfma", "serving", "copy", "nni",
-                                       "parsimony", "modelfinder"),
+                                       "parsimony", "modelfinder", "nonrev"),
                     help="run one leg alone (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -683,6 +790,13 @@ def main():
                 out = leg_parsimony(ia, ctx, w)
                 if not args.no_cpu:
                     out["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
+                w.close()
+            elif args.only == "nonrev":
+                w = wl.dna_200x100k()
+                out = leg_nonrev(ia, ctx, w)
+                if not args.no_cpu:
+                    out["cpu_baseline"] = cpu_nonrev(ia, w, 5.0, host_threads())
+                    out["lnl_vs_oracle"] = out["lnl"] - out["cpu_baseline"]["lnl"]
                 w.close()
             elif args.only == "modelfinder":
                 w = wl.dna_200x100k()
@@ -720,6 +834,10 @@ def main():
             dna["modelfinder_gamma"] = leg_modelfinder(ia, ctx, w, parts)
             if not args.no_cpu:
                 dna["modelfinder_gamma"]["cpu_baseline"] = cpu_modelfinder(ia, w, host_threads())
+            dna["nonrev"] = leg_nonrev(ia, ctx, w)
+            if not args.no_cpu:
+                dna["nonrev"]["cpu_baseline"] = cpu_nonrev(ia, w, 5.0, host_threads())
+                dna["nonrev"]["lnl_vs_oracle"] = dna["nonrev"]["lnl"] - dna["nonrev"]["cpu_baseline"]["lnl"]
             w.close()
             log(f"dna: {json.dumps(dna)}")
         if not args.no_mfma:

@@ -897,7 +897,8 @@ plk_status nr_eval(plk_context* ctx, Instance* in, double length, int derv) {
     HIP_TRY(hipGetLastError());
     prof_end(ctx, pe, K_NR_PREP, 3.0 * C * n * (n + SU + 1.0) * 8, 0.0);
     NrRoot R{T, lut, node.id, dad_tip ? 0 : dad.id, dad_tip, dad_tip ? dad.id : 0, derv};
-    const int grid = (in->Ppad + kThreads - 1) / kThreads;
+    // one thread per (pattern, category)
+    const int grid = (int)(((size_t)in->Ppad * C + kThreads - 1) / kThreads);
     pe = prof_begin(ctx);
     const bool f32 = in->d.precision == PLK_PREC_F32;
     plk_status st;

@@ -187,105 +187,112 @@ struct NrRoot {
     int node_slot, dad_slot, dad_tip, dad_taxon, derv;
 };
 
-// The branch lnL (derv = 0) or df, ddf and lnL (derv = 1) per pattern, one
-// thread per pattern: sum_c sum_x N[c][x] sum_y Tk[c][x][y] D[c][y]. Safe
-// scaling: the lnL takes per-category sums of both ends' counters against
-// their minimum (tree/phylokernelnonrev.h:1422-1442), the derivatives each
-// end against its own minimum (893-947). Block sums [grid][3]: lnL * freq, or
-// df * freq, ddf terms, lnL * freq.
+// The branch lnL (derv = 0) or df, ddf and lnL (derv = 1): sum_c sum_x
+// N[c][x] sum_y Tk[c][x][y] D[c][y], one thread per (pattern, category), the
+// C threads of a pattern adjacent in the wavefront and combined by lane
+// shuffles. Safe scaling: the lnL takes per-category sums of both ends'
+// counters against their minimum (tree/phylokernelnonrev.h:1422-1442), the
+// derivatives each end against its own minimum (893-947). Block sums
+// [grid][3]: lnL * freq, or df * freq, ddf terms, lnL * freq.
+template <int C>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < C; m <<= 1) v += __shfl_xor(v, m, C);
+    return v;
+}
+template <int C>
+__device__ inline int group_min(int v) {
+#pragma unroll
+    for (int m = 1; m < C; m <<= 1) v = min(v, __shfl_xor(v, m, C));
+    return v;
+}
+
 template <typename T, int N, int C>
 __global__ void __launch_bounds__(kThreads)
 k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
     using S = typename Prec<T>::scale_t;
     __shared__ double sh[3 * (kThreads / 64)];
-    const int p = blockIdx.x * kThreads + threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
+    const int tid = blockIdx.x * kThreads + threadIdx.x;
+    const int p = tid / C, c = tid % C;
     const int nobs = I.P - I.nasc;
-    if (p < nobs) {
-        const T* pn = reinterpret_cast<const T*>(I.partial) + R.node_slot * I.slot_stride + (size_t)p * N * C;
-        const S* sn = reinterpret_cast<const S*>(I.scale) + R.node_slot * I.scale_stride;
-        const T* pd = R.dad_tip ? nullptr
-                                : reinterpret_cast<const T*>(I.partial) + R.dad_slot * I.slot_stride + (size_t)p * N * C;
-        const S* sd = R.dad_tip ? nullptr : reinterpret_cast<const S*>(I.scale) + R.dad_slot * I.scale_stride;
-        const int s = R.dad_tip ? I.states[(size_t)R.dad_taxon * I.Ppad + p] : 0;
-        const size_t ls = (size_t)(I.SU + 1) * C * N;
-        double a0[C], a1[C], a2[C];
+    const bool live = p < nobs;
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    int a = 0, b = 0;
+    if (live) {
+        const T* pn = reinterpret_cast<const T*>(I.partial) + R.node_slot * I.slot_stride + ((size_t)p * C + c) * N;
+        if (R.dad_tip) {
+            const int s = I.states[(size_t)R.dad_taxon * I.Ppad + p];
+            const size_t ls = (size_t)(I.SU + 1) * C * N;
+            const double* l0 = R.lut + ((size_t)s * C + c) * N;
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-            double u0 = 0.0, u1 = 0.0, u2 = 0.0;
             for (int x = 0; x < N; x++) {
-                const double nv = (double)pn[c * N + x];
+                const double nv = (double)pn[x];
+                u0 = fma(nv, l0[x], u0);
+                if (R.derv) {
+                    u1 = fma(nv, l0[ls + x], u1);
+                    u2 = fma(nv, l0[2 * ls + x], u2);
+                }
+            }
+        } else {
+            const T* pd = reinterpret_cast<const T*>(I.partial) + R.dad_slot * I.slot_stride + ((size_t)p * C + c) * N;
+            double dv[N];
+#pragma unroll
+            for (int y = 0; y < N; y++) dv[y] = (double)pd[y];
+            const double* T0 = R.T + (size_t)c * N * N;
+            const double* T1 = T0 + (size_t)C * N * N;
+            const double* T2 = T1 + (size_t)C * N * N;
+            for (int x = 0; x < N; x++) {
                 double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-                if (R.dad_tip) {
-                    const size_t o = ((size_t)s * C + c) * N + x;
-                    v0 = R.lut[o];
+#pragma unroll
+                for (int y = 0; y < N; y++) {
+                    v0 = fma(T0[x * N + y], dv[y], v0);
                     if (R.derv) {
-                        v1 = R.lut[ls + o];
-                        v2 = R.lut[2 * ls + o];
-                    }
-                } else {
-                    const double* T0 = R.T + ((size_t)c * N + x) * N;
-                    const double* T1 = T0 + (size_t)C * N * N;
-                    const double* T2 = T1 + (size_t)C * N * N;
-                    for (int y = 0; y < N; y++) {
-                        const double dv = (double)pd[c * N + y];
-                        v0 = fma(T0[y], dv, v0);
-                        if (R.derv) {
-                            v1 = fma(T1[y], dv, v1);
-                            v2 = fma(T2[y], dv, v2);
-                        }
+                        v1 = fma(T1[x * N + y], dv[y], v1);
+                        v2 = fma(T2[x * N + y], dv[y], v2);
                     }
                 }
+                const double nv = (double)pn[x];
                 u0 = fma(nv, v0, u0);
                 u1 = fma(nv, v1, u1);
                 u2 = fma(nv, v2, u2);
             }
-            a0[c] = u0;
-            a1[c] = u1;
-            a2[c] = u2;
         }
-        // scale counters
-        double fl[C], fd[C];
-        int minsc;
         if (I.safe) {
-            int sum[C], mn = 0, md = 0;
-            minsc = 0;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int a = sn[(size_t)p * C + c], b = sd ? (int)sd[(size_t)p * C + c] : 0;
-                sum[c] = a + b;
-                if (c == 0 || sum[c] < minsc) minsc = sum[c];
-                if (c == 0 || a < mn) mn = a;
-                if (c == 0 || b < md) md = b;
-            }
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int a = sn[(size_t)p * C + c], b = sd ? (int)sd[(size_t)p * C + c] : 0;
-                fl[c] = Prec<T>::safe_adjust(sum[c] - minsc);
-                fd[c] = Prec<T>::safe_adjust(a - mn) * Prec<T>::safe_adjust(b - md);
-            }
-        } else {
-            minsc = (int)sn[p] + (sd ? (int)sd[p] : 0);
-#pragma unroll
-            for (int c = 0; c < C; c++) fl[c] = fd[c] = 1.0;
+            a = reinterpret_cast<const S*>(I.scale)[R.node_slot * I.scale_stride + (size_t)p * C + c];
+            b = R.dad_tip ? 0 : (int)reinterpret_cast<const S*>(I.scale)[R.dad_slot * I.scale_stride + (size_t)p * C + c];
         }
-        double lh = 0.0;
-#pragma unroll
-        for (int c = 0; c < C; c++) lh += a0[c] * fl[c];
-        lh += I.invar[p];
-        const double lp = log(lh) + minsc * Prec<T>::log_unit;
+    }
+    // every lane takes part in the shuffles
+    int minsc;
+    double fl = 1.0, fd = 1.0;
+    if (I.safe) {
+        minsc = group_min<C>(a + b);
+        const int mn = group_min<C>(a), md = group_min<C>(b);
+        fl = Prec<T>::safe_adjust(a + b - minsc);
+        fd = Prec<T>::safe_adjust(a - mn) * Prec<T>::safe_adjust(b - md);
+    } else {
+        minsc = 0;
+        if (live) {
+            const S* sn = reinterpret_cast<const S*>(I.scale) + R.node_slot * I.scale_stride;
+            minsc = (int)sn[p];
+            if (!R.dad_tip) minsc += (int)reinterpret_cast<const S*>(I.scale)[R.dad_slot * I.scale_stride + p];
+        }
+    }
+    const double lh = group_sum<C>(u0 * fl);
+    double l = 0.0, d1 = 0.0, d2 = 0.0;
+    if (R.derv) {
+        l = group_sum<C>(u0 * fd);
+        d1 = group_sum<C>(u1 * fd);
+        d2 = group_sum<C>(u2 * fd);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (live && c == 0) {
+        const double lp = log(lh + I.invar[p]) + minsc * Prec<T>::log_unit;
         const double fq = I.freq[p];
         if (!R.derv) {
             I.ptn_lh[p] = lp;
             acc[0] = lp * fq;
         } else {
-            double l = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                l += a0[c] * fd[c];
-                d1 += a1[c] * fd[c];
-                d2 += a2[c] * fd[c];
-            }
             const double inv = 1.0 / (l + I.invar[p]);
             const double df_frac = d1 * inv, ddf_frac = d2 * inv;
             const double tmp1 = df_frac * fq, tmp2 = ddf_frac * fq;
@@ -293,7 +300,7 @@ k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
             acc[1] = tmp2 - tmp1 * df_frac;
             acc[2] = lp * fq;
         }
-    } else if (p < I.Ppad && !R.derv) {
+    } else if (!live && c == 0 && p < I.Ppad && !R.derv) {
         I.ptn_lh[p] = 0.0;
     }
     block_sum<3>(acc, sh);

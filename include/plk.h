@@ -134,8 +134,9 @@ plk_status plk_set_model(plk_context* ctx, int instance, const double* eval,
  * transposed transition matrix), 4 the root leaf (with 1). 4 or 20 states;
  * plk_set_model switches back to the reversible kernels. Branch calls,
  * theta/derivative/from-buffer calls and plk_optimize_branch apply it;
- * batched traversals evaluate such items alone; the batched derivative
- * call, the device sweep and +ASC refuse it (PLK_ERR_UNSUPPORTED). */
+ * +ASC applies as for reversible models (tree/phylokernelnonrev.h:1000-1047,
+ * 1455-1515); batched traversals evaluate such items alone; the batched
+ * derivative call and the device sweep refuse it (PLK_ERR_UNSUPPORTED). */
 plk_status plk_set_model_nonrev(plk_context* ctx, int instance, int nmix, int fused,
                                 const double* ceval, const double* cevec, const double* cinv_evec,
                                 const double* rate_matrix, const double* state_freq,

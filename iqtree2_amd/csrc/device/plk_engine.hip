@@ -927,8 +927,13 @@ plk_status nr_lnl(plk_context* ctx, Instance* in, double length, double* lnl, do
     in->ptn_lh_ready = true;
     const bool nonfinite = !std::isfinite(*lnl);
     if (nonfinite && (st = fixup_nonfinite(ctx, in, lnl, pattern_lh)) != PLK_OK) return st;
+    if (in->nasc) {
+        double a[4];
+        if ((st = asc_sums(ctx, in, a)) != PLK_OK) return st;
+        if ((st = asc_correct_lnl(ctx, in, a[0], lnl, pattern_lh)) != PLK_OK) return st;
+    }
     if (nonfinite) return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-branch; per-pattern fix-up applied");
-    if (pattern_lh) {
+    if (pattern_lh && !in->nasc) {
         HIP_TRY(hipMemcpyAsync(pattern_lh, in->ptn_lh, sizeof(double) * in->d.npatterns,
                                hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1182,7 +1187,6 @@ plk_status plk_set_asc(plk_context* ctx, int id, int nasc) {
     if (!in) return fail(PLK_ERR_ARG, "invalid instance");
     if (nasc < 0 || nasc >= in->d.npatterns)
         return fail(PLK_ERR_ARG, "%d unobserved patterns of %d", nasc, in->d.npatterns);
-    if (nasc && in->nonrev) return fail(PLK_ERR_UNSUPPORTED, "+ASC with a state-basis model");
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (nasc > kAscMax) {
         plk_status st = grow(ctx, in, &in->asc_vals, in->asc_cap, 4 * ((size_t)nasc + 1));
@@ -1234,7 +1238,6 @@ plk_status plk_set_model_nonrev(plk_context* ctx, int id, int nmix, int fused, c
     if (nmix < 1 || nrates < 1 || C != (fused ? nrates : nrates * nmix) || (fused && nmix != nrates))
         return fail(PLK_ERR_ARG, "%d categories for %d mixture components and %d rates%s", C, nmix, nrates,
                     fused ? " (fused)" : "");
-    if (in->nasc) return fail(PLK_ERR_UNSUPPORTED, "+ASC with a state-basis model");
     const size_t nn = (size_t)n * n;
     std::vector<double> h;
     h.insert(h.end(), ceval, ceval + (size_t)nmix * 2 * n);
@@ -1396,6 +1399,16 @@ plk_status plk_likelihood_derv(plk_context* ctx, int id, double length, double* 
         *df = ctx->h_out[0];
         *ddf = ctx->h_out[1];
         if (lnl) *lnl = ctx->h_out[2];
+        if (in->nasc) {
+            // ASC_Lewis (tree/phylokernelnonrev.h:1040-1047)
+            double a[4];
+            if ((st = asc_sums(ctx, in, a)) != PLK_OK) return st;
+            const double pv = 1.0 - a[1];
+            const double df_frac = a[2] / pv, ddf_frac = a[3] / pv;
+            *df += in->nsite * df_frac;
+            *ddf += in->nsite * (ddf_frac + df_frac * df_frac);
+            if (lnl) *lnl -= in->nsite * std::log(1.0 - a[0]);
+        }
         if (!std::isfinite(*df)) {
             *df = *ddf = 0.0;
             return fail(PLK_ERR_NONFINITE, "numerical underflow for lh-derivative");

@@ -215,7 +215,7 @@ k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
     const int tid = blockIdx.x * kThreads + threadIdx.x;
     const int p = tid / C, c = tid % C;
     const int nobs = I.P - I.nasc;
-    const bool live = p < nobs;
+    const bool live = p < I.P;   // observed, or an unobserved pattern of +ASC
     double u0 = 0.0, u1 = 0.0, u2 = 0.0;
     int a = 0, b = 0;
     if (live) {
@@ -263,11 +263,12 @@ k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
         }
     }
     // every lane takes part in the shuffles
-    int minsc;
+    int minsc, minsd;
     double fl = 1.0, fd = 1.0;
     if (I.safe) {
         minsc = group_min<C>(a + b);
         const int mn = group_min<C>(a), md = group_min<C>(b);
+        minsd = mn + md;
         fl = Prec<T>::safe_adjust(a + b - minsc);
         fd = Prec<T>::safe_adjust(a - mn) * Prec<T>::safe_adjust(b - md);
     } else {
@@ -277,6 +278,7 @@ k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
             minsc = (int)sn[p];
             if (!R.dad_tip) minsc += (int)reinterpret_cast<const S*>(I.scale)[R.dad_slot * I.scale_stride + p];
         }
+        minsd = minsc;
     }
     const double lh = group_sum<C>(u0 * fl);
     double l = 0.0, d1 = 0.0, d2 = 0.0;
@@ -286,7 +288,18 @@ k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
         d2 = group_sum<C>(u2 * fd);
     }
     double acc[3] = {0.0, 0.0, 0.0};
-    if (live && c == 0) {
+    if (live && c == 0 && p >= nobs) {
+        // an unobserved pattern (+ASC, tree/phylokernelnonrev.h:1000-1013,
+        // 1455-1469): lh at asc[4k], and for derivatives the lh, d1, d2 the
+        // derivative sums count at 4k+1..3, each rescaled once when scaled
+        double* o = I.asc + 4 * (size_t)(p - nobs);
+        o[0] = Prec<T>::asc_scale(lh + I.invar[p], minsc);
+        if (R.derv) {
+            o[1] = Prec<T>::asc_scale(l + I.invar[p], minsd);
+            o[2] = Prec<T>::asc_scale(d1, minsd);
+            o[3] = Prec<T>::asc_scale(d2, minsd);
+        }
+    } else if (live && c == 0) {
         const double lp = log(lh + I.invar[p]) + minsc * Prec<T>::log_unit;
         const double fq = I.freq[p];
         if (!R.derv) {
@@ -300,9 +313,8 @@ k_nr_root(InstDev I, NrRoot R, double* __restrict__ block_sums) {
             acc[1] = tmp2 - tmp1 * df_frac;
             acc[2] = lp * fq;
         }
-    } else if (!live && c == 0 && p < I.Ppad && !R.derv) {
-        I.ptn_lh[p] = 0.0;
     }
+    if (p >= nobs && c == 0 && p < I.Ppad && !R.derv) I.ptn_lh[p] = 0.0;
     block_sum<3>(acc, sh);
     if (threadIdx.x == 0) {
         block_sums[(size_t)blockIdx.x * 3 + 0] = acc[0];

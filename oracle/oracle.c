@@ -846,9 +846,10 @@ static double nr_eval(orc_tree* t, double length, int derv, double* df_out, doub
     const uint16_t* sd = dad_tip ? NULL : scale_ptr(t, dad.id);
     const uint8_t* drow = dad_tip ? t->states + (size_t)dad.id * t->Ppad : NULL;
     const size_t nobs = (size_t)(t->P - t->nasc);
-    double all = 0.0, all_df = 0.0, all_ddf = 0.0;
-#pragma omp parallel for num_threads(t->nthreads) reduction(+ : all, all_df, all_ddf)
-    for (size_t p = 0; p < nobs; p++) {
+    double all = 0.0, all_df = 0.0, all_ddf = 0.0, all_pc = 0.0, all_dfc = 0.0, all_ddfc = 0.0;
+#pragma omp parallel for num_threads(t->nthreads) \
+    reduction(+ : all, all_df, all_ddf, all_pc, all_dfc, all_ddfc)
+    for (size_t p = 0; p < (size_t)t->P; p++) {
         double adjn[16], adjd[16], lh_cat[16];
         int minsc = 0;
         if (safe) {
@@ -906,6 +907,18 @@ static double nr_eval(orc_tree* t, double length, int derv, double* df_out, doub
         }
         for (int c = 0; c < C; c++) lh += lh_cat[c] * (derv ? 1.0 : adjn[c]);
         lh += t->invar[p];
+        if (p >= nobs) {
+            /* an unobserved pattern (+ASC): its probability, rescaled once when
+             * its counter is not zero (tree/phylokernelnonrev.h:1000-1013,
+             * 1455-1469) */
+            const double f = minsc != 0 ? SCALING_THRESHOLD : 1.0;
+            all_pc += lh * f;
+            if (derv) {
+                all_dfc += d1 * f;
+                all_ddfc += d2 * f;
+            }
+            continue;
+        }
         if (!derv) {
             const double lp = log(lh) + minsc * LOG_SCALING_THRESHOLD;
             t->pattern_lh[p] = lp;
@@ -921,6 +934,13 @@ static double nr_eval(orc_tree* t, double length, int derv, double* df_out, doub
     free(lut);
     int st = 0;
     if (derv) {
+        if (t->nasc) {
+            /* ASC_Lewis (tree/phylokernelnonrev.h:1040-1047) */
+            const double pv = 1.0 - all_pc;
+            const double df_frac = all_dfc / pv, ddf_frac = all_ddfc / pv;
+            all_df += t->nsite * df_frac;
+            all_ddf += t->nsite * (ddf_frac + df_frac * df_frac);
+        }
         *df_out = all_df;
         *ddf_out = all_ddf;
         if (!isfinite(all_df)) {
@@ -935,6 +955,12 @@ static double nr_eval(orc_tree* t, double length, int derv, double* df_out, doub
             if (!isfinite(t->pattern_lh[p])) t->pattern_lh[p] = LOG_SCALING_THRESHOLD * 4;
             all += t->pattern_lh[p] * t->freq[p];
         }
+    }
+    if (!derv && t->nasc) {
+        /* ASC_Lewis (tree/phylokernelnonrev.h:1505-1515) */
+        const double corr = log(1.0 - all_pc);
+        for (size_t p = 0; p < nobs; p++) t->pattern_lh[p] -= corr;
+        all -= t->nsite * corr;
     }
     if (status) *status = st;
     return all;

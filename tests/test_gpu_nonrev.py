@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import iqtree2_amd as ia
+from oracle.binding import OracleTree
 from refmath import indicators
 from statebasis import adjacency, categories, nonrev_model, parents, plan
 from test_oracle_nonrev import internal_branches, oracle
@@ -152,12 +153,10 @@ def test_fp32_and_batches(ctx):
         assert out[0] == lnl
         lp = plain.likelihood_branch(ops, node, dad, length)
         assert abs(out[1] - lp) <= 1e-12 * abs(lp)
-        # the batched derivative call, the device sweep and +ASC refuse it
+        # the batched derivative call refuses it
         inst.likelihood_buffer(ops, node, dad)
         with pytest.raises(ia.PlkError):
             ia.likelihood_derv_batch(ctx, [(inst, length)])
-        with pytest.raises(ia.PlkError):
-            inst.set_asc(4)
         # RELL over the state-basis pattern lnL
         _, plh = inst.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
         wts = np.random.default_rng(1).integers(0, 3, (8, w.npatterns)).astype(np.float32)
@@ -177,3 +176,47 @@ def test_fp32_and_batches(ctx):
     finally:
         for x in (inst, inst32, plain):
             x.close()
+
+
+@pytest.mark.parametrize("safe", [False, True])
+def test_asc_vs_oracle(ctx, safe):
+    """+ASC with a non-reversible mixture: the constant columns unobserved."""
+    from test_oracle_asc import asc_patterns
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=30, nsite=2500, ncat=4, mean_len=0.4 if safe else 0.15,
+                    seed=77)
+    st, fq, inv, nasc = asc_patterns(w)
+    nm = nonrev_model(4, 2, 2, False, seed=5)
+    C = len(categories(nm))
+    ind = indicators(ia.DNA, 4, w.state_unknown)
+    inst = ia.Instance(ctx, w.ntaxa, 4, C, st.shape[1], w.ntaxa, w.state_unknown, ia.PREC_F64, safe)
+    inst.set_patterns(st, fq, inv)
+    inst.set_model_nonrev(nm, ind)
+    inst.set_asc(nasc)
+    orc = OracleTree(w.ntaxa, 4, C, st.shape[1], w.ntaxa, w.state_unknown, safe, 2)
+    orc.set_patterns(st, fq, inv)
+    orc.set_model_nonrev(nm, ind)
+    orc.set_asc(nasc)
+    pobs = st.shape[1] - nasc
+    try:
+        for u, v in internal_branches(w)[::6]:
+            ops, node, dad, length = plan(w.ntaxa, w.edges(), u, v)
+            lg, pg = inst.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+            lo, po = orc.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+            assert abs(lg - lo) <= 1e-9 * abs(lo), (u, v, lg, lo)
+            np.testing.assert_allclose(pg[:pobs], po, rtol=0, atol=1e-9)
+            inst.likelihood_buffer(ops, node, dad)
+            orc.likelihood_buffer(ops, node, dad)
+            for t in (length, 0.5 * length + 1e-3):
+                dfg, ddfg, lng = inst.likelihood_derv(t)
+                dfo, ddfo = orc.likelihood_derv(t)
+                assert abs(dfg - dfo) <= 1e-7 * max(1.0, abs(dfo))
+                assert abs(ddfg - ddfo) <= 1e-7 * max(1.0, abs(ddfo))
+                assert abs(lng - orc.likelihood_from_buffer(t)) <= 1e-9 * abs(lg)
+        u, v = internal_branches(w)[2]
+        ops, node, dad, length = plan(w.ntaxa, w.edges(), u, v)
+        xg, lg, ng = inst.optimize_branch(ops, node, dad, length)
+        xo, lo, no, so = orc.optimize_branch(ops, node, dad, length)
+        assert so == 0 and ng == no
+        assert abs(xg - xo) <= 1e-7 * max(1.0, xo)
+    finally:
+        inst.close()

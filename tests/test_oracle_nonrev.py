@@ -175,3 +175,36 @@ def test_safe_kernel_matches_normal():
         _, sc = norm.read_partial(node.id, partial=False)
         scaled = scaled or sc.max() > 0
     assert scaled
+
+
+def test_asc_by_brute_force():
+    """+ASC with a non-reversible model: the constant columns unobserved,
+    lnL - nsite log(1 - sum_s L(const_s)), every L by brute force rooted at
+    the node end; derivatives against central differences."""
+    w = ia.Workload(seq_type=ia.DNA, ntaxa=6, nsite=500, ncat=4, mean_len=0.3, seed=41)
+    nm = nonrev_model(4, 2, 2, False, seed=12)
+    states, freq = w.get("states"), w.get("freq")
+    const = np.all(states == states[:1], axis=0) & (states[0] < 4)
+    cst = np.tile(np.arange(4, dtype=np.uint8), (w.ntaxa, 1))
+    st = np.ascontiguousarray(np.concatenate([states[:, ~const], cst], axis=1))
+    fq = np.concatenate([freq[~const], np.zeros(4)])
+    pobs = st.shape[1] - 4
+    C = len(categories(nm))
+    orc = OracleTree(w.ntaxa, 4, C, st.shape[1], w.ntaxa, w.state_unknown, False, 2)
+    orc.set_patterns(st, fq, np.zeros(st.shape[1]))
+    orc.set_model_nonrev(nm, indicators(ia.DNA, 4, w.state_unknown))
+    orc.set_asc(4)
+    ind = indicators(ia.DNA, 4, w.state_unknown)
+    for u, v in internal_branches(w)[::2]:
+        ops, node, dad, length = plan(w.ntaxa, w.edges(), u, v)
+        lnl, plh = orc.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+        L = brute_force_rooted(w.ntaxa, w.edges(), u, st, nm, ind)
+        corr = np.log(1.0 - L[pobs:].sum())
+        ref = np.log(L[:pobs]) @ fq[:pobs] - fq[:pobs].sum() * corr
+        assert abs(lnl - ref) <= TOL * abs(ref), (u, v, lnl, ref)
+        np.testing.assert_allclose(plh, np.log(L[:pobs]) - corr, rtol=0, atol=1e-10)
+        orc.likelihood_buffer(ops, node, dad)
+        df, ddf = orc.likelihood_derv(length)
+        h = 1e-5 * max(length, 1e-2)
+        fp, fm = (orc.likelihood_from_buffer(x) for x in (length + h, length - h))
+        assert abs((fp - fm) / (2 * h) - df) <= 1e-5 * max(1.0, abs(df))

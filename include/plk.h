@@ -126,8 +126,9 @@ plk_status plk_set_model(plk_context* ctx, int instance, const double* eval,
  * 1300-1370), rate_matrix [nmix][n][n] and state_freq [nmix][n];
  * mix_weights [nmix] (getMixtureWeight); the nrates site-rate categories
  * (getRate, getProp); tip [SU+1][n] the plain tip vectors (tip_partial_lh,
- * tree/phylotreesse.cpp:387-420). The instance's ncat is ncat_mix: nrates
- * with fused_mix_rate (nmix == nrates), else nrates * nmix. rooted = the
+ * tree/phylotreesse.cpp:387-420). ncat_mix is nrates with fused_mix_rate
+ * (nmix == nrates), else nrates * nmix; the instance's ncat (1, 2, 4 or 8)
+ * must be at least ncat_mix, the extra categories being weight-0 copies. rooted = the
  * tree's `rooted`: 0 puts the frequencies at the node end of the evaluated
  * branch, 1 takes them from the root leaf. Op children and branch ends then
  * carry flags in is_tip: 1 a tip, 2 (children) direction TOWARD_ROOT (the

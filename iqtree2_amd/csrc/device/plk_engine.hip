@@ -1235,7 +1235,8 @@ plk_status plk_set_model_nonrev(plk_context* ctx, int id, int nmix, int fused, c
         return fail(PLK_ERR_ARG, "null argument");
     const int n = in->d.nstates, C = in->d.ncat, SU = in->d.state_unknown;
     if (n != 4 && n != kProtN) return fail(PLK_ERR_UNSUPPORTED, "state-basis models for %d states", n);
-    if (nmix < 1 || nrates < 1 || C != (fused ? nrates : nrates * nmix) || (fused && nmix != nrates))
+    const int Cm = fused ? nrates : nrates * nmix;   // ncat_mix
+    if (nmix < 1 || nrates < 1 || Cm > C || (fused && nmix != nrates))
         return fail(PLK_ERR_ARG, "%d categories for %d mixture components and %d rates%s", C, nmix, nrates,
                     fused ? " (fused)" : "");
     const size_t nn = (size_t)n * n;
@@ -1251,9 +1252,13 @@ plk_status plk_set_model_nonrev(plk_context* ctx, int id, int nmix, int fused, c
     std::vector<double> rate(C), wt(C);
     std::vector<int> mix(C);
     for (int c = 0; c < C; c++) {
-        mix[c] = c / denom;
-        rate[c] = site_rates[c % nrates];
-        wt[c] = site_props[c % nrates] * mix_weights[mix[c]];
+        // categories beyond ncat_mix (the instance's count is 1, 2, 4 or 8):
+        // copies of category 0 with weight 0, which leave every likelihood,
+        // derivative and rescale decision as they are
+        const int cc = c < Cm ? c : 0;
+        mix[c] = cc / denom;
+        rate[c] = site_rates[cc % nrates];
+        wt[c] = c < Cm ? site_props[cc % nrates] * mix_weights[mix[c]] : 0.0;
     }
     h.insert(h.end(), rate.begin(), rate.end());
     h.insert(h.end(), wt.begin(), wt.end());

@@ -220,3 +220,36 @@ def test_asc_vs_oracle(ctx, safe):
         assert abs(xg - xo) <= 1e-7 * max(1.0, xo)
     finally:
         inst.close()
+
+
+@pytest.mark.parametrize("seq_type,nmix,nrates,safe", [(ia.DNA, 3, 2, False), (ia.DNA, 5, 1, True),
+                                                      (ia.PROTEIN, 3, 1, False)])
+def test_category_count_padded(ctx, seq_type, nmix, nrates, safe):
+    """A mixture whose ncat_mix (6, 5, 3) is not a built category count runs
+    in an instance of the next one up, the extra categories weight-0 copies:
+    the oracle at the model's own count gives the same lnL and derivatives."""
+    w = ia.Workload(seq_type=seq_type, ntaxa=24, nsite=1500, ncat=4, mean_len=0.3 if safe else 0.15,
+                    seed=55 + nmix)
+    nm = nonrev_model(w.nstates, nmix, nrates, False, seed=nmix + 17)
+    Cm = len(categories(nm))
+    Cpad = 1 << (Cm - 1).bit_length()
+    ind = indicators(seq_type, w.nstates, w.state_unknown)
+    inst = ia.Instance(ctx, w.ntaxa, w.nstates, Cpad, w.npatterns, w.ntaxa, w.state_unknown, ia.PREC_F64, safe)
+    inst.set_patterns(w.get("states"), w.get("freq"), np.zeros(w.npatterns))
+    inst.set_model_nonrev(nm, ind)
+    orc = oracle(w, nm, safe=safe)
+    try:
+        for u, v in internal_branches(w)[::5]:
+            ops, node, dad, length = plan(w.ntaxa, w.edges(), u, v)
+            lg, pg = inst.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+            lo, po = orc.likelihood_branch(ops, node, dad, length, want_pattern_lh=True)
+            assert abs(lg - lo) <= 1e-9 * abs(lo), (u, v, lg, lo)
+            np.testing.assert_allclose(pg, po, rtol=0, atol=1e-9)
+            inst.likelihood_buffer(ops, node, dad)
+            orc.likelihood_buffer(ops, node, dad)
+            dfg, ddfg, _ = inst.likelihood_derv(length)
+            dfo, ddfo = orc.likelihood_derv(length)
+            assert abs(dfg - dfo) <= 1e-7 * max(1.0, abs(dfo))
+            assert abs(ddfg - ddfo) <= 1e-7 * max(1.0, abs(ddfo))
+    finally:
+        inst.close()

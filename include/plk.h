@@ -342,6 +342,20 @@ typedef struct {
     double flops;
 } plk_kernel_stat;
 
+/* Sankoff parsimony with a cost matrix (PhyloTree::initCostMatrix /
+ * loadCostMatrixFile, computeTipPartialParsimony, computePartialParsimonySankoff
+ * and computeParsimonyBranchSankoff, tree/phylotreepars.cpp:464-885):
+ * cost [n][n] (state i to j, unsigned), 4 or 20 states; the tip costs follow
+ * the reference's rules (a state's row of the matrix, an ambiguity set 0
+ * inside and the least cost into it outside, STATE_UNKNOWN 0). The branch
+ * call runs the ops (every listed node's costs, kept apart from the Fitch
+ * sets and the partial likelihoods), then returns the frequency-weighted
+ * tree score at (node, dad) and, in branch_subst (nullable), the weighted
+ * cost on that branch at the first minimum. Binary nodes. */
+plk_status plk_set_cost_matrix(plk_context* ctx, int instance, const uint32_t* cost);
+plk_status plk_parsimony_sankoff(plk_context* ctx, int instance, const plk_op* ops, int nops,
+                                 plk_end node, plk_end dad, double* score, double* branch_subst);
+
 /* Fitch parsimony (PhyloTree::computePartialParsimonyFast and
  * computeParsimonyBranchFast, tree/phylotreepars.cpp:32-300; the SIMD kernel
  * tree/phylokernel.h:1465-2094): ops compute the Fitch state sets and subtree

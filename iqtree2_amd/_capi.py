@@ -154,6 +154,8 @@ _PROTOS = {
     "plkh_rate_matrix": (c_int, [c_int, _DP, _DP, _DP]),
     "plkh_decompose": (c_int, [c_int, _DP, _DP, _DP, _DP, _DP]),
     "plkh_gamma_rates": (c_int, [c_int, c_double, c_double, _DP, _DP]),
+    "plk_set_cost_matrix": (c_int, [c_void_p, c_int, P(ctypes.c_uint32)]),
+    "plk_parsimony_sankoff": (c_int, [c_void_p, c_int, c_void_p, c_int, PlkEnd, PlkEnd, _DP, _DP]),
     "plk_set_model_nonrev": (c_int, [c_void_p, c_int, c_int, c_int] + [_DP] * 8 + [c_int, _DP, c_int]),
     "plkh_uninformative_patterns": (c_int, [c_int, c_int, ctypes.c_long, _U8, ctypes.c_long,
                                             ctypes.POINTER(ctypes.c_long)]),

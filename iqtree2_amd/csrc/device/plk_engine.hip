@@ -146,6 +146,10 @@ struct Instance {
     size_t nr_model_cap = 0;
     double* nr_root = nullptr;    // root tables T[3][C][n][n], lookups [3][SU+1][C][n]
     size_t nr_root_cap = 0;
+    // Sankoff parsimony: cost matrix [n][n], tip costs [SU+1][n], costs [nslots][Ppad][n]
+    uint32_t* sk_cost = nullptr;
+    uint32_t* sk_part = nullptr;
+    bool sk_ready = false;
     uint8_t* nr_flags = nullptr;  // child flags of the staged ops [2*nops]
     size_t nr_flags_cap = 0;
     plk_end nr_node{0, 0}, nr_dad{0, 0};   // ends of the last buffer call
@@ -196,12 +200,12 @@ size_t footprint_of(const plk_instance_desc* d, int Ppad) {
 enum ProfKind { K_TRAVERSE = 0, K_TRAVERSE_LNL, K_TRAVERSE_THETA, K_DERV, K_FROM_BUFFER,
                 K_PREP_OPS, K_PREP_ROOT, K_REDUCE, K_RELL, K_SWEEP, K_BATCH, K_PREP_BATCH,
                 K_TRAVERSE_MFMA, K_ROOT, K_BATCH_DNA, K_BATCH_PROT, K_BATCH_CODON, K_PARSIMONY,
-                K_NR_PREP, K_NR_ROOT, K_NKIND };
+                K_NR_PREP, K_NR_ROOT, K_SANKOFF, K_NKIND };
 static const char* kProfNames[K_NKIND] = {"traverse", "traverse+lnl", "traverse+theta",
                                           "derv", "lnl_from_buffer", "prep_ops", "prep_root",
                                           "reduce", "rell", "newton_sweep", "batch", "prep_batch",
                                           "traverse_mfma", "root", "batch_dna", "batch_protein",
-                                          "batch_codon", "parsimony", "nonrev_prep", "nonrev_root"};
+                                          "batch_codon", "parsimony", "nonrev_prep", "nonrev_root", "sankoff"};
 
 struct ProfRec {
     int kind;
@@ -360,7 +364,7 @@ plk_status destroy_instance(plk_context* ctx, int id) {
                     in->sw_nderv, in->sw_part, in->sw_flags, in->sw_out, in->sw_timing, in->sw_res, in->sw_tabs,
                     in->pars_tip_planes, in->pars_planes, in->pars_counts, in->pars_sets,
                     in->pars_changes, in->pars_ops, in->asc_vals, in->nr_model, in->nr_root,
-                    in->nr_flags};
+                    in->nr_flags, in->sk_cost, in->sk_part};
     for (void* b : bufs) dfree(ctx, nullptr, b, 0);
     delete in;
     ctx->inst[id] = nullptr;
@@ -2298,6 +2302,89 @@ plk_status plk_optimize_sweep_ex(plk_context* ctx, int id, const plk_sweep_op* o
     if (status == PLK_ERR_NONFINITE) return fail(PLK_ERR_NONFINITE, "non-finite derivative in the Newton sweep");
     if (status == PLK_ERR_DEVICE) return fail(PLK_ERR_DEVICE, "grid reduction in the Newton sweep timed out");
     if (status != 0) return fail((plk_status)status, "Newton sweep failed (status %d)", status);
+    return PLK_OK;
+}
+
+plk_status plk_set_cost_matrix(plk_context* ctx, int id, const uint32_t* cost) {
+    Instance* in = get_instance(ctx, id);
+    if (!in || !cost) return fail(PLK_ERR_ARG, "null argument");
+    const int n = in->d.nstates, SU = in->d.state_unknown;
+    if ((n != 4 && n != kProtN) || SU > kSankoffMaxSU)
+        return fail(PLK_ERR_UNSUPPORTED, "Sankoff parsimony for %d states", n);
+    // the tip costs of computeTipPartialParsimony (tree/phylotreepars.cpp:561-633):
+    // a state's row of the cost matrix, an ambiguity set 0 inside and the least
+    // cost into it outside (DNA bit mask + 3, protein B/Z/J), STATE_UNKNOWN 0
+    static const int ambi_aa[3] = {4 + 8, 32 + 64, 512 + 1024};
+    std::vector<uint32_t> h((size_t)n * n + (size_t)(SU + 1) * n, 0u);
+    std::memcpy(h.data(), cost, sizeof(uint32_t) * n * n);
+    uint32_t* tip = h.data() + (size_t)n * n;
+    std::memcpy(tip, cost, sizeof(uint32_t) * n * n);
+    for (int s = n; s < SU; s++) {
+        const int mask = n == 4 ? s - n + 1 : ambi_aa[s - kProtN];
+        for (int i = 0; i < n; i++) {
+            uint32_t v = 0;
+            if (!(mask & (1 << i))) {
+                v = 0xFFFFFFFFu;
+                for (int j = 0; j < n; j++)
+                    if ((mask & (1 << j)) && cost[i * n + j] < v) v = cost[i * n + j];
+            }
+            tip[(size_t)s * n + i] = v;
+        }
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!in->sk_cost) {
+        plk_status st = dmalloc(ctx, in, &in->sk_cost, h.size() * sizeof(uint32_t));
+        if (st != PLK_OK) return st;
+    }
+    HIP_TRY(hipMemcpyAsync(in->sk_cost, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    in->sk_ready = true;
+    return PLK_OK;
+}
+
+plk_status plk_parsimony_sankoff(plk_context* ctx, int id, const plk_op* ops, int nops, plk_end node,
+                                 plk_end dad, double* score, double* branch_subst) {
+    Instance* in = get_instance(ctx, id);
+    if (!in) return fail(PLK_ERR_ARG, "invalid instance");
+    if (!in->patterns_set) return fail(PLK_ERR_STATE, "patterns not set");
+    if (!in->sk_ready) return fail(PLK_ERR_STATE, "cost matrix not set");
+    if (!score) return fail(PLK_ERR_ARG, "null score");
+    plk_status st = check_ops(in, ops, nops);
+    if (st != PLK_OK) return st;
+    if ((st = check_end(in, node, false)) != PLK_OK) return st;
+    if ((st = check_end(in, dad, true)) != PLK_OK) return st;
+    std::vector<plk_op> plain(ops, ops + nops);   // parsimony has no direction
+    for (plk_op& o : plain)
+        for (int k = 0; k < 2; k++) o.child[k].is_tip &= kNrTip;
+    dad.is_tip &= kNrTip;
+    const int n = in->d.nstates, SU = in->d.state_unknown;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!in->sk_part) {
+        if ((st = dmalloc(ctx, in, &in->sk_part, (size_t)in->d.nslots * in->Ppad * n * sizeof(uint32_t))) != PLK_OK)
+            return st;
+    }
+    if (nops > in->ops_cap) {
+        // the op buffer is shared with the likelihood calls: grow it as stage_ops does
+        if ((st = stage_ops(ctx, in, plain.data(), nops)) != PLK_OK) return st;
+    }
+    if (nops) HIP_TRY(hipMemcpyAsync(in->ops, plain.data(), sizeof(plk_op) * nops, hipMemcpyHostToDevice, ctx->stream));
+    const int grid = (in->d.npatterns + kThreads - 1) / kThreads;
+    const int pe = prof_begin(ctx);
+    if (n == 4)
+        hipLaunchKernelGGL(k_sankoff<4>, dim3(grid), dim3(kThreads), 0, ctx->stream, in->ops, nops, in->sk_cost,
+                           in->sk_cost + 16, SU, in->states, in->d.npatterns, in->Ppad, in->sk_part, node, dad,
+                           in->freq, in->block_sums);
+    else
+        hipLaunchKernelGGL(k_sankoff<kProtN>, dim3(grid), dim3(kThreads), 0, ctx->stream, in->ops, nops, in->sk_cost,
+                           in->sk_cost + kProtN * kProtN, SU, in->states, in->d.npatterns, in->Ppad, in->sk_part,
+                           node, dad, in->freq, in->block_sums);
+    HIP_TRY(hipGetLastError());
+    prof_end(ctx, pe, K_SANKOFF, (double)in->d.npatterns * n * 4 * (2.0 * nops + 2), 0.0);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->block_sums, grid, 3, in->red_out);
+    HIP_TRY(hipGetLastError());
+    if ((st = fetch_scalars(ctx, in, 2)) != PLK_OK) return st;
+    *score = ctx->h_out[0];
+    if (branch_subst) *branch_subst = ctx->h_out[1];
     return PLK_OK;
 }
 

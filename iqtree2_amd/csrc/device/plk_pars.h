@@ -301,3 +301,107 @@ k_pars_sets(ParsDev D, const plk_op* __restrict__ ops, int nops, plk_end node, p
 }
 
 }  // namespace plkdev
+
+namespace plkdev {
+
+constexpr int kSankoffMaxSU = 23;   // tip-cost rows in LDS: DNA (18) and protein (23)
+
+// Sankoff parsimony (tree/phylotreepars.cpp:636-885): one thread per pattern
+// runs the whole op list (a pattern's costs depend on its own children only),
+// costs pattern-major [slot][Ppad][N] in HBM, the cost matrix and tip costs
+// in LDS; then the branch: min over the node's states of the two ends' costs
+// (through the cost matrix for an internal dad) and the cost on the branch at
+// the first minimum, both weighted by the pattern frequency. Block sums
+// [grid][2].
+template <int N>
+__global__ void __launch_bounds__(kThreads)
+k_sankoff(const plk_op* __restrict__ ops, int nops, const uint32_t* __restrict__ cost_g,
+          const uint32_t* __restrict__ tip_g, int SU, const uint8_t* __restrict__ states, int P, int Ppad,
+          uint32_t* __restrict__ part, plk_end node, plk_end dad, const double* __restrict__ freq,
+          double* __restrict__ block_sums) {
+    __shared__ uint32_t cost[N * N];
+    __shared__ uint32_t tip[(kSankoffMaxSU + 1) * N];
+    __shared__ double sh[2 * (kThreads / 64)];
+    for (int t = threadIdx.x; t < N * N; t += kThreads) cost[t] = cost_g[t];
+    for (int t = threadIdx.x; t < (SU + 1) * N; t += kThreads) tip[t] = tip_g[t];
+    __syncthreads();
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    double acc[2] = {0.0, 0.0};
+    if (p < P) {
+        for (int o = 0; o < nops; o++) {
+            const plk_op op = ops[o];
+            int left = (!op.child[0].is_tip && op.child[1].is_tip) ? 1 : 0;
+            const plk_child L = op.child[left], R = op.child[1 - left];
+            uint32_t out[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) out[i] = 0;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const plk_child ch = k == 0 ? L : R;
+                if (ch.is_tip) {
+                    const uint32_t* tp = tip + (int)states[(size_t)ch.id * Ppad + p] * N;
+#pragma unroll
+                    for (int i = 0; i < N; i++) out[i] += tp[i];
+                } else {
+                    const uint32_t* cp = part + ((size_t)ch.id * Ppad + p) * N;
+                    uint32_t cv[N];
+#pragma unroll
+                    for (int j = 0; j < N; j++) cv[j] = cp[j];
+#pragma unroll
+                    for (int i = 0; i < N; i++) {
+                        uint32_t m = cv[0] + cost[i * N];
+#pragma unroll
+                        for (int j = 1; j < N; j++) m = min(m, cv[j] + cost[i * N + j]);
+                        out[i] += m;
+                    }
+                }
+            }
+            uint32_t* dp = part + ((size_t)op.dad_slot * Ppad + p) * N;
+#pragma unroll
+            for (int i = 0; i < N; i++) dp[i] = out[i];
+        }
+        const uint32_t* nb = part + ((size_t)node.id * Ppad + p) * N;
+        uint32_t best, br;
+        if (dad.is_tip) {
+            const uint32_t* tp = tip + (int)states[(size_t)dad.id * Ppad + p] * N;
+            best = tp[0] + nb[0];
+            br = tp[0];
+            for (int i = 1; i < N; i++)
+                if (tp[i] + nb[i] < best) {
+                    best = tp[i] + nb[i];
+                    br = tp[i];
+                }
+        } else {
+            const uint32_t* db = part + ((size_t)dad.id * Ppad + p) * N;
+            uint32_t dv[N];
+#pragma unroll
+            for (int j = 0; j < N; j++) dv[j] = db[j];
+            best = 0xFFFFFFFFu;
+            br = 0xFFFFFFFFu;
+            for (int i = 0; i < N; i++) {
+                uint32_t m = dv[0] + cost[i * N], bs = cost[i * N];
+#pragma unroll
+                for (int j = 1; j < N; j++)
+                    if (dv[j] + cost[i * N + j] < m) {
+                        m = dv[j] + cost[i * N + j];
+                        bs = cost[i * N + j];
+                    }
+                m += nb[i];
+                if (m < best) {
+                    best = m;
+                    br = bs;
+                }
+            }
+        }
+        acc[0] = (double)best * freq[p];
+        acc[1] = (double)br * freq[p];
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0) {
+        block_sums[(size_t)blockIdx.x * 3] = acc[0];
+        block_sums[(size_t)blockIdx.x * 3 + 1] = acc[1];
+        block_sums[(size_t)blockIdx.x * 3 + 2] = 0.0;
+    }
+}
+
+}  // namespace plkdev

@@ -261,6 +261,21 @@ class Instance:
               "plk_parsimony_branch")
         return score.value, sub.value
 
+    def set_cost_matrix(self, cost):
+        """Sankoff cost matrix [n][n] (state i to j, unsigned)."""
+        self._cost = np.ascontiguousarray(cost, np.uint32)
+        check(lib().plk_set_cost_matrix(self.ctx.handle, self.id,
+                                        self._cost.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+              "plk_set_cost_matrix")
+
+    def parsimony_sankoff(self, ops, node, dad):
+        """computeParsimonyBranchSankoff: (tree score, weighted cost on the branch)."""
+        p, n = _ops_ptr(ops)
+        score, sub = ctypes.c_double(), ctypes.c_double()
+        check(lib().plk_parsimony_sankoff(self.ctx.handle, self.id, p, n, node, dad, ctypes.byref(score),
+                                          ctypes.byref(sub)), "plk_parsimony_sankoff")
+        return score.value, sub.value
+
     def set_model_nonrev(self, nm, tip, rooted=False):
         """State-basis (non-reversible / mixture) model: nm holds ceval
         [nmix][n], cevec and cinv_evec [nmix][n][n] (complex), Q [nmix][n][n],

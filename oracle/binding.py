@@ -37,6 +37,9 @@ _PROTOS = {
                                        ctypes.POINTER(c_int)]),
     "orc_read_partial": (None, [c_void_p, c_int, _DP, ctypes.POINTER(ctypes.c_int32)]),
     "orc_set_asc": (None, [c_void_p, c_int]),
+    "orc_set_cost_matrix": (None, [c_void_p, ctypes.POINTER(ctypes.c_uint32), c_int]),
+    "orc_parsimony_sankoff": (c_double, [c_void_p, c_void_p, c_int, OrcEnd, OrcEnd,
+                                         ctypes.POINTER(ctypes.c_uint32), _DP]),
     "orc_set_model_nonrev": (c_int, [c_void_p, c_int, c_int] + [_DP] * 8 + [c_int, _DP, c_int]),
     "orc_parsimony_branch": (c_double, [c_void_p, c_void_p, c_int, OrcEnd, OrcEnd,
                                         ctypes.POINTER(ctypes.c_int32), _DP]),
@@ -133,6 +136,22 @@ class OracleTree:
                                         tip.ctypes.data_as(_DP), int(bool(rooted)))
         if st != 0:
             raise ValueError("state-basis model does not match the instance's categories")
+
+    def set_cost_matrix(self, cost, seq_type):
+        """Sankoff cost matrix [n][n] (uint32); seq_type picks the tip ambiguity rules."""
+        self._cost = np.ascontiguousarray(cost, np.uint32)
+        lib().orc_set_cost_matrix(self._h, self._cost.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                  int(seq_type))
+
+    def parsimony_sankoff(self, ops, node, dad, want_pattern=False):
+        """(tree score, weighted cost on the branch[, per-pattern scores])."""
+        p, n = _ops(ops)
+        per = np.zeros(self.P, np.uint32) if want_pattern else None
+        sub = c_double()
+        s = lib().orc_parsimony_sankoff(self._h, p, n, _end(node), _end(dad),
+                                        per.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if want_pattern
+                                        else None, ctypes.byref(sub))
+        return (s, sub.value, per) if want_pattern else (s, sub.value)
 
     def set_threads(self, n):
         lib().orc_set_threads(self._h, n)

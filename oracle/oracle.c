@@ -59,6 +59,9 @@ struct orc_tree {
     uint8_t* nr_flags;     /* [2*nops] child flags of the staged ops */
     int nr_flags_cap;
     orc_end nr_node, nr_dad;
+    /* Sankoff parsimony: cost matrix [n][n], tip costs [SU+1][n], partial
+     * costs [nslots][Ppad][n] */
+    uint32_t *sk_cost, *sk_tip, *sk_partial;
 };
 
 static void* amalloc(size_t bytes) {
@@ -350,6 +353,7 @@ void orc_destroy(orc_tree* t) {
     free(t->pf_score);
     free(t->nr_ceval); free(t->nr_cevec); free(t->nr_cinv); free(t->nr_Q); free(t->nr_pi);
     free(t->nr_mix); free(t->nr_flags);
+    free(t->sk_cost); free(t->sk_tip); free(t->sk_partial);
     free(t);
 }
 
@@ -1594,4 +1598,104 @@ double orc_parsimony_fast(orc_tree* t, const orc_op* ops, int nops, orc_end node
     }
     if (branch_subst) *branch_subst = (double)on;
     return (double)(sx + sy + on);
+}
+
+/* ---- Sankoff parsimony (tree/phylotreepars.cpp:464-892) ------------------ */
+
+/* initCostMatrix / loadCostMatrixFile, then computeTipPartialParsimony's
+ * tip costs (tree/phylotreepars.cpp:561-633): a state's row of the cost
+ * matrix, an ambiguity set 0 inside and the least cost into it outside
+ * (DNA bit mask + 3, protein B/Z/J), STATE_UNKNOWN 0 */
+void orc_set_cost_matrix(orc_tree* t, const uint32_t* cost, int seq_type) {
+    const int n = t->n;
+    free(t->sk_cost);
+    free(t->sk_tip);
+    t->sk_cost = (uint32_t*)amalloc((size_t)n * n * 4);
+    t->sk_tip = (uint32_t*)amalloc((size_t)(t->SU + 1) * n * 4);
+    memcpy(t->sk_cost, cost, (size_t)n * n * 4);
+    memcpy(t->sk_tip, cost, (size_t)n * n * 4);
+    static const int ambi_aa[3] = {4 + 8, 32 + 64, 512 + 1024};
+    for (int s = n; s <= t->SU; s++) {
+        int mask = 0;
+        if (s < t->SU) mask = seq_type == 0 ? s - n + 1 : (seq_type == 1 ? ambi_aa[s - 20] : 0);
+        for (int i = 0; i < n; i++) {
+            uint32_t v = 0;
+            if (mask && !(mask & (1 << i))) {
+                v = UINT32_MAX;
+                for (int j = 0; j < n; j++)
+                    if ((mask & (1 << j)) && cost[i * n + j] < v) v = cost[i * n + j];
+            }
+            t->sk_tip[(size_t)s * n + i] = v;
+        }
+    }
+    if (!t->sk_partial) t->sk_partial = (uint32_t*)amalloc((size_t)t->nslots * t->Ppad * n * 4);
+}
+
+/* min_j child[j] + cost[i][j] */
+static inline uint32_t sk_min(const uint32_t* cost_row, const uint32_t* child, int n) {
+    uint32_t m = child[0] + cost_row[0];
+    for (int j = 1; j < n; j++) {
+        const uint32_t v = child[j] + cost_row[j];
+        if (v < m) m = v;
+    }
+    return m;
+}
+
+/* computePartialParsimonySankoff for binary nodes (636-789) over the ops,
+ * then computeParsimonyBranchSankoff (791-885) at (node, dad) */
+double orc_parsimony_sankoff(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                             uint32_t* ptn_score, double* branch_subst) {
+    const int n = t->n;
+    const uint32_t* C = t->sk_cost;
+    const size_t P = (size_t)t->P, Ppad = (size_t)t->Ppad;
+    for (int o = 0; o < nops; o++) {
+        const orc_op* op = &ops[o];
+        int left = 0, right = 1;
+        if (!op->child[0].is_tip && op->child[1].is_tip) { left = 1; right = 0; }
+        const orc_child* L = &op->child[left];
+        const orc_child* R = &op->child[right];
+        uint32_t* dad_p = t->sk_partial + (size_t)op->dad_slot * Ppad * n;
+        for (size_t p = 0; p < P; p++) {
+            uint32_t* out = dad_p + p * n;
+            const uint32_t* lt = L->is_tip ? t->sk_tip + (size_t)t->states[(size_t)L->id * Ppad + p] * n
+                                           : t->sk_partial + ((size_t)L->id * Ppad + p) * n;
+            const uint32_t* rt = R->is_tip ? t->sk_tip + (size_t)t->states[(size_t)R->id * Ppad + p] * n
+                                           : t->sk_partial + ((size_t)R->id * Ppad + p) * n;
+            for (int i = 0; i < n; i++) {
+                const uint32_t a = L->is_tip ? lt[i] : sk_min(C + (size_t)i * n, lt, n);
+                const uint32_t b = R->is_tip ? rt[i] : sk_min(C + (size_t)i * n, rt, n);
+                out[i] = a + b;
+            }
+        }
+    }
+    double tree = 0.0, branch = 0.0;
+    const uint32_t* pn = t->sk_partial + (size_t)node.id * Ppad * n;
+    for (size_t p = 0; p < P; p++) {
+        const uint32_t* nb = pn + p * n;
+        uint32_t best, br;
+        if (dad.is_tip) {
+            const uint32_t* tp = t->sk_tip + (size_t)t->states[(size_t)dad.id * Ppad + p] * n;
+            best = tp[0] + nb[0];
+            br = tp[0];
+            for (int i = 1; i < n; i++)
+                if (tp[i] + nb[i] < best) { best = tp[i] + nb[i]; br = tp[i]; }
+        } else {
+            const uint32_t* db = t->sk_partial + ((size_t)dad.id * Ppad + p) * n;
+            best = UINT32_MAX;
+            br = UINT32_MAX;
+            for (int i = 0; i < n; i++) {
+                const uint32_t* row = C + (size_t)i * n;
+                uint32_t m = db[0] + row[0], bs = row[0];
+                for (int j = 1; j < n; j++)
+                    if (db[j] + row[j] < m) { m = db[j] + row[j]; bs = row[j]; }
+                m += nb[i];
+                if (m < best) { best = m; br = bs; }
+            }
+        }
+        if (ptn_score) ptn_score[p] = best;
+        tree += (double)best * t->freq[p];
+        branch += (double)br * t->freq[p];
+    }
+    if (branch_subst) *branch_subst = branch;
+    return tree;
 }

@@ -95,6 +95,16 @@ double orc_parsimony_branch(orc_tree* t, const orc_op* ops, int nops, orc_end no
 double orc_parsimony_fast(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
                           double* branch_subst);
 
+/* Sankoff parsimony (tree/phylotreepars.cpp:464-892): the cost matrix
+ * [n][n] (cost[i][j], state i to j; seq_type 0 DNA, 1 protein, 2 other for
+ * the ambiguity rules of the tip costs), then the frequency-weighted tree
+ * score at (node, dad) after the ops; ptn_score (nullable) [P] each
+ * pattern's score, branch_subst (nullable) the weighted cost on the branch
+ * at the optimum */
+void orc_set_cost_matrix(orc_tree* t, const uint32_t* cost, int seq_type);
+double orc_parsimony_sankoff(orc_tree* t, const orc_op* ops, int nops, orc_end node, orc_end dad,
+                             uint32_t* ptn_score, double* branch_subst);
+
 /* minimizeNewton on a caller-supplied function (utils/optimization.cpp:422-499);
  * fderv(x, &f, &df) returns f and its derivative. status 0 ok, 7 no convergence */
 typedef void (*orc_fderv)(void* user, double x, double* f, double* df);

@@ -398,6 +398,62 @@ def leg_parsimony(ia, ctx, w, reps=20):
         tree.close()
 
 
+def sankoff_cost():
+    """A symmetric DNA cost matrix: transitions 1, transversions 2."""
+    import numpy as np
+    c = np.full((4, 4), 2, np.uint32)
+    np.fill_diagonal(c, 0)
+    c[0, 2] = c[2, 0] = c[1, 3] = c[3, 1] = 1
+    return c
+
+
+def leg_sankoff(ia, ctx, w, reps=20):
+    """Sankoff parsimony with a cost matrix: the whole tree from scratch."""
+    inst = ia.Instance.from_workload(ctx, w)
+    tree = ia.PhyloTree(None, w)
+    try:
+        inst.set_cost_matrix(sankoff_cost())
+        ops, node, dad = tree.plan_full(*tree.branches()[0])
+        score = inst.parsimony_sankoff(ops, node, dad)
+        ctx.synchronize()
+        ctx.profile_start()
+        t = time.perf_counter()
+        for _ in range(reps):
+            assert inst.parsimony_sankoff(ops, node, dad) == score
+        dt = (time.perf_counter() - t) / reps
+        prof = ctx.profile_stop()
+        k = prof["sankoff"]
+        avg_s = k["total_ms"] * 1e-3 / k["launches"]
+        bytes_per = k["bytes"] / k["launches"]
+        return {"score": score[0], "branch_cost": score[1], "ops": len(ops), "full_tree_ms": 1e3 * dt,
+                "kernel_us": 1e6 * avg_s, "pattern_node_evals_per_s": w.npatterns * len(ops) / avg_s,
+                "roofline": {"bound": "hbm", "achieved": bytes_per / avg_s / 1e9, "peak": HBM_PEAK / 1e9,
+                             "unit": "GB/s", "frac": bytes_per / avg_s / HBM_PEAK, "bytes_per_launch": bytes_per}}
+    finally:
+        inst.close()
+        tree.close()
+
+
+def cpu_sankoff(ia, w, budget_s):
+    """The oracle's Sankoff restatement (one thread, as the reference's loop)."""
+    from oracle.binding import OracleTree
+    planner = ia.PhyloTree(None, w)
+    orc = OracleTree.from_workload(w)
+    orc.set_cost_matrix(sankoff_cost(), ia.DNA)
+    try:
+        ops, node, dad = planner.plan_full(*planner.branches()[0])
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s or n == 0:
+            score = orc.parsimony_sankoff(ops, node, dad)
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+    finally:
+        planner.close()
+    return {"full_tree_ms": 1e3 * dt, "score": score[0], "cores": 1, "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"{n} whole-tree Sankoff evaluations of the 200 x 100k workload"}
+
+
 def cpu_parsimony(ia, w, budget_s, threads):
     """The reference's bit-parallel Fitch (sites expanded by frequency,
     32-site planes, popcount, OpenMP over 256-site blocks) restated in the
@@ -788,8 +844,10 @@ def main():
             elif args.only == "parsimony":
                 w = wl.dna_200x100k()
                 out = leg_parsimony(ia, ctx, w)
+                out["sankoff"] = leg_sankoff(ia, ctx, w)
                 if not args.no_cpu:
                     out["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
+                    out["sankoff"]["cpu_baseline"] = cpu_sankoff(ia, w, 3.0)
                 w.close()
             elif args.only == "nonrev":
                 w = wl.dna_200x100k()
@@ -829,8 +887,10 @@ def main():
                 dna["cpu_baseline"] = cpu_dna(ia, w, args.cpu_seconds / 2, host_threads())
             dna["nni"] = leg_nni(ia, ctx, w)
             dna["parsimony"] = leg_parsimony(ia, ctx, w)
+            dna["parsimony"]["sankoff"] = leg_sankoff(ia, ctx, w)
             if not args.no_cpu:
                 dna["parsimony"]["cpu_baseline"] = cpu_parsimony(ia, w, 5.0, host_threads())
+                dna["parsimony"]["sankoff"]["cpu_baseline"] = cpu_sankoff(ia, w, 3.0)
             dna["modelfinder_gamma"] = leg_modelfinder(ia, ctx, w, parts)
             if not args.no_cpu:
                 dna["modelfinder_gamma"]["cpu_baseline"] = cpu_modelfinder(ia, w, host_threads())

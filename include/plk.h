@@ -345,7 +345,7 @@ typedef struct {
 /* Sankoff parsimony with a cost matrix (PhyloTree::initCostMatrix /
  * loadCostMatrixFile, computeTipPartialParsimony, computePartialParsimonySankoff
  * and computeParsimonyBranchSankoff, tree/phylotreepars.cpp:464-885):
- * cost [n][n] (state i to j, unsigned), 4 or 20 states; the tip costs follow
+ * cost [n][n] (state i to j, unsigned), 4, 20 or 61 states; the tip costs follow
  * the reference's rules (a state's row of the matrix, an ambiguity set 0
  * inside and the least cost into it outside, STATE_UNKNOWN 0). The branch
  * call runs the ops (every listed node's costs, kept apart from the Fitch

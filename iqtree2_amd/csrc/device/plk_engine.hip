@@ -2309,7 +2309,7 @@ plk_status plk_set_cost_matrix(plk_context* ctx, int id, const uint32_t* cost) {
     Instance* in = get_instance(ctx, id);
     if (!in || !cost) return fail(PLK_ERR_ARG, "null argument");
     const int n = in->d.nstates, SU = in->d.state_unknown;
-    if ((n != 4 && n != kProtN) || SU > kSankoffMaxSU)
+    if ((n != 4 && n != kProtN && n != kCodonN) || SU > kSankoffMaxSU)
         return fail(PLK_ERR_UNSUPPORTED, "Sankoff parsimony for %d states", n);
     // the tip costs of computeTipPartialParsimony (tree/phylotreepars.cpp:561-633):
     // a state's row of the cost matrix, an ambiguity set 0 inside and the least
@@ -2319,7 +2319,7 @@ plk_status plk_set_cost_matrix(plk_context* ctx, int id, const uint32_t* cost) {
     std::memcpy(h.data(), cost, sizeof(uint32_t) * n * n);
     uint32_t* tip = h.data() + (size_t)n * n;
     std::memcpy(tip, cost, sizeof(uint32_t) * n * n);
-    for (int s = n; s < SU; s++) {
+    for (int s = n; s < SU; s++) {   // ambiguity codes (none for codons)
         const int mask = n == 4 ? s - n + 1 : ambi_aa[s - kProtN];
         for (int i = 0; i < n; i++) {
             uint32_t v = 0;
@@ -2374,10 +2374,14 @@ plk_status plk_parsimony_sankoff(plk_context* ctx, int id, const plk_op* ops, in
         hipLaunchKernelGGL(k_sankoff<4>, dim3(grid), dim3(kThreads), 0, ctx->stream, in->ops, nops, in->sk_cost,
                            in->sk_cost + 16, SU, in->states, in->d.npatterns, in->Ppad, in->sk_part, node, dad,
                            in->freq, in->block_sums);
-    else
+    else if (n == kProtN)
         hipLaunchKernelGGL(k_sankoff<kProtN>, dim3(grid), dim3(kThreads), 0, ctx->stream, in->ops, nops, in->sk_cost,
                            in->sk_cost + kProtN * kProtN, SU, in->states, in->d.npatterns, in->Ppad, in->sk_part,
                            node, dad, in->freq, in->block_sums);
+    else
+        hipLaunchKernelGGL(k_sankoff<kCodonN>, dim3(grid), dim3(kThreads), 0, ctx->stream, in->ops, nops,
+                           in->sk_cost, in->sk_cost + kCodonN * kCodonN, SU, in->states, in->d.npatterns, in->Ppad,
+                           in->sk_part, node, dad, in->freq, in->block_sums);
     HIP_TRY(hipGetLastError());
     prof_end(ctx, pe, K_SANKOFF, (double)in->d.npatterns * n * 4 * (2.0 * nops + 2), 0.0);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, ctx->stream, in->block_sums, grid, 3, in->red_out);

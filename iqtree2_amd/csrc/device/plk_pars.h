@@ -304,7 +304,7 @@ k_pars_sets(ParsDev D, const plk_op* __restrict__ ops, int nops, plk_end node, p
 
 namespace plkdev {
 
-constexpr int kSankoffMaxSU = 23;   // tip-cost rows in LDS: DNA (18) and protein (23)
+constexpr int kSankoffMaxSU = 61;   // tip-cost rows in LDS: DNA 18, protein 23, codon 61
 
 // Sankoff parsimony (tree/phylotreepars.cpp:636-885): one thread per pattern
 // runs the whole op list (a pattern's costs depend on its own children only),
@@ -321,6 +321,7 @@ k_sankoff(const plk_op* __restrict__ ops, int nops, const uint32_t* __restrict__
           double* __restrict__ block_sums) {
     __shared__ uint32_t cost[N * N];
     __shared__ uint32_t tip[(kSankoffMaxSU + 1) * N];
+    constexpr int kUnrollI = N <= 20 ? N : 1, kUnrollJ = N <= 20 ? N : 4;   // codons: no full unroll
     __shared__ double sh[2 * (kThreads / 64)];
     for (int t = threadIdx.x; t < N * N; t += kThreads) cost[t] = cost_g[t];
     for (int t = threadIdx.x; t < (SU + 1) * N; t += kThreads) tip[t] = tip_g[t];
@@ -347,10 +348,10 @@ k_sankoff(const plk_op* __restrict__ ops, int nops, const uint32_t* __restrict__
                     uint32_t cv[N];
 #pragma unroll
                     for (int j = 0; j < N; j++) cv[j] = cp[j];
-#pragma unroll
+#pragma unroll kUnrollI
                     for (int i = 0; i < N; i++) {
                         uint32_t m = cv[0] + cost[i * N];
-#pragma unroll
+#pragma unroll kUnrollJ
                         for (int j = 1; j < N; j++) m = min(m, cv[j] + cost[i * N + j]);
                         out[i] += m;
                     }

@@ -16,12 +16,13 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("seq_type,ntaxa,nsite,symmetric", [(ia.DNA, 40, 3000, True), (ia.DNA, 24, 2000, False),
                                                              (ia.PROTEIN, 20, 1500, True),
-                                                             (ia.PROTEIN, 12, 800, False)])
+                                                             (ia.PROTEIN, 12, 800, False),
+                                                             (ia.CODON, 10, 300, True)])
 def test_sankoff_vs_oracle(ctx, seq_type, ntaxa, nsite, symmetric):
     w = ia.Workload(seq_type=seq_type, ntaxa=ntaxa, nsite=nsite, ncat=4, mean_len=0.3, gap_frac=0.05,
                     seed=200 + ntaxa)
     cost = random_cost(w.nstates, np.random.default_rng(ntaxa), symmetric)
-    inst = ia.Instance.from_workload(ctx, w)
+    inst = ia.Instance.from_workload(ctx, w, safe_numeric=seq_type == ia.CODON)   # 61 states are always safe
     inst.set_cost_matrix(cost)
     orc = OracleTree.from_workload(w)
     orc.set_cost_matrix(cost, seq_type)

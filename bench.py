@@ -549,7 +549,8 @@ def nonrev_inputs(w, seed=7):
     vectors (ambiguity code = bitmask + 3, STATE_UNKNOWN = every state)."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    Q = rng.uniform(0.05, 2.0, (4, 4))
+    n = w.nstates
+    Q = rng.uniform(0.05, 2.0, (n, n))
     np.fill_diagonal(Q, 0.0)
     np.fill_diagonal(Q, -Q.sum(axis=1))
     ev, vl = np.linalg.eig(Q.T)
@@ -561,14 +562,16 @@ def nonrev_inputs(w, seed=7):
     nm = dict(Q=Q[None], pi=pi[None], ceval=lam[None], cevec=V[None], cinv_evec=np.linalg.inv(V)[None],
               mixw=np.ones(1), rates=m["rates"], props=m["props"], fused=False)
     SU = w.state_unknown
-    tip = np.zeros((SU + 1, 4))
+    tip = np.zeros((SU + 1, n))
     for s in range(SU + 1):
-        if s < 4:
+        if s < n:
             tip[s, s] = 1.0
         elif s == SU:
             tip[s] = 1.0
-        else:
+        elif n == 4:
             tip[s] = [(s - 3) >> x & 1 for x in range(4)]
+        else:
+            tip[s, list({20: (2, 3), 21: (5, 6), 22: (9, 10)}[s])] = 1.0   # B, Z, J
     return nm, tip
 
 
@@ -578,7 +581,7 @@ def leg_nonrev(ia, ctx, w, reps=10):
     same tree and patterns."""
     import numpy as np
     nm, tip = nonrev_inputs(w)
-    inst = ia.Instance(ctx, w.ntaxa, 4, w.ncat, w.npatterns, w.ntaxa, w.state_unknown)
+    inst = ia.Instance(ctx, w.ntaxa, w.nstates, w.ncat, w.npatterns, w.ntaxa, w.state_unknown)
     inst.set_patterns(w.get("states"), w.get("freq"), np.zeros(w.npatterns))
     inst.set_model_nonrev(nm, tip)
     rev = ia.Instance.from_workload(ctx, w)
@@ -613,8 +616,8 @@ def leg_nonrev(ia, ctx, w, reps=10):
                 "derivative_call_us": 1e6 * dtd,
                 "kernels_us": {k: 1e3 * v["total_ms"] / v["launches"] for k, v in prof.items()
                                if v["launches"]},
-                "roofline": roofline_of(prof, ("traverse", "nonrev_root"), "{kernel}_nonrev_dna_200x100k"),
-                "roofline_derivative": roofline_of(profd, ("nonrev_root",), "{kernel}_nonrev_derv_dna_200x100k")}
+                "roofline": roofline_of(prof, ("traverse", "traverse_mfma", "nonrev_root"), "{kernel}_nonrev"),
+                "roofline_derivative": roofline_of(profd, ("nonrev_root",), "{kernel}_nonrev_derv")}
     finally:
         inst.close()
         rev.close()
@@ -627,7 +630,7 @@ def cpu_nonrev(ia, w, budget_s, threads):
     from oracle.binding import OracleTree
     nm, tip = nonrev_inputs(w)
     planner = ia.PhyloTree(None, w)
-    orc = OracleTree(w.ntaxa, 4, w.ncat, w.npatterns, w.ntaxa, w.state_unknown, False, threads)
+    orc = OracleTree(w.ntaxa, w.nstates, w.ncat, w.npatterns, w.ntaxa, w.state_unknown, False, threads)
     orc.set_patterns(w.get("states"), w.get("freq"), np.zeros(w.npatterns))
     orc.set_model_nonrev(nm, tip)
     try:
@@ -644,7 +647,7 @@ def cpu_nonrev(ia, w, budget_s, threads):
         planner.close()
     return {"traversal_ple_per_s": w.npatterns / dt, "traversal_ms": 1e3 * dt, "lnl": lnl,
             "cores": threads, "cpu_model": cpu_model(), "kind": "port",
-            "sample": f"{n} full traversals + branch lnL of the 200 x 100k workload"}
+            "sample": f"{n} full traversals + branch lnL of the {w.ntaxa} x {w.npatterns}-pattern workload"}
 
 
 # ---------------------------------------------------------------------------
@@ -856,6 +859,11 @@ def main():
                     out["cpu_baseline"] = cpu_nonrev(ia, w, 5.0, host_threads())
                     out["lnl_vs_oracle"] = out["lnl"] - out["cpu_baseline"]["lnl"]
                 w.close()
+                wp = wl.protein_100x20k()
+                out["protein_100x20k"] = leg_nonrev(ia, ctx, wp)
+                if not args.no_cpu:
+                    out["protein_100x20k"]["cpu_baseline"] = cpu_nonrev(ia, wp, 5.0, host_threads())
+                wp.close()
             elif args.only == "modelfinder":
                 w = wl.dna_200x100k()
                 out = leg_modelfinder(ia, ctx, w, wl.partitions_500(args.partitions))

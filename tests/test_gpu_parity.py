@@ -610,3 +610,47 @@ def test_batch_theta_mode_bit_identical(ctx):
     for (a, *_), (b, *_) in zip(jobs, refs):
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("seq_type,ncat,safe", [(ia.DNA, 5, False), (ia.DNA, 3, True), (ia.PROTEIN, 6, False)])
+def test_category_count_padded_reversible(ctx, seq_type, ncat, safe):
+    """A rate model with a category count that is not built (FreeRate +R3,
+    +R5, +R6) runs in an instance of the next built count, the extra
+    categories copies of category 0 with weight 0: lnL, derivatives and Newton
+    equal the oracle's at the model's own count."""
+    w = ia.Workload(seq_type=seq_type, ntaxa=24, nsite=1500, ncat=4, mean_len=0.3 if safe else 0.15,
+                    seed=300 + ncat)
+    m = w.model_arrays()
+    rng = np.random.default_rng(ncat)
+    rates = np.sort(rng.uniform(0.1, 3.0, ncat))
+    props = rng.dirichlet(np.full(ncat, 3.0))
+    rates /= rates @ props
+    cpad = 1 << (ncat - 1).bit_length()
+    r_pad = np.concatenate([rates, np.full(cpad - ncat, rates[0])])
+    p_pad = np.concatenate([props, np.zeros(cpad - ncat)])
+    inst = ia.Instance(ctx, w.ntaxa, w.nstates, cpad, w.npatterns, w.ntaxa, w.state_unknown, ia.PREC_F64, safe)
+    inst.set_patterns(w.get("states"), w.get("freq"), w.get("invar"))
+    inst.set_model(m["eval"], m["evec"], m["inv_evec"], m["tip"], r_pad, p_pad)
+    orc = OracleTree(w.ntaxa, w.nstates, ncat, w.npatterns, w.ntaxa, w.state_unknown, safe, 2)
+    orc.set_patterns(w.get("states"), w.get("freq"), w.get("invar"))
+    orc.set_model(m["eval"], m["evec"], m["inv_evec"], m["tip"], rates, props)
+    planner = ia.PhyloTree(None, w)
+    try:
+        for a, b in planner.branches()[::6]:
+            ops, node, dad, length = planner.plan(a, b)
+            lg = inst.likelihood_branch(ops, node, dad, length)
+            lo = orc.likelihood_branch(ops, node, dad, length)
+            assert abs(lg - lo) <= 1e-9 * abs(lo), (a, b, lg, lo)
+            inst.likelihood_buffer(ops, node, dad)
+            orc.likelihood_buffer(ops, node, dad)
+            dfg, ddfg, _ = inst.likelihood_derv(length)
+            dfo, ddfo = orc.likelihood_derv(length)
+            assert abs(dfg - dfo) <= 1e-7 * max(1.0, abs(dfo))
+            assert abs(ddfg - ddfo) <= 1e-7 * max(1.0, abs(ddfo))
+        ops, node, dad, length = planner.plan(*planner.branches()[1][::-1])
+        xg, lg, ng = inst.optimize_branch(ops, node, dad, length)
+        xo, lo, no, so = orc.optimize_branch(ops, node, dad, length)
+        assert so == 0 and ng == no and abs(xg - xo) <= 1e-7 * max(1.0, xo)
+    finally:
+        inst.close()
+        planner.close()

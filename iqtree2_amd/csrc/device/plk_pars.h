@@ -328,6 +328,15 @@ k_sankoff(const plk_op* __restrict__ ops, int nops, const uint32_t* __restrict__
     __syncthreads();
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double acc[2] = {0.0, 0.0};
+    // the last KC dad slots' costs stay in registers: in post-order a child
+    // is usually one of the latest dads, whose store would otherwise be read
+    // back through the memory hierarchy before the op can start
+    constexpr int KC = N <= 4 ? 4 : (N <= 20 ? 2 : 1);
+    uint32_t cache[KC][N];
+    int cslot[KC];
+#pragma unroll
+    for (int k = 0; k < KC; k++) cslot[k] = -1;
+    int cnext = 0;
     if (p < P) {
         for (int o = 0; o < nops; o++) {
             const plk_op op = ops[o];
@@ -344,10 +353,20 @@ k_sankoff(const plk_op* __restrict__ ops, int nops, const uint32_t* __restrict__
 #pragma unroll
                     for (int i = 0; i < N; i++) out[i] += tp[i];
                 } else {
-                    const uint32_t* cp = part + ((size_t)ch.id * Ppad + p) * N;
                     uint32_t cv[N];
+                    bool hit = false;
 #pragma unroll
-                    for (int j = 0; j < N; j++) cv[j] = cp[j];
+                    for (int kk = 0; kk < KC; kk++)
+                        if (cslot[kk] == ch.id) {
+                            hit = true;
+#pragma unroll
+                            for (int j = 0; j < N; j++) cv[j] = cache[kk][j];
+                        }
+                    if (!hit) {
+                        const uint32_t* cp = part + ((size_t)ch.id * Ppad + p) * N;
+#pragma unroll
+                        for (int j = 0; j < N; j++) cv[j] = cp[j];
+                    }
 #pragma unroll kUnrollI
                     for (int i = 0; i < N; i++) {
                         uint32_t m = cv[0] + cost[i * N];
@@ -360,6 +379,18 @@ k_sankoff(const plk_op* __restrict__ ops, int nops, const uint32_t* __restrict__
             uint32_t* dp = part + ((size_t)op.dad_slot * Ppad + p) * N;
 #pragma unroll
             for (int i = 0; i < N; i++) dp[i] = out[i];
+            // a slot written again replaces its cached copy
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++)
+                if (cslot[kk] == op.dad_slot) cslot[kk] = -1;
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++)
+                if (kk == cnext) {
+                    cslot[kk] = op.dad_slot;
+#pragma unroll
+                    for (int i = 0; i < N; i++) cache[kk][i] = out[i];
+                }
+            cnext = cnext + 1 == KC ? 0 : cnext + 1;
         }
         const uint32_t* nb = part + ((size_t)node.id * Ppad + p) * N;
         uint32_t best, br;
